@@ -1,0 +1,219 @@
+/*
+ * winding_amd.h -- C ABI of the MI355X splat projection + stable depth sort.
+ *
+ * One library, libwinding_amd.so, holds the gfx950 kernels and everything
+ * that owns device memory. The JavaScript module (winding_amd/js) reaches it
+ * through a thin N-API addon (winding_amd.node); Python reaches it through
+ * ctypes. Every entry point takes plain pointers and sizes, returns WD_OK (0)
+ * or a negative WD_E* status, and leaves a message for wd_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the Winding v1.9.0 tree):
+ *   new Splats(rhi, data) / destroy()        src/render/splats.js:389-416
+ *   storageCapacity(rhi, bytesPerItem)       src/rhi/buffer.js:32-36
+ *   SplatPass._stateFor / destroyState       src/render/splats.js:629-674,735-737
+ *   SplatPass.prepare (per-cloud uniform)    src/render/splats.js:546-607
+ *   _encodeSort: makeKeys, scanCounts, place src/render/splats.js:155-219,676-697
+ *   Cloud uniform layout                     src/render/splats.js:80-92
+ *
+ * Conventions (all from the reference):
+ *   - matrices are column-major, m[col*4 + row]   (src/core/math/mat4.js:1-12)
+ *   - a .splat record is 32 bytes: pos f32x3, scale f32x3, rgba u8x4,
+ *     quaternion u8x4 stored w,x,y,z as (q*128+128)  (src/scene/splats.js:373-388)
+ *   - key = 65535 - floor(clamp((-view.z - depthMin) * depthScale, 0, 65535));
+ *     ascending key is back to front            (src/render/splats.js:179-181)
+ *
+ * What differs, on purpose: the order within a key is stable (splat index
+ * ascending), where the reference's comes out of atomics
+ * (src/render/splats.js:22-25), and every cull and key decision equals the
+ * FP64 evaluation of the reference's formulas on the f32 inputs.
+ */
+#ifndef WINDING_AMD_H
+#define WINDING_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WD_OK 0
+#define WD_E_ARG -1      /* a bad argument: null, wrong size, out of range */
+#define WD_E_HIP -2      /* the HIP runtime failed */
+#define WD_E_CAPACITY -3 /* more than a buffer or a sort holds */
+#define WD_E_DATA -4     /* input refused by content (a NaN position, ...) */
+#define WD_E_STATE -5    /* wrong object state: destroyed, mismatched, ... */
+#define WD_E_FALLBACK -6 /* not an error: a learned sort ran the exact path */
+
+/* Bytes in one .splat record. src/scene/splats.js:41 */
+#define WD_RECORD_BYTES 32
+/* Depth buckets; keys are 16 bits. src/render/splats.js:64 */
+#define WD_BUCKETS 65536
+/* Default cull, opacity x pixels. src/render/splats.js:78 */
+#define WD_SPLAT_CULL 0.5f
+/* Most splats one sort takes: the stable scatter packs an index in 24 bits. */
+#define WD_MAX_SORT (1u << 24)
+/* Coarse depth bins the learned predictor works in: the key's high byte. */
+#define WD_COARSE_BINS 256
+/* Pose features the predictor reads: the model-view's top three rows. */
+#define WD_POSE_FLOATS 12
+
+typedef struct wd_context wd_context;
+typedef struct wd_cloud wd_cloud;
+typedef struct wd_state wd_state;
+typedef struct wd_batch wd_batch;
+typedef struct wd_mlp wd_mlp;
+
+/*
+ * What one sort depends on: the first 37 floats of the reference's Cloud
+ * uniform (src/render/splats.js:80-92, written at :574-587), all f32 as there.
+ */
+typedef struct wd_view {
+  float model_view[16]; /* view * world, column-major */
+  float projection[16]; /* column-major; [15] == 0 means perspective */
+  float viewport[2];    /* width, height in pixels */
+  float depth_min;      /* nearest view depth the cloud's box reaches, >= 0 */
+  float depth_scale;    /* buckets per unit depth: 65535 / max(far - near, 1e-6) */
+  float cull;           /* opacity x pixels a splat must add; 0 keeps all */
+} wd_view;
+
+/* Counters a sort leaves, read back with wd_state_stats. */
+typedef struct wd_stats {
+  uint32_t count;         /* survivors V: the reference's DrawArgs.instanceCount */
+  uint32_t recomputed;    /* splats whose area cull was decided in FP64 */
+  uint32_t fallback;      /* learned sorts that ran the exact path (0 or 1) */
+  uint32_t overflow_bins; /* coarse bins whose prediction was too small */
+} wd_stats;
+
+/* ---- errors, devices ------------------------------------------------- */
+
+/* The last failure's message on this thread ("" when none). */
+const char *wd_last_error(void);
+
+/* Opens device `device` (HIP ordinal) with one stream of default priority. */
+int wd_open(int device, wd_context **out);
+int wd_close(wd_context *ctx);
+/* Waits for everything enqueued on the context's stream. */
+int wd_sync(wd_context *ctx);
+/* The hipStream_t the context enqueues on, for callers that time or chain. */
+int wd_stream(wd_context *ctx, void **stream);
+
+/*
+ * How many items of `bytes_per_item` one allocation may hold on this device.
+ * Replaces storageCapacity (src/rhi/buffer.js:32-36): the limit is the
+ * device's free memory at open, capped at WD_MAX_SORT for sortable items.
+ */
+int wd_capacity(wd_context *ctx, uint64_t bytes_per_item, uint64_t *out);
+
+/* ---- clouds: new Splats(rhi, data), src/render/splats.js:389-416 ------ */
+
+/*
+ * Uploads `bytes` of .splat records (host memory) and checks them on the GPU:
+ * bytes % 32 == 0 and > 0 (src/scene/splats.js:374-376), every position
+ * finite (src/scene/splats.js:336-338). Fills min/max of the centres
+ * (src/scene/splats.js:343-348). Refuses more than WD_MAX_SORT splats by name.
+ */
+int wd_cloud_create(wd_context *ctx, const void *records, uint64_t bytes,
+                    wd_cloud **out, float min_out[3], float max_out[3]);
+/* As wd_cloud_create, from records already in device memory (copied). */
+int wd_cloud_create_device(wd_context *ctx, const void *device_records,
+                           uint64_t bytes, wd_cloud **out, float min_out[3],
+                           float max_out[3]);
+int wd_cloud_info(const wd_cloud *cloud, uint64_t *count, float min_out[3],
+                  float max_out[3]);
+/* Device address of the records (32 B each), for callers that read them. */
+int wd_cloud_records(const wd_cloud *cloud, const void **device_records);
+int wd_cloud_destroy(wd_cloud *cloud);
+
+/* ---- the per-view uniform: SplatPass.prepare, src/render/splats.js:559-587 */
+
+/*
+ * modelView = view * world (mat4Multiply, src/core/math/mat4.js:96-126, in
+ * f64 rounded to f32); depth range over the box's eight corners; depthMin =
+ * max(near, 0); depthScale = 65535 / max(far - near, 1e-6). Returns 1 (and
+ * leaves *out untouched) when far <= 0: the cloud is wholly behind the
+ * camera and the reference skips it (src/render/splats.js:572).
+ * world may be null for identity.
+ */
+int wd_view_prepare(const float view[16], const float world[16],
+                    const float projection[16], float width, float height,
+                    const float min[3], const float max[3], float cull,
+                    wd_view *out);
+
+/* ---- sort state: SplatPass._stateFor, src/render/splats.js:629-674 ---- */
+
+/* Buffers for sorting clouds of up to `capacity` splats. */
+int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out);
+int wd_state_destroy(wd_state *state);
+/*
+ * Device addresses of the outputs:
+ *   order  u32 x capacity : splat indices, back to front, count valid
+ *   count  u32            : survivors V
+ *   cov2d  f32x4 x capacity indexed by splat: (xx, xy, yy) in px^2 before the
+ *          draw's +0.3 (src/render/splats.js:95-96), and the rgba8 bits in .w
+ *          (src/scene/splats.js:342); written for survivors only.
+ */
+int wd_state_buffers(wd_state *state, void **order, void **count, void **cov2d);
+
+/*
+ * Projects, culls, keys and stably sorts `cloud` for `view` into `state`
+ * (makeKeys -> scanCounts -> place, src/render/splats.js:155-219). Enqueued
+ * on the context's stream; returns before the GPU finishes.
+ */
+int wd_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view *view,
+            wd_state *state);
+
+/* Waits, then copies the results to host memory. Any pointer may be null. */
+int wd_state_read(wd_context *ctx, wd_state *state, uint32_t *order,
+                  uint64_t order_capacity, float *cov2d, uint64_t cov2d_capacity,
+                  wd_stats *stats);
+int wd_state_stats(wd_context *ctx, wd_state *state, wd_stats *stats);
+
+/*
+ * Copies the 16-bit keys of every splat and the survivor bitmask (bit i of
+ * word i/64) of the last wd_sort: the parity tests compare them per splat.
+ */
+int wd_state_read_keys(wd_context *ctx, wd_state *state, uint16_t *keys,
+                       uint64_t *mask, uint64_t n);
+
+/* ---- timing: HIP events around each kernel on the context's stream ---- */
+
+/* Turns per-kernel event timing on (1) or off (0); clears the totals. */
+int wd_profile(wd_context *ctx, int on);
+/*
+ * Per-kernel totals since wd_profile(ctx, 1): names[k] (static strings),
+ * summed milliseconds and launches. Returns the number of kernels in *n.
+ */
+int wd_profile_read(wd_context *ctx, const char **names, double *ms,
+                    uint64_t *launches, int capacity, int *n);
+
+/* ---- synthetic inputs: the workloads' seeded generator ---------------- */
+
+/*
+ * Writes n .splat records from the seeded generator. kind 0: three-component
+ * Gaussian mixture; kind 1: 80% slab |z| <= 0.25 plus 20% uniform in a 40^3
+ * box. Host memory; deterministic for (kind, seed, n).
+ */
+int wd_synth_records(int kind, uint64_t seed, uint64_t n, void *records_out);
+
+/* ---- checks of the exact path (used by the parity tests) ------------- */
+
+/*
+ * The FP64 definition on the GPU for every splat, no f32 fast path: per splat
+ * five doubles (passes frustum and alpha 0/1, alpha*area, key, 0, 0). Needs
+ * the reverse-Z perspective and an affine model-view.
+ */
+int wd_debug_canonical(wd_context *ctx, const wd_cloud *cloud, const wd_view *view,
+                       double *out5);
+/*
+ * Runs the DPP wave scan on known values and f64 sqrt(a[2i]) and
+ * a[2i]/a[2i+1] against host-rounded expect[2i], expect[2i+1]; counts
+ * mismatches of each into mismatch3.
+ */
+int wd_debug_selftest(wd_context *ctx, const double *args, const double *expect,
+                      uint32_t count, uint32_t mismatch3[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WINDING_AMD_H */

@@ -1,0 +1,299 @@
+/*
+ * splat_oracle.c -- CPU restatement of Winding's splat projection, key and
+ * stable counting sort, in FP64. TEST INFRASTRUCTURE ONLY: the parity tests,
+ * the smoke check and the benchmark's CPU-baseline leg load it as the judge.
+ * The product (libwinding_amd.so) never links, calls or falls back to it.
+ *
+ * Canonical definition (the reference's formulas, decided in FP64):
+ *   Every quantity is computed in double from the f32 inputs (record floats,
+ *   record bytes, the f32 uniform) with the reference's own expression order,
+ *   no fused multiply-add (built with -ffp-contract=off), IEEE division and
+ *   square root. Constants are the WGSL f32 literals widened: 1.3f, 0.3f,
+ *   6.2831853f. The reference runs these formulas in f32 on whatever WebGPU
+ *   stack it meets (src/render/splats.js:155-186), so its exact bits are not
+ *   fixed anywhere; this file fixes them.
+ *
+ * Functions and what each follows (Winding v1.9.0 paths):
+ *   or_sigma      put(): quaternion made unit, R, M = R S, Sigma = M M^T
+ *                 src/scene/splats.js:350-368; length = sqrt(sum of squares)
+ *                 as hypot3 takes it (src/core/math/vec3.js:92-94), exact for
+ *                 byte quaternions, where put's Math.hypot may differ by an
+ *                 ulp of f64 (never visible after the f32 rounding put does).
+ *   or_footprint  footprint(): J W Sigma W^T J^T, src/render/splats.js:102-125
+ *   or_splat      makeKeys() cull and key, src/render/splats.js:155-186
+ *   or_sort       stable one-pass counting sort over 65536 buckets, in the
+ *                 idiom of DrawList.sort (src/render/drawlist.js:158-203);
+ *                 ties keep splat index order (the reference's place() uses
+ *                 atomics and does not, src/render/splats.js:214-219)
+ *   or_bounds     put()'s finite check and min/max, src/scene/splats.js:336-348
+ *   or_prepare    SplatPass.prepare's per-cloud uniform,
+ *                 src/render/splats.js:559-587, with mat4Multiply
+ *                 src/core/math/mat4.js:96-126
+ *   or_look_at, or_perspective  mat4LookAt / mat4PerspectiveReverseZInfinite
+ *                 src/core/math/mat4.js:243-273,455-468
+ *
+ * Pinned by: tests/test_oracle.py (known answers from test/splats.test.js:51-71,
+ * test/core.test.js:214-223,343-360, test/render.test.js:428-440 and
+ * test/gpu.test.js:1804-1962 restated as orders; closed forms; brute force).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct or_view {
+  float model_view[16];
+  float projection[16];
+  float viewport[2];
+  float depth_min;
+  float depth_scale;
+  float cull;
+} or_view;
+
+#define OR_BUCKETS 65536
+
+static const double C13 = (double)1.3f;
+static const double C03 = (double)0.3f;
+static const double C2PI = (double)6.2831853f;
+
+static float rd_f32(const uint8_t *p) {
+  float f;
+  memcpy(&f, p, 4);
+  return f;
+}
+
+/* put(): src/scene/splats.js:350-368. Sigma as xx, xy, xz, yy, yz, zz. */
+void or_sigma(const uint8_t *rec, double sigma[6]) {
+  double sx = rd_f32(rec + 12), sy = rd_f32(rec + 16), sz = rd_f32(rec + 20);
+  double qw = (rec[28] - 128) / 128.0, qx = (rec[29] - 128) / 128.0;
+  double qy = (rec[30] - 128) / 128.0, qz = (rec[31] - 128) / 128.0;
+  double length = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+  if (length == 0) length = 1;
+  double w = qw / length, u = qx / length, v = qy / length, t = qz / length;
+  double r00 = 1 - 2 * (v * v + t * t), r01 = 2 * (u * v - w * t), r02 = 2 * (u * t + w * v);
+  double r10 = 2 * (u * v + w * t), r11 = 1 - 2 * (u * u + t * t), r12 = 2 * (v * t - w * u);
+  double r20 = 2 * (u * t - w * v), r21 = 2 * (v * t + w * u), r22 = 1 - 2 * (u * u + v * v);
+  double m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
+  double m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
+  double m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
+  sigma[0] = m00 * m00 + m01 * m01 + m02 * m02;
+  sigma[1] = m00 * m10 + m01 * m11 + m02 * m12;
+  sigma[2] = m00 * m20 + m01 * m21 + m02 * m22;
+  sigma[3] = m10 * m10 + m11 * m11 + m12 * m12;
+  sigma[4] = m10 * m20 + m11 * m21 + m12 * m22;
+  sigma[5] = m20 * m20 + m21 * m21 + m22 * m22;
+}
+
+/* view = modelView * (p, 1), columns summed left to right (WGSL mat * vec). */
+static void view_of(const or_view *v, double x, double y, double z, double out[4]) {
+  const float *m = v->model_view;
+  for (int r = 0; r < 4; r++) out[r] = (double)m[r] * x + (double)m[4 + r] * y + (double)m[8 + r] * z + (double)m[12 + r];
+}
+
+/*
+ * footprint(): src/render/splats.js:102-125. T = J W (rows 0 and 1; J's third
+ * row is zero), cov = (T Sigma) T^T.
+ */
+void or_footprint(const or_view *v, const double view[4], const double sg[6], double cov[3]) {
+  const float *p = v->projection, *m = v->model_view;
+  double p00 = p[0], p11 = p[5];
+  double fx = p00 * (double)v->viewport[0] * 0.5;
+  double fy = p11 * (double)v->viewport[1] * 0.5;
+  double j00 = fx, j11 = fy, j02 = 0, j12 = 0;
+  if (p[15] == 0.0f) {
+    double d = -view[2];
+    double lx = C13 / p00, ly = C13 / p11;
+    double cx = view[0] / d, cy = view[1] / d;
+    if (cx < -lx) cx = -lx;
+    if (cx > lx) cx = lx;
+    if (cy < -ly) cy = -ly;
+    if (cy > ly) cy = ly;
+    double x = cx * d, y = cy * d;
+    j00 = fx / d;
+    j11 = fy / d;
+    j02 = fx * x / (d * d);
+    j12 = fy * y / (d * d);
+  }
+  /* W[k][c] = modelView[c*4 + k] */
+  double t0[3], t1[3];
+  for (int c = 0; c < 3; c++) {
+    t0[c] = j00 * (double)m[c * 4 + 0] + j02 * (double)m[c * 4 + 2];
+    t1[c] = j11 * (double)m[c * 4 + 1] + j12 * (double)m[c * 4 + 2];
+  }
+  double S[3][3] = {{sg[0], sg[1], sg[2]}, {sg[1], sg[3], sg[4]}, {sg[2], sg[4], sg[5]}};
+  double u0[3], u1[3];
+  for (int c = 0; c < 3; c++) {
+    u0[c] = t0[0] * S[0][c] + t0[1] * S[1][c] + t0[2] * S[2][c];
+    u1[c] = t1[0] * S[0][c] + t1[1] * S[1][c] + t1[2] * S[2][c];
+  }
+  cov[0] = u0[0] * t0[0] + u0[1] * t0[1] + u0[2] * t0[2];
+  cov[1] = u0[0] * t1[0] + u0[1] * t1[1] + u0[2] * t1[2];
+  cov[2] = u1[0] * t1[0] + u1[1] * t1[1] + u1[2] * t1[2];
+}
+
+/*
+ * makeKeys() for one record: src/render/splats.js:157-185. Returns 1 for a
+ * survivor and writes its key; cov (may be null) gets the FP64 footprint
+ * whenever the splat passes the frustum and alpha tests; *value (may be null)
+ * gets alpha * area when the area test ran, else NAN.
+ */
+int or_splat(const uint8_t *rec, const or_view *v, uint16_t *key, double cov[3], double *value) {
+  double px = rd_f32(rec), py = rd_f32(rec + 4), pz = rd_f32(rec + 8);
+  double view[4], clip[4];
+  view_of(v, px, py, pz, view);
+  const float *P = v->projection;
+  for (int r = 0; r < 4; r++)
+    clip[r] = (double)P[r] * view[0] + (double)P[4 + r] * view[1] + (double)P[8 + r] * view[2] + (double)P[12 + r] * view[3];
+  double w = clip[3];
+  if (value) *value = NAN;
+  if (view[2] >= 0 || clip[2] > w || fabs(clip[0]) > C13 * w || fabs(clip[1]) > C13 * w) return 0;
+  unsigned a = rec[27];
+  if (a <= 1) return 0;
+  double f[3];
+  double sg[6];
+  or_sigma(rec, sg);
+  or_footprint(v, view, sg, f);
+  if (cov) memcpy(cov, f, sizeof f);
+  double cull = v->cull;
+  if (cull > 0) {
+    double det = (f[0] + C03) * (f[2] + C03) - f[1] * f[1];
+    double area = C2PI * sqrt(det > 0 ? det : 0);
+    double alpha = a / 255.0;
+    if (value) *value = alpha * area;
+    if (alpha * area < cull) return 0;
+  }
+  double t = (-view[2] - (double)v->depth_min) * (double)v->depth_scale;
+  if (t < 0) t = 0;
+  if (t > 65535.0) t = 65535.0;
+  uint32_t bucket = (uint32_t)t;
+  *key = (uint16_t)(65535u - bucket);
+  return 1;
+}
+
+/* Every splat of a cloud: keys (culled get 0), keep flags, FP64 cov (n x 3, may be null), value (may be null). */
+void or_keys(const uint8_t *records, uint64_t n, const or_view *v, uint16_t *keys, uint8_t *keep,
+             double *cov, double *value) {
+  for (uint64_t i = 0; i < n; i++) {
+    uint16_t k = 0;
+    double c[3] = {NAN, NAN, NAN}, val;
+    int s = or_splat(records + i * 32, v, &k, c, &val);
+    keys[i] = s ? k : 0;
+    keep[i] = (uint8_t)s;
+    if (cov) memcpy(cov + 3 * i, c, sizeof c);
+    if (value) value[i] = val;
+  }
+}
+
+/*
+ * Stable counting sort of the survivors by key, ties by index: one pass,
+ * since the key fits 65536 buckets. Returns the survivor count.
+ */
+uint32_t or_sort(const uint16_t *keys, const uint8_t *keep, uint64_t n, uint32_t *order) {
+  uint32_t *counts = (uint32_t *)calloc(OR_BUCKETS, sizeof(uint32_t));
+  if (!counts) return 0xffffffffu;
+  for (uint64_t i = 0; i < n; i++)
+    if (keep[i]) counts[keys[i]]++;
+  uint32_t total = 0;
+  for (uint32_t b = 0; b < OR_BUCKETS; b++) {
+    uint32_t c = counts[b];
+    counts[b] = total;
+    total += c;
+  }
+  for (uint64_t i = 0; i < n; i++)
+    if (keep[i]) order[counts[keys[i]]++] = (uint32_t)i;
+  free(counts);
+  return total;
+}
+
+/* The whole path for one view: returns V; order must hold n. cov n x 3 (may be null). */
+uint32_t or_run(const uint8_t *records, uint64_t n, const or_view *v, uint32_t *order, uint16_t *keys,
+                uint8_t *keep, double *cov) {
+  or_keys(records, n, v, keys, keep, cov, NULL);
+  return or_sort(keys, keep, n, order);
+}
+
+/* Finite check and bounds, as put(): returns the first bad index, or -1. */
+int64_t or_bounds(const uint8_t *records, uint64_t n, float mn[3], float mx[3]) {
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint64_t i = 0; i < n; i++) {
+    double p[3];
+    for (int k = 0; k < 3; k++) p[k] = rd_f32(records + i * 32 + 4 * k);
+    if (!(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]))) return (int64_t)i;
+    for (int k = 0; k < 3; k++) {
+      if (p[k] < lo[k]) lo[k] = p[k];
+      if (p[k] > hi[k]) hi[k] = p[k];
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    mn[k] = (float)lo[k];
+    mx[k] = (float)hi[k];
+  }
+  return -1;
+}
+
+/* mat4Multiply(out, a, b): out = a * b, each entry summed in f64 and stored as f32. */
+void or_mat4_multiply(float out[16], const float a[16], const float b[16]) {
+  float r[16];
+  for (int c = 0; c < 4; c++) {
+    double b0 = b[c * 4], b1 = b[c * 4 + 1], b2 = b[c * 4 + 2], b3 = b[c * 4 + 3];
+    for (int row = 0; row < 4; row++)
+      r[c * 4 + row] = (float)(b0 * a[row] + b1 * a[4 + row] + b2 * a[8 + row] + b3 * a[12 + row]);
+  }
+  memcpy(out, r, sizeof r);
+}
+
+/* SplatPass.prepare for one cloud. Returns 1 when the cloud is wholly behind (far <= 0). */
+int or_prepare(const float view[16], const float world[16], const float proj[16], float width, float height,
+               const float mn[3], const float mx[3], float cull, or_view *out) {
+  static const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  float mv[16];
+  or_mat4_multiply(mv, view, world ? world : identity);
+  double near = INFINITY, far = -INFINITY;
+  for (int k = 0; k < 8; k++) {
+    double x = (k & 1) ? mx[0] : mn[0];
+    double y = (k & 2) ? mx[1] : mn[1];
+    double z = (k & 4) ? mx[2] : mn[2];
+    double depth = -((double)mv[2] * x + (double)mv[6] * y + (double)mv[10] * z + (double)mv[14]);
+    if (depth < near) near = depth;
+    if (depth > far) far = depth;
+  }
+  if (far <= 0) return 1;
+  near = near > 0 ? near : 0;
+  memcpy(out->model_view, mv, sizeof mv);
+  memcpy(out->projection, proj, 16 * sizeof(float));
+  out->viewport[0] = width;
+  out->viewport[1] = height;
+  out->depth_min = (float)near;
+  double span = far - near;
+  out->depth_scale = (float)(65535.0 / (span > 1e-6 ? span : 1e-6));
+  out->cull = cull;
+  return 0;
+}
+
+/* mat4LookAt: src/core/math/mat4.js:243-273 (hypot3 = sqrt of the sum of squares). */
+void or_look_at(float out[16], const double eye[3], const double center[3], const double up[3]) {
+  double zx = eye[0] - center[0], zy = eye[1] - center[1], zz = eye[2] - center[2];
+  double zl = sqrt(zx * zx + zy * zy + zz * zz);
+  zx /= zl; zy /= zl; zz /= zl;
+  double xx = up[1] * zz - up[2] * zy, xy = up[2] * zx - up[0] * zz, xz = up[0] * zy - up[1] * zx;
+  double xl = sqrt(xx * xx + xy * xy + xz * xz);
+  xx /= xl; xy /= xl; xz /= xl;
+  double yx = zy * xz - zz * xy, yy = zz * xx - zx * xz, yz = zx * xy - zy * xx;
+  out[0] = (float)xx; out[1] = (float)yx; out[2] = (float)zx; out[3] = 0;
+  out[4] = (float)xy; out[5] = (float)yy; out[6] = (float)zy; out[7] = 0;
+  out[8] = (float)xz; out[9] = (float)yz; out[10] = (float)zz; out[11] = 0;
+  out[12] = (float)(-(xx * eye[0] + xy * eye[1] + xz * eye[2]));
+  out[13] = (float)(-(yx * eye[0] + yy * eye[1] + yz * eye[2]));
+  out[14] = (float)(-(zx * eye[0] + zy * eye[1] + zz * eye[2]));
+  out[15] = 1;
+}
+
+/* mat4PerspectiveReverseZInfinite: src/core/math/mat4.js:455-468. */
+void or_perspective(float out[16], double fovy, double aspect, double near) {
+  double f = 1 / tan(fovy * 0.5);
+  memset(out, 0, 16 * sizeof(float));
+  out[0] = (float)(f / aspect);
+  out[5] = (float)f;
+  out[11] = -1;
+  out[14] = (float)near;
+}

@@ -1,0 +1,229 @@
+"""Parity of the HIP path with the CPU oracle, through the C ABI, on an MI355X.
+
+The bar: keys, survivor flags and the draw order bit-exact; cov2d within 1e-5 of the FP64 oracle,
+relative to max(|xx|, |yy|) per splat (the xy term can be near zero); the FP64 fallback bit-exact.
+"""
+import math
+import struct
+
+import numpy as np
+import pytest
+
+import oracle
+import winding_amd as wa
+
+pytestmark = pytest.mark.gpu
+
+COV_TOL = 1e-5
+
+
+def record(pos, scale, rgba, quat):
+    return np.frombuffer(struct.pack("<6f4B4B", *pos, *scale, *rgba, *quat), np.uint8).copy()
+
+
+def run_gpu(ctx, recs, view, state=None):
+    cloud = wa.Cloud(ctx, recs)
+    own = state is None
+    st = state or wa.State(ctx, len(recs))
+    try:
+        st.sort(cloud, view)
+        order, cov, stats = st.read(n_cov=len(recs))
+        keys, keep = st.read_keys(len(recs))
+    finally:
+        cloud.destroy()
+        if own:
+            st.destroy()
+    return order, keys, keep, cov, stats
+
+
+def cov_error(cov_gpu, cov64, keep):
+    g = cov_gpu[keep, :3].astype(np.float64)
+    w = cov64[keep]
+    scale = np.maximum(np.abs(w[:, 0]), np.abs(w[:, 2]))
+    return float(np.max(np.abs(g - w).max(axis=1) / scale)) if len(g) else 0.0
+
+
+def assert_parity(ctx, recs, view, label=""):
+    order, keys, keep, cov, stats = run_gpu(ctx, recs, view)
+    o_order, o_keys, o_keep, o_cov = oracle.run(recs, view)
+    print("[%s] n=%d V=%d recomputed=%d" % (label, len(recs), stats.count, stats.recomputed), flush=True)
+    assert np.array_equal(keep, o_keep), "survivor flags differ at %s" % np.nonzero(keep != o_keep)[0][:10]
+    assert np.array_equal(keys[keep], o_keys[o_keep]), "keys differ"
+    assert stats.count == len(o_order)
+    assert np.array_equal(order, o_order), "order differs first at %d" % int(np.argmax(order != o_order))
+    err = cov_error(cov, o_cov, keep)
+    assert err <= COV_TOL, "cov2d relative error %g" % err
+    colours = cov[keep, 3].view(np.uint32)
+    assert np.array_equal(colours, recs[keep, 24:28].copy().view(np.uint32).ravel())
+    return stats, err
+
+
+def test_selftest_dpp_scan_and_f64_rounding(ctx):
+    rng = np.random.default_rng(5)
+    # every sum of squares a byte quaternion can give (sqrt of S/16384), then random magnitudes
+    a = np.concatenate([np.arange(0, 65537) / 16384.0, np.exp(rng.uniform(-40, 40, 100000))])
+    b = np.concatenate([np.full(65537, 255.0), np.exp(rng.uniform(-40, 40, 100000))])
+    args = np.stack([a, b], axis=1).ravel()
+    expect = np.stack([np.sqrt(a), a / b], axis=1).ravel()
+    mm = ctx.selftest(args, expect)
+    assert list(mm) == [0, 0, 0], mm
+
+
+@pytest.mark.parametrize("cull", [0.5, 0.0, 2.0])
+def test_plain_64k_parity(ctx, cull):
+    recs, view, _ = wa.workloads.plain("plain-64k", cull=cull)
+    stats, err = assert_parity(ctx, recs, view, "plain-64k cull=%g" % cull)
+    print("cov2d max rel err %.3g" % err)
+    assert stats.recomputed < len(recs) // 1000
+
+
+def test_fp64_definition_on_gpu_is_bit_exact(ctx):
+    recs, view, _ = wa.workloads.plain("plain-64k", n=20000, cull=0.5)
+    cloud = wa.Cloud(ctx, recs)
+    try:
+        got = cloud.canonical(view)
+    finally:
+        cloud.destroy()
+    k, keep, cov, val = oracle.keys(recs, view)
+    passed = got[:, 0] == 1
+    tested = ~np.isnan(val)
+    assert np.array_equal(passed, tested)
+    assert np.array_equal(got[passed, 1].view(np.uint64), val[passed].view(np.uint64))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 255, 257, 4095, 4097, 12289, 70001])
+def test_ragged_sizes(ctx, n):
+    recs, view, _ = wa.workloads.plain("plain-64k", n=n)
+    assert_parity(ctx, recs, view, "n=%d" % n)
+
+
+def test_back_to_front_known_answer(ctx):
+    # test/gpu.test.js:1804-1870 restated as orders.
+    red = record([0, 0, 1], [0.4] * 3, [255, 0, 0, 255], [255, 128, 128, 128])
+    blue = record([0, 0, -1], [0.4] * 3, [0, 0, 255, 255], [255, 128, 128, 128])
+    for recs, ri, bi in [(np.stack([red, blue]), 0, 1), (np.stack([blue, red]), 1, 0)]:
+        for eye, want in [((0, 0, 5), [bi, ri]), ((0, 0, -5), [ri, bi])]:
+            view = wa.workloads.view_for(recs, eye, 160, 120)
+            p = wa.perspective(0.8, 160 / 120, 0.1)
+            view.projection[:] = list(p)
+            order, _, _, _, _ = run_gpu(ctx, recs, view)
+            assert list(order) == want
+
+
+def test_faint_speck_cull(ctx):
+    # test/gpu.test.js:1914-1962
+    speck = record([0, 0, 0], [0.0005] * 3, [255, 255, 255, 50], [255, 128, 128, 128])
+    big = record([1, 0, 0], [0.2] * 3, [255, 255, 255, 255], [255, 128, 128, 128])
+    recs = np.stack([speck, big])
+    for cull, want in [(0.5, [1]), (0.0, [0, 1])]:
+        mn, mx = wa.workloads.bounds_of(recs)
+        view = wa.prepare_view(wa.look_at((0, 0, 5)), wa.perspective(0.8, 160 / 120, 0.1), 160.0, 120.0, mn, mx, cull)
+        order, _, _, _, _ = run_gpu(ctx, recs, view)
+        assert sorted(order) == want
+
+
+def test_all_culled_and_all_one_key(ctx):
+    recs, view, _ = wa.workloads.plain("plain-64k", n=5000)
+    behind = wa.workloads.view_for(recs, (0, 0, 10), 1280, 720)
+    behind.model_view[14] += 100.0  # push everything behind the camera
+    order, keys, keep, cov, stats = run_gpu(ctx, recs, behind)
+    assert stats.count == 0 and not keep.any()
+    # Every splat at one depth: one bucket, so the order is the survivors in index order.
+    flat = recs.copy()
+    pos = flat.view(np.float32).reshape(-1, 8)
+    pos[:, 2] = 0.0
+    view = wa.workloads.view_for(flat, (0, 0, 10), 1280, 720)
+    assert_parity(ctx, flat, view, "one depth")
+
+
+def test_near_edges_and_thresholds(ctx):
+    # Splats placed on bucket edges and at the cull threshold: the f32 fast path must hand them to FP64.
+    recs, view, _ = wa.workloads.plain("plain-64k", n=20000, cull=0.5)
+    pos = recs.view(np.float32).reshape(-1, 8)
+    mv = np.array(view.model_view, np.float64).reshape(4, 4).T
+    rng = np.random.default_rng(9)
+    for i in range(0, len(recs), 3):  # move to the nearest bucket edge along the view axis
+        p = pos[i, :3].astype(np.float64)
+        d = -(mv[2, :3] @ p + mv[2, 3])
+        t = (d - view.depth_min) * view.depth_scale
+        edge = round(t) + rng.choice([-1e-4, 0, 1e-4])
+        dd = edge / view.depth_scale + view.depth_min
+        pos[i, :3] = (p + (d - dd) * mv[2, :3]).astype(np.float32)
+    # rescale some splats so alpha * area sits on the cull threshold
+    k, keep, cov, val = oracle.keys(recs, view)
+    for i in np.nonzero(~np.isnan(val))[0][1::4]:
+        pos[i, 3:6] *= np.float32(min(4.0, max(0.25, math.sqrt(0.5 / max(val[i], 1e-9)))))
+    stats, _ = assert_parity(ctx, recs, view, "edges")
+
+
+def test_orthographic_and_jittered_projections(ctx):
+    recs, view, _ = wa.workloads.plain("plain-64k", n=30000, cull=0.5)
+    jit = wa.View.from_buffer_copy(bytes(view))
+    jit.projection[8] = 0.37 / 1280
+    jit.projection[9] = -0.21 / 720
+    assert_parity(ctx, recs, jit, "jittered")
+    ortho = wa.View.from_buffer_copy(bytes(view))
+    # mat4OrthographicReverseZ-like: x, y scaled, z mapped, w = 1 (the generic kernel path)
+    P = np.zeros(16, np.float32)
+    P[0], P[5], P[10], P[14], P[15] = 1 / 8.0, 1 / 4.5, 0.01, 0.5, 1.0
+    ortho.projection[:] = list(P)
+    assert_parity(ctx, recs, ortho, "orthographic")
+
+
+def test_sort_is_deterministic_and_state_reusable(ctx):
+    recs, view, _ = wa.workloads.plain("plain-64k", n=100000)
+    st = wa.State(ctx, 100000)
+    try:
+        a = run_gpu(ctx, recs, view, st)[0]
+        small, sview, _ = wa.workloads.plain("plain-64k", n=3000)
+        b = run_gpu(ctx, small, sview, st)[0]
+        c = run_gpu(ctx, recs, view, st)[0]
+    finally:
+        st.destroy()
+    assert np.array_equal(a, c)
+    assert np.array_equal(b, oracle.run(small, sview)[0])
+
+
+def test_plain_1m_parity(ctx):
+    recs, view, _ = wa.workloads.plain("plain-1m")
+    assert_parity(ctx, recs, view, "plain-1m")
+
+
+def test_slab_8m_parity_and_properties(ctx):
+    recs, view, spec = wa.workloads.plain("slab-8m")
+    order, keys, keep, cov, stats = run_gpu(ctx, recs, view)
+    # size-independent properties first: a permutation of the survivors, keys ascending, ties by index
+    assert stats.count == int(keep.sum()) and len(np.unique(order)) == len(order)
+    assert keep[order].all()
+    ko = keys[order].astype(np.int64)
+    assert np.all(np.diff(ko) >= 0)
+    same = np.diff(ko) == 0
+    assert np.all(np.diff(order.astype(np.int64))[same] > 0)
+    o_order, o_keys, o_keep, o_cov = oracle.run(recs, view)
+    assert np.array_equal(keep, o_keep) and np.array_equal(order, o_order)
+    assert cov_error(cov, o_cov, keep) <= COV_TOL
+    print("slab-8m V=%d recomputed=%d" % (stats.count, stats.recomputed), flush=True)
+
+
+def test_refusals(ctx):
+    with pytest.raises(wa.WindingError, match="not a whole number of 32-byte"):
+        wa.Cloud(ctx, np.zeros(33, np.uint8))
+    nan = record([float("nan"), 0, 0], [1, 1, 1], [0, 0, 0, 0], [255, 128, 128, 128])
+    ok = record([0, 0, 0], [1, 1, 1], [0, 0, 0, 0], [255, 128, 128, 128])
+    with pytest.raises(wa.WindingError, match="splat 1 has a position that is not a number"):
+        wa.Cloud(ctx, np.stack([ok, nan]))
+    recs, view, _ = wa.workloads.plain("plain-64k", n=100)
+    cloud = wa.Cloud(ctx, recs)
+    st = wa.State(ctx, 50)
+    with pytest.raises(wa.WindingError, match="more than this state holds"):
+        st.sort(cloud, view)
+    bad = wa.View.from_buffer_copy(bytes(view))
+    bad.cull = -1
+    st2 = wa.State(ctx, 100)
+    with pytest.raises(wa.WindingError, match="splatCull must be 0 or more"):
+        st2.sort(cloud, bad)
+    cloud.destroy()
+    st.destroy()
+    st2.destroy()
+    with pytest.raises(wa.WindingError, match="more than one sort holds"):
+        wa.State(ctx, wa.WD_MAX_SORT + 1)

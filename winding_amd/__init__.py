@@ -1,0 +1,282 @@
+"""Python binding of libwinding_amd.so (ctypes over include/winding_amd.h).
+
+The host API the parity tests and the benchmark drive; the JavaScript module
+under winding_amd/js drives the same library through the N-API addon. Loading
+fails loudly when the library is not built: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(ROOT, "libwinding_amd.so")
+
+WD_OK = 0
+WD_RECORD_BYTES = 32
+WD_BUCKETS = 65536
+WD_SPLAT_CULL = 0.5
+WD_MAX_SORT = 1 << 24
+WD_COARSE_BINS = 256
+WD_POSE_FLOATS = 12
+
+# Every symbol include/winding_amd.h declares, in header order.
+EXPORTS = [
+    "wd_last_error", "wd_open", "wd_close", "wd_sync", "wd_stream", "wd_capacity",
+    "wd_cloud_create", "wd_cloud_create_device", "wd_cloud_info", "wd_cloud_records", "wd_cloud_destroy",
+    "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
+    "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_profile", "wd_profile_read",
+    "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest",
+]
+
+
+class View(C.Structure):
+    """wd_view: the first 37 floats of the reference's Cloud uniform (src/render/splats.js:80-92)."""
+    _fields_ = [
+        ("model_view", C.c_float * 16),
+        ("projection", C.c_float * 16),
+        ("viewport", C.c_float * 2),
+        ("depth_min", C.c_float),
+        ("depth_scale", C.c_float),
+        ("cull", C.c_float),
+    ]
+
+    def as_array(self):
+        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+
+class Stats(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("recomputed", C.c_uint32), ("fallback", C.c_uint32),
+                ("overflow_bins", C.c_uint32)]
+
+
+class WindingError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it was not built (python -m winding_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("winding_amd: %s is not built; run python -m winding_amd.build" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        P, U64, U32, I = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+        FP = C.POINTER(C.c_float)
+        sig = {
+            "wd_last_error": (C.c_char_p, []),
+            "wd_open": (I, [I, C.POINTER(P)]),
+            "wd_close": (I, [P]),
+            "wd_sync": (I, [P]),
+            "wd_stream": (I, [P, C.POINTER(P)]),
+            "wd_capacity": (I, [P, U64, C.POINTER(U64)]),
+            "wd_cloud_create": (I, [P, P, U64, C.POINTER(P), FP, FP]),
+            "wd_cloud_create_device": (I, [P, P, U64, C.POINTER(P), FP, FP]),
+            "wd_cloud_info": (I, [P, C.POINTER(U64), FP, FP]),
+            "wd_cloud_records": (I, [P, C.POINTER(P)]),
+            "wd_cloud_destroy": (I, [P]),
+            "wd_view_prepare": (I, [FP, FP, FP, C.c_float, C.c_float, FP, FP, C.c_float, C.POINTER(View)]),
+            "wd_state_create": (I, [P, U64, C.POINTER(P)]),
+            "wd_state_destroy": (I, [P]),
+            "wd_state_buffers": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+            "wd_sort": (I, [P, P, C.POINTER(View), P]),
+            "wd_state_read": (I, [P, P, P, U64, P, U64, C.POINTER(Stats)]),
+            "wd_state_stats": (I, [P, P, C.POINTER(Stats)]),
+            "wd_state_read_keys": (I, [P, P, P, P, U64]),
+            "wd_profile": (I, [P, I]),
+            "wd_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(U64), I,
+                                    C.POINTER(I)]),
+            "wd_synth_records": (I, [I, U64, U64, P]),
+            "wd_debug_canonical": (I, [P, P, C.POINTER(View), P]),
+            "wd_debug_selftest": (I, [P, P, P, U32, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != WD_OK and rc != 1:
+        raise WindingError(rc, lib().wd_last_error().decode())
+    return rc
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def synth_records(kind, seed, n):
+    """n seeded .splat records (uint8 array n x 32): kind 0 Gaussian mixture, 1 slab."""
+    out = np.empty((n, 32), dtype=np.uint8)
+    check(lib().wd_synth_records(kind, seed, n, out.ctypes.data))
+    return out
+
+
+def prepare_view(view, projection, width, height, mn, mx, cull=WD_SPLAT_CULL, world=None):
+    """SplatPass.prepare's uniform for one cloud (src/render/splats.js:559-587); None when wholly behind."""
+    v = View()
+    view = np.ascontiguousarray(view, np.float32)
+    projection = np.ascontiguousarray(projection, np.float32)
+    mn = np.ascontiguousarray(mn, np.float32)
+    mx = np.ascontiguousarray(mx, np.float32)
+    w = None if world is None else _fp(np.ascontiguousarray(world, np.float32))
+    rc = check(lib().wd_view_prepare(_fp(view), w, _fp(projection), width, height, _fp(mn), _fp(mx), cull,
+                                     C.byref(v)))
+    return None if rc == 1 else v
+
+
+def look_at(eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)):
+    """mat4LookAt (src/core/math/mat4.js:243-273), f64 maths stored as f32."""
+    eye, target, up = (np.asarray(x, np.float64) for x in (eye, target, up))
+    z = eye - target
+    z = z / np.sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2])
+    x = np.array([up[1] * z[2] - up[2] * z[1], up[2] * z[0] - up[0] * z[2], up[0] * z[1] - up[1] * z[0]])
+    x = x / np.sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2])
+    y = np.array([z[1] * x[2] - z[2] * x[1], z[2] * x[0] - z[0] * x[2], z[0] * x[1] - z[1] * x[0]])
+    m = np.zeros(16)
+    m[0], m[1], m[2] = x[0], y[0], z[0]
+    m[4], m[5], m[6] = x[1], y[1], z[1]
+    m[8], m[9], m[10] = x[2], y[2], z[2]
+    m[12] = -(x[0] * eye[0] + x[1] * eye[1] + x[2] * eye[2])
+    m[13] = -(y[0] * eye[0] + y[1] * eye[1] + y[2] * eye[2])
+    m[14] = -(z[0] * eye[0] + z[1] * eye[1] + z[2] * eye[2])
+    m[15] = 1
+    return m.astype(np.float32)
+
+
+def perspective(fovy, aspect, near):
+    """mat4PerspectiveReverseZInfinite (src/core/math/mat4.js:455-468)."""
+    f = 1.0 / np.tan(fovy * 0.5)
+    m = np.zeros(16)
+    m[0] = f / aspect
+    m[5] = f
+    m[11] = -1
+    m[14] = near
+    return m.astype(np.float32)
+
+
+class Context:
+    """A device and its stream (wd_open)."""
+
+    def __init__(self, device=0):
+        p = C.c_void_p()
+        check(lib().wd_open(device, C.byref(p)))
+        self.ptr = p
+
+    def close(self):
+        if self.ptr:
+            check(lib().wd_close(self.ptr))
+            self.ptr = None
+
+    def sync(self):
+        check(lib().wd_sync(self.ptr))
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().wd_stream(self.ptr, C.byref(s)))
+        return s.value
+
+    def profile(self, on):
+        check(lib().wd_profile(self.ptr, 1 if on else 0))
+
+    def profile_read(self):
+        names = (C.c_char_p * 16)()
+        ms = (C.c_double * 16)()
+        launches = (C.c_uint64 * 16)()
+        n = C.c_int()
+        check(lib().wd_profile_read(self.ptr, names, ms, launches, 16, C.byref(n)))
+        return {names[i].decode(): (ms[i], launches[i]) for i in range(n.value) if launches[i]}
+
+    def selftest(self, args, expect):
+        args = np.ascontiguousarray(args, np.float64)
+        expect = np.ascontiguousarray(expect, np.float64)
+        mm = np.zeros(3, np.uint32)
+        check(lib().wd_debug_selftest(self.ptr, args.ctypes.data, expect.ctypes.data, len(args) // 2,
+                                      mm.ctypes.data))
+        return mm
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Cloud:
+    """Uploaded .splat records (new Splats(rhi, data), src/render/splats.js:389-409)."""
+
+    def __init__(self, ctx, records, device=False, nbytes=None):
+        p = C.c_void_p()
+        self.min = np.zeros(3, np.float32)
+        self.max = np.zeros(3, np.float32)
+        if device:
+            check(lib().wd_cloud_create_device(ctx.ptr, records, nbytes, C.byref(p), _fp(self.min), _fp(self.max)))
+            self.count = nbytes // 32
+        else:
+            records = np.ascontiguousarray(records, dtype=np.uint8)
+            check(lib().wd_cloud_create(ctx.ptr, records.ctypes.data, records.nbytes, C.byref(p), _fp(self.min),
+                                        _fp(self.max)))
+            self.count = records.nbytes // 32
+        self.ctx = ctx
+        self.ptr = p
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().wd_cloud_destroy(self.ptr))
+            self.ptr = None
+
+    def canonical(self, view):
+        out = np.zeros((self.count, 5), np.float64)
+        check(lib().wd_debug_canonical(self.ctx.ptr, self.ptr, C.byref(view), out.ctypes.data))
+        return out
+
+
+class State:
+    """Sort buffers for clouds up to `capacity` splats (SplatPass._stateFor, src/render/splats.js:629-674)."""
+
+    def __init__(self, ctx, capacity):
+        p = C.c_void_p()
+        check(lib().wd_state_create(ctx.ptr, capacity, C.byref(p)))
+        self.ctx = ctx
+        self.ptr = p
+        self.capacity = capacity
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().wd_state_destroy(self.ptr))
+            self.ptr = None
+
+    def sort(self, cloud, view):
+        check(lib().wd_sort(self.ctx.ptr, cloud.ptr, C.byref(view), self.ptr))
+
+    def stats(self):
+        s = Stats()
+        check(lib().wd_state_stats(self.ctx.ptr, self.ptr, C.byref(s)))
+        return s
+
+    def read(self, n_cov=0):
+        """(order[:V], cov2d n_cov x 4 float32, stats)."""
+        s = self.stats()
+        order = np.empty(max(s.count, 1), np.uint32)
+        cov = np.empty((max(n_cov, 1), 4), np.float32)
+        check(lib().wd_state_read(self.ctx.ptr, self.ptr, order.ctypes.data, len(order),
+                                  cov.ctypes.data if n_cov else None, n_cov, C.byref(s)))
+        return order[:s.count], cov[:n_cov], s
+
+    def read_keys(self, n):
+        keys = np.empty(n, np.uint16)
+        mask = np.empty((n + 63) // 64, np.uint64)
+        check(lib().wd_state_read_keys(self.ctx.ptr, self.ptr, keys.ctypes.data, mask.ctypes.data, n))
+        keep = np.unpackbits(mask.view(np.uint8), bitorder="little")[:n].astype(bool)
+        return keys, keep
+
+
+from . import workloads  # noqa: E402  (needs the names above)

@@ -1,0 +1,449 @@
+// Host side of the C ABI (include/winding_amd.h): contexts, clouds, sort
+// states, reads, timing. Kernels and their launchers are in splat_sort.hip.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "wd_internal.h"
+
+namespace {
+thread_local char g_error[512] = "";
+}
+
+namespace wd {
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_error, sizeof g_error, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) { return fail(WD_E_HIP, "%s: %s", what, hipGetErrorString(e)); }
+
+int launch_selftest_kernel(wd_context *ctx, const double *args, const double *expect, uint32_t count,
+                           uint32_t *mismatch3);
+
+}  // namespace wd
+
+using wd::fail;
+
+int wd_context::prof_begin(int kernel, hipEvent_t *a) {
+  *a = nullptr;
+  if (!profiling) return WD_OK;
+  if (spare.empty()) {
+    hipEvent_t e;
+    WD_HIP(hipEventCreate(&e), "profile: event");
+    spare.push_back(e);
+  }
+  *a = spare.back();
+  spare.pop_back();
+  WD_HIP(hipEventRecord(*a, stream), "profile: record");
+  (void)kernel;
+  return WD_OK;
+}
+
+int wd_context::prof_end(int kernel, hipEvent_t a) {
+  if (!profiling || !a) return WD_OK;
+  if (spare.empty()) {
+    hipEvent_t e;
+    WD_HIP(hipEventCreate(&e), "profile: event");
+    spare.push_back(e);
+  }
+  hipEvent_t b = spare.back();
+  spare.pop_back();
+  WD_HIP(hipEventRecord(b, stream), "profile: record");
+  pending.push_back({kernel, a, b});
+  return WD_OK;
+}
+
+int wd_context::prof_collect() {
+  if (pending.empty()) return WD_OK;
+  WD_HIP(hipStreamSynchronize(stream), "profile: sync");
+  for (auto &p : pending) {
+    float ms = 0;
+    WD_HIP(hipEventElapsedTime(&ms, p.a, p.b), "profile: elapsed");
+    prof_ms[p.kernel] += ms;
+    prof_launches[p.kernel] += 1;
+    spare.push_back(p.a);
+    spare.push_back(p.b);
+  }
+  pending.clear();
+  return WD_OK;
+}
+
+extern "C" {
+
+const char *wd_last_error(void) { return g_error; }
+
+int wd_open(int device, wd_context **out) {
+  if (!out) return fail(WD_E_ARG, "open: out is null");
+  *out = nullptr;
+  int count = 0;
+  WD_HIP(hipGetDeviceCount(&count), "open: no HIP device");
+  if (device < 0 || device >= count) return fail(WD_E_ARG, "open: device %d, of %d", device, count);
+  WD_HIP(hipSetDevice(device), "open: set device");
+  hipDeviceProp_t prop;
+  WD_HIP(hipGetDeviceProperties(&prop, device), "open: properties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(WD_E_ARG, "open: device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+  wd_context *ctx = new wd_context();
+  ctx->device = device;
+  ctx->cus = prop.multiProcessorCount;
+  size_t free_b = 0, total_b = 0;
+  hipError_t e = hipMemGetInfo(&free_b, &total_b);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return wd::hip_fail(e, "open: stream");
+  }
+  ctx->mem_total = total_b;
+  ctx->mem_free_at_open = free_b;
+  *out = ctx;
+  return WD_OK;
+}
+
+int wd_close(wd_context *ctx) {
+  if (!ctx || ctx->closed) return fail(WD_E_STATE, "close: no context, or closed already");
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto &p : ctx->pending) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  for (auto e : ctx->spare) hipEventDestroy(e);
+  hipStreamDestroy(ctx->stream);
+  ctx->closed = true;
+  delete ctx;
+  return WD_OK;
+}
+
+int wd_sync(wd_context *ctx) {
+  if (!ctx) return fail(WD_E_ARG, "sync: no context");
+  WD_HIP(hipStreamSynchronize(ctx->stream), "sync");
+  return WD_OK;
+}
+
+int wd_stream(wd_context *ctx, void **stream) {
+  if (!ctx || !stream) return fail(WD_E_ARG, "stream: null argument");
+  *stream = (void *)ctx->stream;
+  return WD_OK;
+}
+
+int wd_capacity(wd_context *ctx, uint64_t bytes_per_item, uint64_t *out) {
+  if (!ctx || !out || bytes_per_item == 0) return fail(WD_E_ARG, "capacity: null argument or zero item size");
+  *out = ctx->mem_free_at_open / bytes_per_item;
+  return WD_OK;
+}
+
+static int cloud_from(wd_context *ctx, const void *src, uint64_t bytes, bool device_src, wd_cloud **out,
+                      float *min_out, float *max_out) {
+  if (!ctx || !src || !out) return fail(WD_E_ARG, "splats: null argument");
+  *out = nullptr;
+  if (bytes == 0 || bytes % WD_RECORD_BYTES != 0)
+    return fail(WD_E_DATA, "splats: not a .ply, and %llu bytes is not a whole number of 32-byte .splat records",
+                (unsigned long long)bytes);
+  const uint64_t n = bytes / WD_RECORD_BYTES;
+  if (n > WD_MAX_SORT)
+    return fail(WD_E_CAPACITY, "splats: %llu splats are more than this device holds in one buffer, %u",
+                (unsigned long long)n, WD_MAX_SORT);
+  WD_HIP(hipSetDevice(ctx->device), "splats: set device");
+  wd_cloud *c = new wd_cloud();
+  c->ctx = ctx;
+  c->n = n;
+  uint32_t *scratch = nullptr;
+  uint32_t res[8];
+  hipError_t e = hipMalloc(&c->records, bytes);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->records, src, bytes, device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipMalloc(&scratch, 8 * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    if (scratch) hipFree(scratch);
+    hipFree(c->records);
+    delete c;
+    return wd::hip_fail(e, "splats: upload");
+  }
+  int rc = wd::launch_bounds(ctx, c->records, n, scratch);
+  if (rc == WD_OK) {
+    e = hipMemcpyAsync(res, scratch, sizeof res, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = wd::hip_fail(e, "splats: check");
+  }
+  hipFree(scratch);
+  if (rc == WD_OK && res[6] != 0xFFFFFFFFu) rc = fail(WD_E_DATA, "splats: splat %u has a position that is not a number", res[6]);
+  if (rc != WD_OK) {
+    hipFree(c->records);
+    delete c;
+    return rc;
+  }
+  for (int k = 0; k < 3; ++k) {
+    uint32_t lo = res[k], hi = res[3 + k];
+    lo = (lo & 0x80000000u) ? (lo & 0x7FFFFFFFu) : ~lo;
+    hi = (hi & 0x80000000u) ? (hi & 0x7FFFFFFFu) : ~hi;
+    std::memcpy(&c->min[k], &lo, 4);
+    std::memcpy(&c->max[k], &hi, 4);
+  }
+  if (min_out) std::memcpy(min_out, c->min, sizeof c->min);
+  if (max_out) std::memcpy(max_out, c->max, sizeof c->max);
+  *out = c;
+  return WD_OK;
+}
+
+int wd_cloud_create(wd_context *ctx, const void *records, uint64_t bytes, wd_cloud **out, float min_out[3],
+                    float max_out[3]) {
+  return cloud_from(ctx, records, bytes, false, out, min_out, max_out);
+}
+
+int wd_cloud_create_device(wd_context *ctx, const void *device_records, uint64_t bytes, wd_cloud **out,
+                           float min_out[3], float max_out[3]) {
+  return cloud_from(ctx, device_records, bytes, true, out, min_out, max_out);
+}
+
+int wd_cloud_info(const wd_cloud *cloud, uint64_t *count, float min_out[3], float max_out[3]) {
+  if (!cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "splats: these splats were unloaded");
+  if (count) *count = cloud->n;
+  if (min_out) std::memcpy(min_out, cloud->min, sizeof cloud->min);
+  if (max_out) std::memcpy(max_out, cloud->max, sizeof cloud->max);
+  return WD_OK;
+}
+
+int wd_cloud_records(const wd_cloud *cloud, const void **device_records) {
+  if (!cloud || cloud->magic != 0x53504c54 || !device_records)
+    return fail(WD_E_STATE, "splats: these splats were unloaded");
+  *device_records = cloud->records;
+  return WD_OK;
+}
+
+int wd_cloud_destroy(wd_cloud *cloud) {
+  if (!cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "splats: destroyed twice");
+  hipSetDevice(cloud->ctx->device);
+  hipStreamSynchronize(cloud->ctx->stream);
+  hipFree(cloud->records);
+  cloud->magic = 0;
+  delete cloud;
+  return WD_OK;
+}
+
+int wd_view_prepare(const float view[16], const float world[16], const float projection[16], float width,
+                    float height, const float min[3], const float max[3], float cull, wd_view *out) {
+  if (!view || !projection || !min || !max || !out) return fail(WD_E_ARG, "prepare: null argument");
+  if (!(cull >= 0 && std::isfinite(cull))) return fail(WD_E_ARG, "splatCull must be 0 or more, got %g", (double)cull);
+  static const float identity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  const float *b = world ? world : identity;
+  float mv[16];
+  // mat4Multiply(out, view, world): each entry summed in f64, stored f32.
+  for (int c = 0; c < 4; ++c) {
+    const double b0 = b[c * 4], b1 = b[c * 4 + 1], b2 = b[c * 4 + 2], b3 = b[c * 4 + 3];
+    for (int r = 0; r < 4; ++r)
+      mv[c * 4 + r] = (float)(b0 * view[r] + b1 * view[4 + r] + b2 * view[8 + r] + b3 * view[12 + r]);
+  }
+  double near = INFINITY, far = -INFINITY;
+  for (int k = 0; k < 8; ++k) {
+    const double x = (k & 1) ? max[0] : min[0];
+    const double y = (k & 2) ? max[1] : min[1];
+    const double z = (k & 4) ? max[2] : min[2];
+    const double depth = -((double)mv[2] * x + (double)mv[6] * y + (double)mv[10] * z + (double)mv[14]);
+    if (depth < near) near = depth;
+    if (depth > far) far = depth;
+  }
+  if (far <= 0) return 1;
+  near = std::max(near, 0.0);
+  std::memcpy(out->model_view, mv, sizeof mv);
+  std::memcpy(out->projection, projection, 16 * sizeof(float));
+  out->viewport[0] = width;
+  out->viewport[1] = height;
+  out->depth_min = (float)near;
+  out->depth_scale = (float)(65535.0 / std::max(far - near, 1e-6));
+  out->cull = cull;
+  return WD_OK;
+}
+
+int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
+  if (!ctx || !out) return fail(WD_E_ARG, "state: null argument");
+  *out = nullptr;
+  if (capacity == 0 || capacity > WD_MAX_SORT)
+    return fail(WD_E_CAPACITY, "splats: %llu splats are more than one sort holds, %u", (unsigned long long)capacity,
+                WD_MAX_SORT);
+  WD_HIP(hipSetDevice(ctx->device), "state: set device");
+  wd_state *s = new wd_state();
+  s->ctx = ctx;
+  s->capacity = capacity;
+  s->tiles = (capacity + wd::kTile - 1) / wd::kTile;
+  const uint64_t words = (capacity + 63) / 64;
+  hipError_t e = hipSuccess;
+  auto alloc = [&](void **p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(*p, 0, bytes, ctx->stream);
+  };
+  alloc((void **)&s->ctl, sizeof(wd::Ctl));
+  alloc((void **)&s->keys, capacity * sizeof(uint16_t));
+  alloc((void **)&s->mask, words * sizeof(uint64_t));
+  alloc((void **)&s->packed, capacity * sizeof(uint32_t));
+  alloc((void **)&s->order, capacity * sizeof(uint32_t));
+  alloc((void **)&s->cov, capacity * 4 * sizeof(float));
+  alloc((void **)&s->status_lo, s->tiles * 256 * sizeof(uint64_t));
+  alloc((void **)&s->status_hi, s->tiles * 256 * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    wd_state_destroy(s);
+    return wd::hip_fail(e, "state: allocation");
+  }
+  *out = s;
+  return WD_OK;
+}
+
+int wd_state_destroy(wd_state *s) {
+  if (!s || s->magic != 0x53544154) return fail(WD_E_STATE, "state: destroyed twice");
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  for (void *p : {(void *)s->ctl, (void *)s->keys, (void *)s->mask, (void *)s->packed, (void *)s->order,
+                  (void *)s->cov, (void *)s->status_lo, (void *)s->status_hi})
+    if (p) hipFree(p);
+  s->magic = 0;
+  delete s;
+  return WD_OK;
+}
+
+int wd_state_buffers(wd_state *s, void **order, void **count, void **cov2d) {
+  if (!s || s->magic != 0x53544154) return fail(WD_E_STATE, "state: destroyed");
+  if (order) *order = s->order;
+  if (count) *count = &s->ctl->count;
+  if (cov2d) *cov2d = s->cov;
+  return WD_OK;
+}
+
+static int check_sort_args(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_state *s) {
+  if (!ctx || !view) return fail(WD_E_ARG, "sort: null argument");
+  if (!cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "addSplats: these splats were unloaded; remove the node first");
+  if (!s || s->magic != 0x53544154) return fail(WD_E_STATE, "sort: the sort state was destroyed");
+  if (cloud->ctx != ctx || s->ctx != ctx) return fail(WD_E_STATE, "sort: splats, state and context are of different devices");
+  if (cloud->n > s->capacity)
+    return fail(WD_E_CAPACITY, "sort: %llu splats are more than this state holds, %llu", (unsigned long long)cloud->n,
+                (unsigned long long)s->capacity);
+  if (!(view->cull >= 0 && std::isfinite(view->cull)))
+    return fail(WD_E_ARG, "splatCull must be 0 or more, got %g", (double)view->cull);
+  return WD_OK;
+}
+
+int wd_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_state *s) {
+  int rc = check_sort_args(ctx, cloud, view, s);
+  if (rc) return rc;
+  WD_HIP(hipSetDevice(ctx->device), "sort: set device");
+  return wd::launch_sort(ctx, cloud, *view, s);
+}
+
+int wd_state_stats(wd_context *ctx, wd_state *s, wd_stats *stats) {
+  if (!ctx || !s || s->magic != 0x53544154 || !stats) return fail(WD_E_ARG, "stats: null argument or destroyed state");
+  wd::Ctl head;
+  WD_HIP(hipMemcpyAsync(&head.count, &s->ctl->count, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+         "stats: read");
+  WD_HIP(hipStreamSynchronize(ctx->stream), "stats: sync");
+  stats->count = head.count;
+  stats->recomputed = head.recomputed;
+  stats->fallback = head.fallback;
+  stats->overflow_bins = head.overflow_bins;
+  return WD_OK;
+}
+
+int wd_state_read(wd_context *ctx, wd_state *s, uint32_t *order, uint64_t order_capacity, float *cov2d,
+                  uint64_t cov2d_capacity, wd_stats *stats) {
+  wd_stats st;
+  int rc = wd_state_stats(ctx, s, &st);
+  if (rc) return rc;
+  if (order) {
+    if (order_capacity < st.count)
+      return fail(WD_E_CAPACITY, "read: order holds %llu, the sort has %u", (unsigned long long)order_capacity, st.count);
+    WD_HIP(hipMemcpy(order, s->order, (size_t)st.count * 4, hipMemcpyDeviceToHost), "read: order");
+  }
+  if (cov2d) {
+    const uint64_t n = std::min<uint64_t>(cov2d_capacity, s->last_n);
+    WD_HIP(hipMemcpy(cov2d, s->cov, (size_t)n * 16, hipMemcpyDeviceToHost), "read: cov2d");
+  }
+  if (stats) *stats = st;
+  return WD_OK;
+}
+
+int wd_state_read_keys(wd_context *ctx, wd_state *s, uint16_t *keys, uint64_t *mask, uint64_t n) {
+  if (!ctx || !s || s->magic != 0x53544154) return fail(WD_E_ARG, "read keys: null argument or destroyed state");
+  if (n > s->last_n) return fail(WD_E_ARG, "read keys: %llu asked, last sort had %llu", (unsigned long long)n,
+                                 (unsigned long long)s->last_n);
+  WD_HIP(hipStreamSynchronize(ctx->stream), "read keys: sync");
+  if (keys) WD_HIP(hipMemcpy(keys, s->keys, (size_t)n * 2, hipMemcpyDeviceToHost), "read keys: keys");
+  if (mask) WD_HIP(hipMemcpy(mask, s->mask, (size_t)((n + 63) / 64) * 8, hipMemcpyDeviceToHost), "read keys: mask");
+  return WD_OK;
+}
+
+int wd_profile(wd_context *ctx, int on) {
+  if (!ctx) return fail(WD_E_ARG, "profile: no context");
+  int rc = ctx->prof_collect();
+  if (rc) return rc;
+  ctx->profiling = on != 0;
+  for (int k = 0; k < 16; ++k) {
+    ctx->prof_ms[k] = 0;
+    ctx->prof_launches[k] = 0;
+  }
+  return WD_OK;
+}
+
+int wd_profile_read(wd_context *ctx, const char **names, double *ms, uint64_t *launches, int capacity, int *n) {
+  if (!ctx || !n) return fail(WD_E_ARG, "profile read: null argument");
+  int rc = ctx->prof_collect();
+  if (rc) return rc;
+  int k = 0;
+  for (int i = 0; i < wd::kKernelCount && k < capacity; ++i) {
+    if (names) names[k] = wd::kKernelNames[i];
+    if (ms) ms[k] = ctx->prof_ms[i];
+    if (launches) launches[k] = ctx->prof_launches[i];
+    ++k;
+  }
+  *n = k;
+  return WD_OK;
+}
+
+// ---- debugging entry points used by the exactness tests ----
+
+int wd_debug_canonical(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, double *out5_host) {
+  if (!ctx || !cloud || !view || !out5_host) return fail(WD_E_ARG, "canonical debug: null argument");
+  double *dev = nullptr;
+  const size_t bytes = cloud->n * 5 * sizeof(double);
+  WD_HIP(hipMalloc(&dev, bytes), "canonical debug: alloc");
+  int rc = wd::launch_canonical_debug(ctx, cloud, *view, dev);
+  if (rc == WD_OK) {
+    hipError_t e = hipMemcpyAsync(out5_host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = wd::hip_fail(e, "canonical debug: read");
+  }
+  hipFree(dev);
+  return rc;
+}
+
+int wd_debug_selftest(wd_context *ctx, const double *args, const double *expect, uint32_t count, uint32_t mismatch3[3]) {
+  if (!ctx || !mismatch3 || (count && (!args || !expect))) return fail(WD_E_ARG, "selftest: null argument");
+  double *da = nullptr, *de = nullptr;
+  uint32_t *dm = nullptr;
+  const size_t bytes = (size_t)std::max<uint32_t>(count, 1) * 2 * sizeof(double);
+  hipError_t e = hipMalloc(&da, bytes);
+  if (e == hipSuccess) e = hipMalloc(&de, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dm, 3 * sizeof(uint32_t));
+  if (e == hipSuccess && count) e = hipMemcpy(da, args, (size_t)count * 16, hipMemcpyHostToDevice);
+  if (e == hipSuccess && count) e = hipMemcpy(de, expect, (size_t)count * 16, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(dm, 0, 3 * sizeof(uint32_t));
+  int rc = e == hipSuccess ? wd::launch_selftest_kernel(ctx, da, de, count, dm) : wd::hip_fail(e, "selftest: setup");
+  if (rc == WD_OK) {
+    e = hipMemcpyAsync(mismatch3, dm, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = wd::hip_fail(e, "selftest: read");
+  }
+  hipFree(da);
+  hipFree(de);
+  hipFree(dm);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// Internal declarations shared by the host API (api.cpp) and the kernels
+// (splat_sort.hip, batch.hip). Not installed; include/winding_amd.h is the ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "winding_amd.h"
+
+namespace wd {
+
+// Records the message for wd_last_error() and returns code.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+// A HIP failure: "<what>: <hipGetErrorString>".
+int hip_fail(hipError_t e, const char *what);
+
+#define WD_HIP(call, what)                    \
+  do {                                        \
+    hipError_t wd_e_ = (call);                \
+    if (wd_e_ != hipSuccess) return ::wd::hip_fail(wd_e_, what); \
+  } while (0)
+
+constexpr int kThreads = 256;        // workgroup size of every kernel here
+constexpr int kTile = 4096;          // splats per scatter tile: 256 threads x 16
+constexpr int kKeysPerThread = 16;
+
+// Flags of a look-back status word: value in the low 32 bits, epoch << 2 | flag above.
+constexpr uint32_t kFlagAggregate = 1;
+constexpr uint32_t kFlagInclusive = 2;
+
+// Device control block of one sort state. The K1 histogram, its finish counter
+// and the tickets reset themselves (the last K1 workgroup zeroes them), so a
+// sort never needs a memset, in a graph or out of one.
+struct Ctl {
+  uint32_t hist[512];     // survivors per low byte (0..255) and high byte (256..511)
+  uint32_t base[512];     // exclusive prefix of hist: where each digit starts
+  uint32_t count;         // V, survivors of the last sort
+  uint32_t done;          // projection workgroups finished
+  uint32_t ticket_lo;     // tile tickets of the two scatter passes
+  uint32_t ticket_hi;
+  uint32_t recomputed;    // FP64 area decisions in the last sort
+  uint32_t recomputed_acc;
+  uint32_t fallback;      // learned sort: 1 when the exact path ran
+  uint32_t overflow_bins;
+  uint32_t pad[56];
+};
+static_assert(sizeof(Ctl) % 256 == 0, "Ctl keeps 256-byte alignment");
+
+// The per-view constants a kernel takes by value: the uniform plus what the
+// f32 fast path and its error bound precompute once per view.
+struct ViewK {
+  float mv[16];
+  float P[16];
+  float vpw, vph;
+  float dmin, dscale, cull;
+  float fx, fy;       // P00 * W / 2, P11 * H / 2 (f32)
+  float limx, limy;   // 1.3 / P00, 1.3 / P11 (f32)
+  float n0, n1;       // row norms of T's bound (see k_project)
+  int persp;          // P[15] == 0
+};
+
+// Builds ViewK from a wd_view (host).
+ViewK make_viewk(const wd_view &v);
+
+}  // namespace wd
+
+struct wd_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cus = 0;
+  uint64_t mem_total = 0;
+  uint64_t mem_free_at_open = 0;
+  bool closed = false;
+  // Per-kernel event timing.
+  bool profiling = false;
+  struct Pending { int kernel; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> spare;
+  double prof_ms[16] = {};
+  uint64_t prof_launches[16] = {};
+  int prof_begin(int kernel, hipEvent_t *a);
+  int prof_end(int kernel, hipEvent_t a);
+  int prof_collect();
+};
+
+struct wd_cloud {
+  wd_context *ctx = nullptr;
+  uint8_t *records = nullptr;  // device, n * 32 bytes
+  uint64_t n = 0;
+  float min[3] = {0, 0, 0}, max[3] = {0, 0, 0};
+  uint32_t magic = 0x53504c54;  // cleared by destroy
+};
+
+struct wd_state {
+  wd_context *ctx = nullptr;
+  uint64_t capacity = 0;
+  uint64_t last_n = 0;
+  uint32_t epoch = 0;          // look-back epoch of the last sort
+  wd::Ctl *ctl = nullptr;      // device
+  uint16_t *keys = nullptr;    // n keys, every splat
+  uint64_t *mask = nullptr;    // survivor bits, ceil(n/64) words
+  uint32_t *packed = nullptr;  // low-digit pass output: high byte << 24 | index
+  uint32_t *order = nullptr;   // the draw order
+  float *cov = nullptr;        // n x 4 floats
+  uint64_t *status_lo = nullptr, *status_hi = nullptr;  // look-back words, tiles x 256
+  uint64_t tiles = 0;
+  uint32_t magic = 0x53544154;
+};
+
+namespace wd {
+
+enum KernelId { kKBounds = 0, kKProject, kKScatterLo, kKScatterHi, kKBatch, kKMlpFwd, kKMlpBwd, kKLearned, kKernelCount };
+extern const char *const kKernelNames[kKernelCount];
+
+// Kernel launchers (splat_sort.hip). Enqueue on ctx->stream.
+int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8);
+int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_state *st);
+int launch_canonical_debug(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, double *out5);
+
+}  // namespace wd

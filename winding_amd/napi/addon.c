@@ -1,0 +1,379 @@
+/*
+ * winding_amd.node -- the N-API addon the JavaScript module loads: a thin
+ * shim from JS values to the C ABI of libwinding_amd.so (include/winding_amd.h).
+ *
+ * Handles are externals holding a small box; GPU objects also keep a strong
+ * reference to their context's external, so a context outlives its clouds and
+ * states. destroy() frees at once; a box the JS side dropped without
+ * destroy() is freed by its finaliser. A call on a destroyed handle throws,
+ * and every C failure becomes a JS Error with wd_last_error()'s message, the
+ * reference's synchronous `throw new Error('splats: ...')` convention
+ * (src/render/splats.js:394-400).
+ */
+#include <node_api.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "winding_amd.h"
+
+enum { K_CTX = 1, K_CLOUD = 2, K_STATE = 3 };
+
+typedef struct Box {
+  int kind;
+  void *ptr;         /* wd_context*, wd_cloud*, wd_state*; NULL once destroyed */
+  napi_ref parent;   /* the context's external, for clouds and states */
+  struct Box *ctx;   /* the context's box */
+  int children;      /* contexts: live clouds and states */
+} Box;
+
+#define CHECK_NAPI(call)                                   \
+  do {                                                     \
+    if ((call) != napi_ok) {                               \
+      napi_throw_error(env, NULL, "winding_amd: N-API call failed"); \
+      return NULL;                                         \
+    }                                                      \
+  } while (0)
+
+static napi_value throw_wd(napi_env env, int rc) {
+  const char *msg = wd_last_error();
+  (void)rc;
+  napi_throw_error(env, NULL, msg && msg[0] ? msg : "winding_amd: failed");
+  return NULL;
+}
+
+static void release_box(napi_env env, Box *b) {
+  if (b->ptr) {
+    if (b->kind == K_CLOUD) wd_cloud_destroy((wd_cloud *)b->ptr);
+    if (b->kind == K_STATE) wd_state_destroy((wd_state *)b->ptr);
+    if (b->kind == K_CTX) wd_close((wd_context *)b->ptr);
+    b->ptr = NULL;
+    if (b->ctx) b->ctx->children--;
+  }
+  if (b->parent) {
+    napi_delete_reference(env, b->parent);
+    b->parent = NULL;
+  }
+}
+
+static void finalize_box(napi_env env, void *data, void *hint) {
+  (void)hint;
+  Box *b = (Box *)data;
+  /* a context is only collected once no cloud or state refers to it */
+  release_box(env, b);
+  free(b);
+}
+
+static napi_value make_handle(napi_env env, int kind, void *ptr, napi_value ctx_value, Box *ctx_box) {
+  Box *b = (Box *)calloc(1, sizeof(Box));
+  napi_value ext;
+  b->kind = kind;
+  b->ptr = ptr;
+  b->ctx = ctx_box;
+  if (napi_create_external(env, b, finalize_box, NULL, &ext) != napi_ok) {
+    free(b);
+    napi_throw_error(env, NULL, "winding_amd: N-API call failed");
+    return NULL;
+  }
+  if (ctx_value) {
+    napi_create_reference(env, ctx_value, 1, &b->parent);
+    ctx_box->children++;
+  }
+  return ext;
+}
+
+static Box *get_box(napi_env env, napi_value v, int kind) {
+  void *data = NULL;
+  if (napi_get_value_external(env, v, &data) != napi_ok || !data || ((Box *)data)->kind != kind) {
+    napi_throw_type_error(env, NULL, "winding_amd: wrong handle");
+    return NULL;
+  }
+  Box *b = (Box *)data;
+  if (!b->ptr) {
+    napi_throw_error(env, NULL,
+                     kind == K_CLOUD ? "addSplats: these splats were unloaded; remove the node first"
+                                     : "winding_amd: this handle was destroyed");
+    return NULL;
+  }
+  return b;
+}
+
+static int get_args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok || argc < want) {
+    napi_throw_type_error(env, NULL, "winding_amd: too few arguments");
+    return 0;
+  }
+  return 1;
+}
+
+static int typed(napi_env env, napi_value v, napi_typedarray_type want, void **data, size_t *length) {
+  bool is = false;
+  napi_typedarray_type t;
+  napi_value buf;
+  size_t off;
+  if (napi_is_typedarray(env, v, &is) != napi_ok || !is ||
+      napi_get_typedarray_info(env, v, &t, length, data, &buf, &off) != napi_ok || t != want) {
+    napi_throw_type_error(env, NULL, "winding_amd: expected a typed array of another type");
+    return 0;
+  }
+  return 1;
+}
+
+static napi_value f32_array(napi_env env, const float *src, size_t n) {
+  napi_value ab, arr;
+  void *data;
+  if (napi_create_arraybuffer(env, n * 4, &data, &ab) != napi_ok) return NULL;
+  memcpy(data, src, n * 4);
+  napi_create_typedarray(env, napi_float32_array, n, ab, 0, &arr);
+  return arr;
+}
+
+static napi_value num(napi_env env, double x) {
+  napi_value v;
+  napi_create_double(env, x, &v);
+  return v;
+}
+
+/* open(device) -> context */
+static napi_value js_open(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  int32_t dev = 0;
+  if (!get_args(env, info, 1, argv)) return NULL;
+  CHECK_NAPI(napi_get_value_int32(env, argv[0], &dev));
+  wd_context *ctx = NULL;
+  int rc = wd_open(dev, &ctx);
+  if (rc) return throw_wd(env, rc);
+  return make_handle(env, K_CTX, ctx, NULL, NULL);
+}
+
+/* close(context): refuses while clouds or states are alive */
+static napi_value js_close(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  Box *b = get_box(env, argv[0], K_CTX);
+  if (!b) return NULL;
+  if (b->children > 0) {
+    napi_throw_error(env, NULL, "winding_amd: close: destroy its splats and sort states first");
+    return NULL;
+  }
+  release_box(env, b);
+  return NULL;
+}
+
+static napi_value js_sync(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  Box *b = get_box(env, argv[0], K_CTX);
+  if (!b) return NULL;
+  int rc = wd_sync((wd_context *)b->ptr);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
+/* cloudCreate(context, Uint8Array records) -> {handle, count, min, max} */
+static napi_value js_cloud_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2], out;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  void *data;
+  size_t len;
+  if (!typed(env, argv[1], napi_uint8_array, &data, &len)) return NULL;
+  wd_cloud *cloud = NULL;
+  float mn[3], mx[3];
+  int rc = wd_cloud_create((wd_context *)c->ptr, data, len, &cloud, mn, mx);
+  if (rc) return throw_wd(env, rc);
+  napi_value h = make_handle(env, K_CLOUD, cloud, argv[0], c);
+  if (!h) return NULL;
+  CHECK_NAPI(napi_create_object(env, &out));
+  napi_set_named_property(env, out, "handle", h);
+  napi_set_named_property(env, out, "count", num(env, (double)(len / 32)));
+  napi_set_named_property(env, out, "min", f32_array(env, mn, 3));
+  napi_set_named_property(env, out, "max", f32_array(env, mx, 3));
+  return out;
+}
+
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  void *data = NULL;
+  if (!get_args(env, info, 1, argv)) return NULL;
+  if (napi_get_value_external(env, argv[0], &data) != napi_ok || !data) {
+    napi_throw_type_error(env, NULL, "winding_amd: wrong handle");
+    return NULL;
+  }
+  Box *b = (Box *)data;
+  if (b->kind == K_CTX) {
+    napi_throw_type_error(env, NULL, "winding_amd: close a context with close()");
+    return NULL;
+  }
+  release_box(env, b);
+  return NULL;
+}
+
+/*
+ * viewPrepare(view16, world16|null, projection16, width, height, min3, max3, cull)
+ * -> Float32Array(37) uniform, or null when the cloud is wholly behind.
+ */
+static napi_value js_view_prepare(napi_env env, napi_callback_info info) {
+  napi_value argv[8], nul;
+  if (!get_args(env, info, 8, argv)) return NULL;
+  void *view, *proj, *mn, *mx, *world = NULL;
+  size_t l;
+  napi_valuetype t;
+  double w, h, cull;
+  if (!typed(env, argv[0], napi_float32_array, &view, &l) || l < 16) return NULL;
+  napi_typeof(env, argv[1], &t);
+  if (t != napi_null && t != napi_undefined && (!typed(env, argv[1], napi_float32_array, &world, &l) || l < 16))
+    return NULL;
+  if (!typed(env, argv[2], napi_float32_array, &proj, &l) || l < 16) return NULL;
+  CHECK_NAPI(napi_get_value_double(env, argv[3], &w));
+  CHECK_NAPI(napi_get_value_double(env, argv[4], &h));
+  if (!typed(env, argv[5], napi_float32_array, &mn, &l) || l < 3) return NULL;
+  if (!typed(env, argv[6], napi_float32_array, &mx, &l) || l < 3) return NULL;
+  CHECK_NAPI(napi_get_value_double(env, argv[7], &cull));
+  wd_view v;
+  int rc = wd_view_prepare((const float *)view, (const float *)world, (const float *)proj, (float)w, (float)h,
+                           (const float *)mn, (const float *)mx, (float)cull, &v);
+  if (rc < 0) return throw_wd(env, rc);
+  if (rc == 1) {
+    napi_get_null(env, &nul);
+    return nul;
+  }
+  return f32_array(env, (const float *)&v, sizeof v / 4);
+}
+
+/* stateCreate(context, capacity) -> state */
+static napi_value js_state_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  double cap;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  CHECK_NAPI(napi_get_value_double(env, argv[1], &cap));
+  wd_state *s = NULL;
+  int rc = wd_state_create((wd_context *)c->ptr, (uint64_t)cap, &s);
+  if (rc) return throw_wd(env, rc);
+  return make_handle(env, K_STATE, s, argv[0], c);
+}
+
+/* sort(context, cloud, uniform Float32Array(37), state) */
+static napi_value js_sort(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *cl = get_box(env, argv[1], K_CLOUD);
+  if (!cl) return NULL;
+  void *u;
+  size_t l;
+  if (!typed(env, argv[2], napi_float32_array, &u, &l) || l < sizeof(wd_view) / 4) return NULL;
+  Box *s = get_box(env, argv[3], K_STATE);
+  if (!s) return NULL;
+  wd_view v;
+  memcpy(&v, u, sizeof v);
+  int rc = wd_sort((wd_context *)c->ptr, (wd_cloud *)cl->ptr, &v, (wd_state *)s->ptr);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
+/* stateRead(context, state, covCount) -> {count, recomputed, order, cov2d?} (waits) */
+static napi_value js_state_read(napi_env env, napi_callback_info info) {
+  napi_value argv[3], out, ab, arr;
+  double ncov;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *s = get_box(env, argv[1], K_STATE);
+  if (!s) return NULL;
+  CHECK_NAPI(napi_get_value_double(env, argv[2], &ncov));
+  wd_stats st;
+  int rc = wd_state_stats((wd_context *)c->ptr, (wd_state *)s->ptr, &st);
+  if (rc) return throw_wd(env, rc);
+  void *order_data, *cov_data = NULL;
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)st.count * 4 + 4, &order_data, &ab));
+  napi_value cov_ab = NULL;
+  if (ncov > 0) CHECK_NAPI(napi_create_arraybuffer(env, (size_t)ncov * 16, &cov_data, &cov_ab));
+  rc = wd_state_read((wd_context *)c->ptr, (wd_state *)s->ptr, (uint32_t *)order_data, st.count + 1,
+                     (float *)cov_data, (uint64_t)ncov, &st);
+  if (rc) return throw_wd(env, rc);
+  CHECK_NAPI(napi_create_object(env, &out));
+  napi_create_typedarray(env, napi_uint32_array, st.count, ab, 0, &arr);
+  napi_set_named_property(env, out, "order", arr);
+  napi_set_named_property(env, out, "count", num(env, st.count));
+  napi_set_named_property(env, out, "recomputed", num(env, st.recomputed));
+  if (cov_ab) {
+    napi_create_typedarray(env, napi_float32_array, (size_t)ncov * 4, cov_ab, 0, &arr);
+    napi_set_named_property(env, out, "cov2d", arr);
+  }
+  return out;
+}
+
+/* synth(kind, seed, n) -> Uint8Array of n seeded .splat records (host only) */
+static napi_value js_synth(napi_env env, napi_callback_info info) {
+  napi_value argv[3], ab, arr;
+  int32_t kind;
+  double seed, n;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  CHECK_NAPI(napi_get_value_int32(env, argv[0], &kind));
+  CHECK_NAPI(napi_get_value_double(env, argv[1], &seed));
+  CHECK_NAPI(napi_get_value_double(env, argv[2], &n));
+  void *data;
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)n * 32, &data, &ab));
+  int rc = wd_synth_records(kind, (uint64_t)seed, (uint64_t)n, data);
+  if (rc) return throw_wd(env, rc);
+  napi_create_typedarray(env, napi_uint8_array, (size_t)n * 32, ab, 0, &arr);
+  return arr;
+}
+
+/* profile(context, on) / profileRead(context) -> {name: [ms, launches]} */
+static napi_value js_profile(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  bool on;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  CHECK_NAPI(napi_get_value_bool(env, argv[1], &on));
+  int rc = wd_profile((wd_context *)c->ptr, on ? 1 : 0);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
+static napi_value js_profile_read(napi_env env, napi_callback_info info) {
+  napi_value argv[1], out, pair;
+  if (!get_args(env, info, 1, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  const char *names[16];
+  double ms[16];
+  uint64_t launches[16];
+  int n = 0;
+  int rc = wd_profile_read((wd_context *)c->ptr, names, ms, launches, 16, &n);
+  if (rc) return throw_wd(env, rc);
+  CHECK_NAPI(napi_create_object(env, &out));
+  for (int i = 0; i < n; ++i) {
+    if (!launches[i]) continue;
+    napi_create_array_with_length(env, 2, &pair);
+    napi_set_element(env, pair, 0, num(env, ms[i]));
+    napi_set_element(env, pair, 1, num(env, (double)launches[i]));
+    napi_set_named_property(env, out, names[i], pair);
+  }
+  return out;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  const struct {
+    const char *name;
+    napi_callback fn;
+  } fns[] = {
+      {"open", js_open},           {"close", js_close},           {"sync", js_sync},
+      {"cloudCreate", js_cloud_create}, {"destroy", js_destroy},  {"viewPrepare", js_view_prepare},
+      {"stateCreate", js_state_create}, {"sort", js_sort},        {"stateRead", js_state_read},
+      {"synth", js_synth},         {"profile", js_profile},       {"profileRead", js_profile_read},
+  };
+  for (size_t i = 0; i < sizeof fns / sizeof fns[0]; ++i) {
+    napi_value f;
+    if (napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].fn, NULL, &f) != napi_ok) return NULL;
+    napi_set_named_property(env, exports, fns[i].name, f);
+  }
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

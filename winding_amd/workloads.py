@@ -1,0 +1,95 @@
+"""The measured workloads: seeded inputs and cameras, by name.
+
+  plain-64k   65,536 splats, Gaussian mixture, seed 0, eye (0,0,10), fovY 60deg, 1280x720
+  plain-1m    1,048,576 splats, the same generator and camera
+  slab-8m     8,388,608 splats, 80% in a slab |z|<=0.25 at distance 10, 20% in a 40^3 box, 1920x1080
+  batch-64    64 scenes, sizes log-uniform in [1e4, 2e6] (about 32M splats), one random orbit view each
+  learned-4m  4,194,304 splats, mixture generator, a 1024-pose orbit; the predictor trains on 4096 random poses
+
+The cull is the reference's default, 0.5 (src/render/splats.js:78), unless a caller passes another.
+"""
+import math
+
+import numpy as np
+
+from . import WD_SPLAT_CULL, look_at, perspective, prepare_view, synth_records
+
+NEAR = 0.1
+FOVY = math.pi / 3
+
+
+class Stream:
+    """splitmix64, the generator's own stream, for sizes and poses."""
+
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+PLAIN = {
+    "plain-64k": dict(n=65536, kind=0, seed=0, eye=(0.0, 0.0, 10.0), width=1280, height=720),
+    "plain-1m": dict(n=1048576, kind=0, seed=0, eye=(0.0, 0.0, 10.0), width=1280, height=720),
+    "slab-8m": dict(n=8388608, kind=1, seed=0, eye=(0.0, 0.0, 10.0), width=1920, height=1080),
+}
+
+
+def camera(eye, width, height, target=(0.0, 0.0, 0.0)):
+    """(view, projection) as Camera.update makes them (src/scene/camera.js:258-278)."""
+    return look_at(eye, target, (0.0, 1.0, 0.0)), perspective(FOVY, width / height, NEAR)
+
+
+def plain(name, cull=WD_SPLAT_CULL, n=None):
+    """(records, wd_view, spec) of a single-view workload; n overrides the size (same generator)."""
+    spec = dict(PLAIN[name])
+    if n is not None:
+        spec["n"] = n
+    rec = synth_records(spec["kind"], spec["seed"], spec["n"])
+    return rec, view_for(rec, spec["eye"], spec["width"], spec["height"], cull), spec
+
+
+def bounds_of(records):
+    pos = np.ascontiguousarray(records).view(np.float32).reshape(-1, 8)[:, :3]
+    return pos.min(axis=0), pos.max(axis=0)
+
+
+def view_for(records, eye, width, height, cull=WD_SPLAT_CULL, target=(0.0, 0.0, 0.0)):
+    mn, mx = bounds_of(records)
+    v, p = camera(eye, width, height, target)
+    return prepare_view(v, p, float(width), float(height), mn, mx, cull)
+
+
+def orbit_pose(stream, rmin=5.0, rmax=20.0):
+    """A random eye on a sphere of radius U[rmin, rmax] about the origin."""
+    r = rmin + (rmax - rmin) * stream.uniform()
+    z = -1.0 + 2.0 * stream.uniform()
+    phi = 2.0 * math.pi * stream.uniform()
+    s = math.sqrt(max(0.0, 1.0 - z * z))
+    return (r * s * math.cos(phi), r * z, r * s * math.sin(phi))
+
+
+def batch_sizes(seed=None, count=64, lo=1e4, hi=2e6, target=32_000_000):
+    """Scene sizes log-uniform in [lo, hi]. With no seed, the seed in 0..999 whose total is nearest target."""
+    def sizes(s):
+        st = Stream(s)
+        return [int(round(math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * st.uniform()))) for _ in range(count)]
+    if seed is None:
+        seed = min(range(1000), key=lambda s: abs(sum(sizes(s)) - target))
+    return seed, sizes(seed)
+
+
+def orbit_sequence(count=1024, radius=10.0):
+    """The learned workload's camera path: a full circle about y at radius 10, height swinging +-3."""
+    eyes = []
+    for k in range(count):
+        a = 2.0 * math.pi * k / count
+        eyes.append((radius * math.sin(a), 3.0 * math.sin(3.0 * a), radius * math.cos(a)))
+    return eyes
