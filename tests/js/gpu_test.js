@@ -1,0 +1,101 @@
+'use strict';
+// The JavaScript API end to end on the GPU: Camera -> SplatScene -> SplatPass.prepare -> sort ->
+// read, checked against the JavaScript CPU path (oracle/splat_oracle.js) in the same process.
+const assert = require('assert');
+const path = require('path');
+
+const root = path.join(__dirname, '..', '..');
+const wd = require(path.join(root, 'winding_amd', 'js', 'index.js'));
+const oracle = require(path.join(root, 'oracle', 'splat_oracle.js'));
+
+let passed = 0;
+function test(name, fn) {
+  fn();
+  passed++;
+  console.log('  ok  ' + name);
+}
+
+function record(pos, scale, rgba, quat) {
+  const b = new Uint8Array(32);
+  const v = new DataView(b.buffer);
+  pos.concat(scale).forEach((x, i) => v.setFloat32(i * 4, x, true));
+  b.set(rgba, 24);
+  b.set(quat, 28);
+  return b;
+}
+
+const gpu = wd.Gpu.open(0);
+
+function sortOnce(bytes, cam, width, height, cull) {
+  const splats = new wd.Splats(gpu, wd.parseSplats(bytes));
+  const scene = new wd.SplatScene();
+  const entity = scene.addSplats({ splats });
+  const pass = new wd.SplatPass(gpu);
+  pass.cull = cull;
+  pass.prepare(scene, cam, null, width, height);
+  pass.sort();
+  const got = pass.read(entity, { cov2d: true });
+  const uniform = wd.cloudUniform(new Float32Array(37), cam.view, new Float32Array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]),
+    cam.projection, width, height, splats.min, splats.max, cull);
+  pass.destroy();
+  splats.destroy();
+  return { got, uniform };
+}
+
+test('65,536 seeded splats: order bit-exact with the JS CPU path, footprints within 1e-5', () => {
+  const bytes = gpu.native.synth(0, 0, 65536);
+  const cam = new wd.Camera({ fovY: Math.PI / 3, near: 0.1 });
+  cam.position.set([0, 0, 10]);
+  cam.update(1280 / 720);
+  const { got, uniform } = sortOnce(bytes, cam, 1280, 720, 0.5);
+  const view = oracle.viewFromFloats(uniform);
+  const cov = new Float64Array(65536 * 3);
+  const want = oracle.run(bytes, view, cov);
+  assert.strictEqual(got.count, want.length);
+  assert.deepStrictEqual(Array.from(got.order), Array.from(want));
+  let worst = 0;
+  for (const i of want) {
+    const s = Math.max(Math.abs(cov[3 * i]), Math.abs(cov[3 * i + 2]));
+    for (let k = 0; k < 3; k++) worst = Math.max(worst, Math.abs(got.cov2d[4 * i + k] - cov[3 * i + k]) / s);
+  }
+  assert.ok(worst <= 1e-5, 'cov2d error ' + worst);
+  console.log('      V = ' + got.count + ', cov2d max rel err ' + worst.toExponential(2));
+});
+
+test('two splats on the view axis are drawn back to front from either side (test/gpu.test.js:1804-1870)', () => {
+  const red = record([0, 0, 1], [0.4, 0.4, 0.4], [255, 0, 0, 255], [255, 128, 128, 128]);
+  const blue = record([0, 0, -1], [0.4, 0.4, 0.4], [0, 0, 255, 255], [255, 128, 128, 128]);
+  for (const [first, second, ri, bi] of [[red, blue, 0, 1], [blue, red, 1, 0]]) {
+    const bytes = new Uint8Array(64);
+    bytes.set(first, 0);
+    bytes.set(second, 32);
+    const cam = new wd.Camera({ fovY: 0.8, near: 0.1 });
+    cam.position.set([0, 0, 5]);
+    cam.update(160 / 120);
+    assert.deepStrictEqual(Array.from(sortOnce(bytes, cam, 160, 120, 0.5).got.order), [bi, ri]);
+    cam.position.set([0, 0, -5]);
+    cam.update(160 / 120);
+    assert.deepStrictEqual(Array.from(sortOnce(bytes, cam, 160, 120, 0.5).got.order), [ri, bi]);
+  }
+});
+
+test('a faint speck is left out at the default cull and kept at 0 (test/gpu.test.js:1914-1962)', () => {
+  const bytes = new Uint8Array(64);
+  bytes.set(record([0, 0, 0], [0.0005, 0.0005, 0.0005], [255, 255, 255, 50], [255, 128, 128, 128]), 0);
+  bytes.set(record([1, 0, 0], [0.2, 0.2, 0.2], [255, 255, 255, 255], [255, 128, 128, 128]), 32);
+  const cam = new wd.Camera({ fovY: 0.8, near: 0.1 });
+  cam.position.set([0, 0, 5]);
+  cam.update(160 / 120);
+  assert.deepStrictEqual(Array.from(sortOnce(bytes, cam, 160, 120, 0.5).got.order), [1]);
+  assert.deepStrictEqual(Array.from(sortOnce(bytes, cam, 160, 120, 0).got.order).sort(), [0, 1]);
+});
+
+test('a NaN position is refused at upload, by index', () => {
+  const bytes = new Uint8Array(64);
+  bytes.set(record([0, 0, 0], [1, 1, 1], [0, 0, 0, 255], [255, 128, 128, 128]), 0);
+  bytes.set(record([NaN, 0, 0], [1, 1, 1], [0, 0, 0, 255], [255, 128, 128, 128]), 32);
+  assert.throws(() => new wd.Splats(gpu, wd.parseSplats(bytes)), /splat 1 has a position that is not a number/);
+});
+
+gpu.destroy();
+console.log(passed + ' checks passed');

@@ -105,7 +105,7 @@ __device__ __forceinline__ void store_status(uint64_t *p, uint32_t epoch, uint32
 // put() (src/scene/splats.js:350-368), footprint() (src/render/splats.js:
 // 102-125) and the area test (:174-178), in the oracle's expression order.
 // Returns alpha * area.
-__device__ __noinline__ double canonical_value(const ViewK &v, uint4 a, uint4 b, double vx, double vy, double vz) {
+__device__ __forceinline__ double canonical_value(const ViewK &v, uint4 a, uint4 b, double vx, double vy, double vz) {
   const double sx = f32_of(a.w), sy = f32_of(b.x), sz = f32_of(b.y);
   const uint32_t q = b.w;
   const double qw = ((int)(q & 255u) - 128) / 128.0, qx = ((int)((q >> 8) & 255u) - 128) / 128.0;
@@ -123,7 +123,7 @@ __device__ __noinline__ double canonical_value(const ViewK &v, uint4 a, uint4 b,
       {m00 * m00 + m01 * m01 + m02 * m02, m00 * m10 + m01 * m11 + m02 * m12, m00 * m20 + m01 * m21 + m02 * m22},
       {m00 * m10 + m01 * m11 + m02 * m12, m10 * m10 + m11 * m11 + m12 * m12, m10 * m20 + m11 * m21 + m12 * m22},
       {m00 * m20 + m01 * m21 + m02 * m22, m10 * m20 + m11 * m21 + m12 * m22, m20 * m20 + m21 * m21 + m22 * m22}};
-  const double p00 = v.P[0], p11 = v.P[5];
+  const double p00 = v.dP[0], p11 = v.dP[5];
   const double fx = p00 * (double)v.vpw * 0.5, fy = p11 * (double)v.vph * 0.5;
   double j00 = fx, j11 = fy, j02 = 0, j12 = 0;
   if (v.persp) {
@@ -143,8 +143,8 @@ __device__ __noinline__ double canonical_value(const ViewK &v, uint4 a, uint4 b,
   double t0[3], t1[3], u0[3], u1[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    t0[c] = j00 * (double)v.mv[c * 4 + 0] + j02 * (double)v.mv[c * 4 + 2];
-    t1[c] = j11 * (double)v.mv[c * 4 + 1] + j12 * (double)v.mv[c * 4 + 2];
+    t0[c] = j00 * v.dmv[c * 4 + 0] + j02 * v.dmv[c * 4 + 2];
+    t1[c] = j11 * v.dmv[c * 4 + 1] + j12 * v.dmv[c * 4 + 2];
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -181,22 +181,22 @@ __device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 
   o.xx = o.xy = o.yy = 0.f;
   const double x = f32_of(a.x), y = f32_of(a.y), z = f32_of(a.z);
   // view = modelView * (p, 1); each product is exact, so fma is the sum as written.
-  const double vx = fma((double)v.mv[8], z, fma((double)v.mv[4], y, (double)v.mv[0] * x)) + (double)v.mv[12];
-  const double vy = fma((double)v.mv[9], z, fma((double)v.mv[5], y, (double)v.mv[1] * x)) + (double)v.mv[13];
-  const double vz = fma((double)v.mv[10], z, fma((double)v.mv[6], y, (double)v.mv[2] * x)) + (double)v.mv[14];
+  const double vx = fma(v.dmv[8], z, fma(v.dmv[4], y, v.dmv[0] * x)) + v.dmv[12];
+  const double vy = fma(v.dmv[9], z, fma(v.dmv[5], y, v.dmv[1] * x)) + v.dmv[13];
+  const double vz = fma(v.dmv[10], z, fma(v.dmv[6], y, v.dmv[2] * x)) + v.dmv[14];
   double cx, cy, cz, cw;
   if (STD) {
-    cx = (double)v.P[0] * vx + (double)v.P[8] * vz;
-    cy = (double)v.P[5] * vy + (double)v.P[9] * vz;
-    cz = (double)v.P[14];
+    cx = v.dP[0] * vx + v.dP[8] * vz;
+    cy = v.dP[5] * vy + v.dP[9] * vz;
+    cz = v.dP[14];
     cw = -vz;
   } else {
-    const double vw = fma((double)v.mv[11], z, fma((double)v.mv[7], y, (double)v.mv[3] * x)) + (double)v.mv[15];
+    const double vw = fma(v.dmv[11], z, fma(v.dmv[7], y, v.dmv[3] * x)) + v.dmv[15];
     const double vv[4] = {vx, vy, vz, vw};
     double c[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      c[r] = (double)v.P[r] * vv[0] + (double)v.P[4 + r] * vv[1] + (double)v.P[8 + r] * vv[2] + (double)v.P[12 + r] * vv[3];
+      c[r] = v.dP[r] * vv[0] + v.dP[4 + r] * vv[1] + v.dP[8 + r] * vv[2] + v.dP[12 + r] * vv[3];
     cx = c[0];
     cy = c[1];
     cz = c[2];
@@ -268,11 +268,11 @@ __device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 
     if (!(val - v.cull > eval)) {
       if (v.cull - val > eval) return o;
       o.recomputed = true;
-      if (canonical_value(v, a, b, vx, vy, vz) < (double)v.cull) return o;
+      if (canonical_value(v, a, b, vx, vy, vz) < v.dcull) return o;
     }
   }
   // Far first: the farthest key is the smallest (src/render/splats.js:179-181).
-  double t = (-vz - (double)v.dmin) * (double)v.dscale;
+  double t = (-vz - v.ddmin) * v.ddscale;
   t = t < 0.0 ? 0.0 : t;
   t = t > 65535.0 ? 65535.0 : t;
   o.key = 65535u - (uint32_t)t;
@@ -301,13 +301,24 @@ __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, 
   const float bw2 = fabsf(v.mv[2]) + fabsf(v.mv[6]) + fabsf(v.mv[10]);
   uint32_t recomp = 0;
   const uint32_t ntiles = (n + 255u) >> 8;
+  // Records of the next tile are loaded before this one is worked on, so a
+  // wave always has a tile's loads in flight behind its arithmetic.
+  uint4 na = make_uint4(0, 0, 0, 0), nb = make_uint4(0, 0, 0, 0);
+  {
+    const uint32_t i0 = (blockIdx.x << 8) + tid;
+    if (i0 < n) {
+      na = rec[2 * (size_t)i0];
+      nb = rec[2 * (size_t)i0 + 1];
+    }
+  }
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint32_t i = (tile << 8) + tid;
     const bool in = i < n;
-    uint4 a = make_uint4(0, 0, 0, 0), b = make_uint4(0, 0, 0, 0);
-    if (in) {
-      a = rec[2 * (size_t)i];
-      b = rec[2 * (size_t)i + 1];
+    const uint4 a = na, b = nb;
+    const uint32_t inext = ((tile + gridDim.x) << 8) + tid;
+    if (tile + gridDim.x < ntiles && inext < n) {
+      na = rec[2 * (size_t)inext];
+      nb = rec[2 * (size_t)inext + 1];
     }
     Projected p;
     p.keep = false;
@@ -389,28 +400,36 @@ __global__ __launch_bounds__(256) void k_scatter(const uint16_t *__restrict__ ke
   const uint32_t wbase = tile0 + w * (64u * KPT);
   const uint32_t nwords = (n + 63u) >> 6;
 
+  // Every load of the wave's 16 rounds is issued before any is used.
+  uint32_t raw[KPT];
+  uint64_t vmask[KPT];
+  const uint32_t wave0 = tile0 + (uint32_t)__builtin_amdgcn_readfirstlane(w) * (64u * KPT);
+#pragma unroll
+  for (int r = 0; r < KPT; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+    if (HI) {
+      raw[r] = i < limit ? in_packed[i] : 0u;
+    } else {
+      raw[r] = i < n ? (uint32_t)keys[i] : 0u;
+      const uint32_t wi = (wave0 + (uint32_t)r * 64u) >> 6;
+      vmask[r] = wi < nwords ? mask[wi] : 0ull;
+    }
+  }
   uint32_t dig[KPT], rank[KPT], val[KPT];
   uint32_t okbits = 0;
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t i = wbase + (uint32_t)r * 64u + lane;
     uint64_t valid;
-    uint32_t d = 0, value = 0;
+    uint32_t d, value;
     if (HI) {
       valid = __ballot(i < limit);
-      if (i < limit) {
-        value = in_packed[i];
-        d = value >> 24;
-        value &= 0xFFFFFFu;
-      }
+      d = raw[r] >> 24;
+      value = raw[r] & 0xFFFFFFu;
     } else {
-      const uint32_t wi = (wbase + (uint32_t)r * 64u) >> 6;
-      valid = wi < nwords ? mask[wi] : 0ull;
-      if ((valid >> lane) & 1ull) {
-        const uint32_t key = keys[i];
-        d = key & 255u;
-        value = ((key >> 8) << 24) | i;
-      }
+      valid = vmask[r];
+      d = raw[r] & 255u;
+      value = ((raw[r] >> 8) << 24) | i;
     }
     const bool ok = (valid >> lane) & 1ull;
     const uint64_t peers = match_digit(d, valid);
@@ -522,9 +541,9 @@ __global__ __launch_bounds__(256) void k_canonical(const uint4 *__restrict__ rec
   if (i >= n) return;
   const uint4 a = rec[2 * (size_t)i], b = rec[2 * (size_t)i + 1];
   const double x = f32_of(a.x), y = f32_of(a.y), z = f32_of(a.z);
-  const double vx = fma((double)v.mv[8], z, fma((double)v.mv[4], y, (double)v.mv[0] * x)) + (double)v.mv[12];
-  const double vy = fma((double)v.mv[9], z, fma((double)v.mv[5], y, (double)v.mv[1] * x)) + (double)v.mv[13];
-  const double vz = fma((double)v.mv[10], z, fma((double)v.mv[6], y, (double)v.mv[2] * x)) + (double)v.mv[14];
+  const double vx = fma(v.dmv[8], z, fma(v.dmv[4], y, v.dmv[0] * x)) + v.dmv[12];
+  const double vy = fma(v.dmv[9], z, fma(v.dmv[5], y, v.dmv[1] * x)) + v.dmv[13];
+  const double vz = fma(v.dmv[10], z, fma(v.dmv[6], y, v.dmv[2] * x)) + v.dmv[14];
   double *o = out5 + 5 * (size_t)i;
   o[0] = 0;
   o[1] = __longlong_as_double(0x7FF8000000000000ll);
@@ -532,13 +551,13 @@ __global__ __launch_bounds__(256) void k_canonical(const uint4 *__restrict__ rec
   o[3] = 0;
   o[4] = 0;
   const double cw = -vz;
-  const double cx = (double)v.P[0] * vx + (double)v.P[8] * vz;
-  const double cy = (double)v.P[5] * vy + (double)v.P[9] * vz;
-  const double cz = (double)v.P[14];
+  const double cx = v.dP[0] * vx + v.dP[8] * vz;
+  const double cy = v.dP[5] * vy + v.dP[9] * vz;
+  const double cz = v.dP[14];
   if (vz >= 0 || cz > cw || fabs(cx) > kC13 * cw || fabs(cy) > kC13 * cw || (b.z >> 24) <= 1u) return;
   o[0] = 1;
   o[1] = canonical_value(v, a, b, vx, vy, vz);
-  double t = (-vz - (double)v.dmin) * (double)v.dscale;
+  double t = (-vz - v.ddmin) * v.ddscale;
   t = t < 0.0 ? 0.0 : t;
   t = t > 65535.0 ? 65535.0 : t;
   o[2] = (double)(65535u - (uint32_t)t);
@@ -590,6 +609,13 @@ ViewK make_viewk(const wd_view &v) {
   k.n0 = 0;
   k.n1 = 0;
   k.persp = v.projection[15] == 0.0f ? 1 : 0;
+  for (int i = 0; i < 16; ++i) {
+    k.dmv[i] = (double)v.model_view[i];
+    k.dP[i] = (double)v.projection[i];
+  }
+  k.ddmin = (double)v.depth_min;
+  k.ddscale = (double)v.depth_scale;
+  k.dcull = (double)v.cull;
   return k;
 }
 

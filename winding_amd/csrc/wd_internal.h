@@ -52,6 +52,9 @@ static_assert(sizeof(Ctl) % 256 == 0, "Ctl keeps 256-byte alignment");
 // The per-view constants a kernel takes by value: the uniform plus what the
 // f32 fast path and its error bound precompute once per view.
 struct ViewK {
+  double dmv[16];     // the uniform's matrices widened once on the host, so the
+  double dP[16];      // kernels read them as scalars instead of converting per lane
+  double ddmin, ddscale, dcull;
   float mv[16];
   float P[16];
   float vpw, vph;

@@ -1,0 +1,296 @@
+'use strict';
+// Gaussian splats on an MI355X: the splat module's JavaScript API over the HIP path.
+//
+// The shapes follow the reference (Winding v1.9.0):
+//   parseSplats(bytes)             src/scene/splats.js:60-76,373-388 (the .splat path)
+//   new Splats(gpu, data) / destroy src/render/splats.js:389-416
+//   SplatPass.prepare(...)          src/render/splats.js:546-618
+//   SplatPass sort                  _encodeSort, src/render/splats.js:676-697
+// What runs where differs: the records go to the GPU as they are (32 bytes a splat) and the
+// covariance put() worked out on the CPU is worked out there, in the projection kernel; the
+// sort is stable (key, then splat index); the survivors' order, count and footprints can be
+// read back into typed arrays, or left on the GPU for whatever draws them.
+const path = require('path');
+
+const SPLAT_BYTES = 32;
+const BUCKETS = 65536;
+/** Opacity x pixels a splat must add to be drawn (src/render/splats.js:78). */
+const SPLAT_CULL = 0.5;
+/** modelView, projection, viewport, depth range, cull: what a sort depends on. */
+const CLOUD_FLOATS = 37;
+/** Frames a node's sort buffers outlive the last frame that drew it (src/render/splats.js:70). */
+const KEEP_FRAMES = 120;
+
+let nativeAddon = null;
+/** The N-API addon; throws when it is not built (there is no CPU fallback). */
+function native() {
+  if (nativeAddon === null) {
+    const file = path.join(__dirname, '..', 'winding_amd.node');
+    try {
+      nativeAddon = require(file);
+    } catch (error) {
+      throw new Error('winding_amd: the addon ' + file + ' is not built (python -m winding_amd.build): ' + error.message);
+    }
+  }
+  return nativeAddon;
+}
+
+/** The GPU: one HIP device and its stream, the reference's rhi for this path. */
+class Gpu {
+  constructor(index, addon) {
+    this.native = addon || native();
+    this.handle = this.native.open(index === undefined ? 0 : index);
+  }
+
+  static open(index) {
+    return new Gpu(index);
+  }
+
+  sync() {
+    this.native.sync(this.handle);
+  }
+
+  destroy() {
+    if (this.handle !== null) this.native.close(this.handle);
+    this.handle = null;
+  }
+}
+
+const isZip = (d) => d[0] === 0x50 && d[1] === 0x4b && d[2] === 0x03 && d[3] === 0x04;
+
+/**
+ * A capture's splats from the bytes of a .splat file: { count, degree, records, min, max }.
+ * `records` are the bytes themselves, 32 a splat; min and max are filled in when the capture
+ * is uploaded (new Splats), where the GPU checks every position.
+ */
+function parseSplats(bytes) {
+  const data = bytes instanceof Uint8Array ? bytes : new Uint8Array(bytes);
+  const isPly = data.length >= 4 && data[0] === 0x70 && data[1] === 0x6c && data[2] === 0x79 && (data[3] === 0x0a || data[3] === 0x0d);
+  if (isPly) throw new Error('splats: a .ply capture is not read here; this path takes .splat records');
+  if (isZip(data)) throw new Error('splats: this is a zip: unpack it with readSplats, which reads a .sog');
+  if (data[0] === 0x1f && data[1] === 0x8b) throw new Error('splats: this is gzipped: unpack it with readSplats, which reads a .spz');
+  if (data.length === 0 || data.length % SPLAT_BYTES !== 0) {
+    throw new Error('splats: not a .ply, and ' + data.length + ' bytes is not a whole number of 32-byte .splat records');
+  }
+  return { count: data.length / SPLAT_BYTES, degree: 0, records: data, min: null, max: null };
+}
+
+/**
+ * A capture on the GPU. `count` splats; `min` and `max`, the corners of the box around
+ * their centres. Refused by name when a position is not a number or the capture is larger
+ * than one sort takes.
+ */
+class Splats {
+  constructor(gpu, data, label) {
+    if (data.degree > 0) throw new Error('splats: harmonics of degree ' + data.degree + ' are not drawn by this path');
+    const made = gpu.native.cloudCreate(gpu.handle, data.records);
+    this.gpu = gpu;
+    this.label = label === undefined ? 'splats' : label;
+    this.handle = made.handle;
+    this.count = made.count;
+    this.degree = 0;
+    this.min = made.min;
+    this.max = made.max;
+    this.unloaded = false;
+  }
+
+  destroy() {
+    if (!this.unloaded) this.gpu.native.destroy(this.handle);
+    this.unloaded = true;
+  }
+}
+
+/** The scene records prepare iterates: entity -> { splats, world }, as scene.splats holds them. */
+class SplatScene {
+  constructor() {
+    this.splats = new Map();
+    this.hidden = null;
+    this._next = 1;
+  }
+
+  addSplats(options) {
+    const o = options || {};
+    const splats = o.splats;
+    if (!splats || typeof splats.count !== 'number' || !splats.handle) {
+      throw new Error('addSplats: splats must be what new Splats returned');
+    }
+    const world = new Float32Array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]);
+    if (o.matrix) world.set(o.matrix);
+    if (o.position) { world[12] = o.position[0]; world[13] = o.position[1]; world[14] = o.position[2]; }
+    const entity = this._next++;
+    this.splats.set(entity, { splats, world });
+    return entity;
+  }
+
+  splatsOf(entity) {
+    const r = this.splats.get(entity);
+    return r === undefined ? null : r;
+  }
+
+  remove(entity) {
+    this.splats.delete(entity);
+  }
+}
+
+function sameFloats(a, b, n) {
+  for (let i = 0; i < n; i++) if (a[i] !== b[i]) return false;
+  return true;
+}
+
+/**
+ * The per-cloud uniform prepare() writes (src/render/splats.js:559-587): modelView = view x world,
+ * the depth the box's eight corners span, depthMin and depthScale; the first 37 floats of Cloud.
+ * Null when the cloud is wholly behind the camera.
+ */
+function cloudUniform(out, view, world, projection, width, height, min, max, cull) {
+  const mv = out.subarray(0, 16);
+  for (let c = 0; c < 4; c++) {
+    const b0 = world[c * 4], b1 = world[c * 4 + 1], b2 = world[c * 4 + 2], b3 = world[c * 4 + 3];
+    for (let r = 0; r < 4; r++) mv[c * 4 + r] = b0 * view[r] + b1 * view[4 + r] + b2 * view[8 + r] + b3 * view[12 + r];
+  }
+  let near = Infinity;
+  let far = -Infinity;
+  for (let k = 0; k < 8; k++) {
+    const x = k & 1 ? max[0] : min[0];
+    const y = k & 2 ? max[1] : min[1];
+    const z = k & 4 ? max[2] : min[2];
+    const depth = -(mv[2] * x + mv[6] * y + mv[10] * z + mv[14]);
+    if (depth < near) near = depth;
+    if (depth > far) far = depth;
+  }
+  if (far <= 0) return null;
+  near = Math.max(near, 0);
+  out.set(projection, 16);
+  out[32] = width;
+  out[33] = height;
+  out[34] = near;
+  out[35] = (BUCKETS - 1) / Math.max(far - near, 1e-6);
+  out[36] = cull;
+  return out;
+}
+
+class SplatPass {
+  constructor(gpu) {
+    this.gpu = gpu;
+    this.native = gpu.native;
+    /** Each placed cloud's sort buffers, by its scene record. */
+    this._states = new Map();
+    this._items = [];
+    this._frame = 0;
+    /** Splats in the clouds the last frame drew, before culling. */
+    this.count = 0;
+    /** Clouds the last prepare marked for sorting; the rest kept the order they had. */
+    this.sorted = 0;
+    /** The least a splat may add to the picture, opacity x pixels, and be drawn. */
+    this.cull = SPLAT_CULL;
+    /** The projection without TAA's jitter, when jittered: what the sort is decided by. */
+    this.unjittered = null;
+    this._key = new Float32Array(CLOUD_FLOATS);
+  }
+
+  /**
+   * This frame's clouds: each one whose camera, transform or canvas moved has its uniform
+   * written, to be sorted again by sort(). Returns how many clouds there are.
+   */
+  prepare(scene, camera, environment, width, height) {
+    const cull = this.cull;
+    if (!(cull >= 0 && Number.isFinite(cull))) throw new Error('splatCull must be 0 or more, got ' + cull);
+    this._frame++;
+    const items = this._items;
+    items.length = 0;
+    this.count = 0;
+    this.sorted = 0;
+    for (const [entity, record] of scene.splats) {
+      if (scene.hidden && scene.hidden[entity] === 1) continue;
+      const splats = record.splats;
+      if (splats.unloaded) throw new Error('addSplats: these splats were unloaded; remove the node first');
+      const state = this._stateFor(record, splats, scene, entity);
+      state.lastFrame = this._frame;
+      const f = cloudUniform(state.uniform, camera.view, record.world, camera.projection, width, height, splats.min, splats.max, cull);
+      if (f === null) continue;   // wholly behind the camera
+      const key = this._key;
+      key.set(f.subarray(0, CLOUD_FLOATS));
+      if (this.unjittered !== null) key.set(this.unjittered, 16);
+      const sort = !sameFloats(state.last, key, CLOUD_FLOATS) || state.cull !== cull;
+      state.cull = cull;
+      if (sort) {
+        state.next.set(key);
+        this.sorted++;
+      }
+      const m = f;
+      const cx = (splats.min[0] + splats.max[0]) / 2, cy = (splats.min[1] + splats.max[1]) / 2, cz = (splats.min[2] + splats.max[2]) / 2;
+      items.push({ state, entity, sort, count: splats.count, depth: -(m[2] * cx + m[6] * cy + m[10] * cz + m[14]) });
+      this.count += splats.count;
+    }
+    items.sort((a, b) => b.depth - a.depth);
+    for (const [record, state] of this._states) {
+      const removed = state.scene === scene && scene.splats.get(state.entity) !== record;
+      if (removed || state.lastFrame < this._frame - KEEP_FRAMES) {
+        this._destroyState(state);
+        this._states.delete(record);
+      }
+    }
+    return items.length;
+  }
+
+  /** Sorts on the GPU every cloud prepare marked; returns how many. Enqueued, not waited for. */
+  sort() {
+    let sorted = 0;
+    for (const item of this._items) {
+      if (!item.sort) continue;
+      const state = item.state;
+      state.last.set(state.next);
+      this.native.sort(this.gpu.handle, state.splats.handle, state.uniform, state.handle);
+      state.sortedOnce = true;
+      sorted++;
+    }
+    return sorted;
+  }
+
+  /** The clouds of the last prepare, far first: [{ entity, count }]. */
+  clouds() {
+    return this._items.map((i) => ({ entity: i.entity, count: i.count }));
+  }
+
+  /**
+   * Waits for entity's last sort and reads it back: { order (Uint32Array, back to front),
+   * count, recomputed, cov2d (Float32Array, 4 a splat by index: xx, xy, yy, rgba bits) }.
+   */
+  read(entity, options) {
+    const item = this._items.find((i) => i.entity === entity);
+    if (item === undefined || !item.state.sortedOnce) throw new Error('splats: entity ' + entity + ' has not been sorted');
+    const wantCov = options && options.cov2d;
+    return this.native.stateRead(this.gpu.handle, item.state.handle, wantCov ? item.count : 0);
+  }
+
+  _stateFor(record, splats, scene, entity) {
+    let state = this._states.get(record);
+    if (state !== undefined && state.splats === splats) return state;
+    if (state !== undefined) this._destroyState(state);
+    const last = new Float32Array(CLOUD_FLOATS);
+    last.fill(NaN);   // NaN matches nothing, so the first frame sorts
+    state = {
+      splats, scene, entity, lastFrame: 0, last, next: new Float32Array(CLOUD_FLOATS),
+      uniform: new Float32Array(CLOUD_FLOATS), cull: undefined, sortedOnce: false,
+      handle: this.native.stateCreate(this.gpu.handle, Math.max(1, splats.count)),
+    };
+    this._states.set(record, state);
+    return state;
+  }
+
+  _destroyState(state) {
+    if (state.handle !== null) this.native.destroy(state.handle);
+    state.handle = null;
+  }
+
+  destroy() {
+    for (const state of this._states.values()) this._destroyState(state);
+    this._states.clear();
+    this._items.length = 0;
+  }
+}
+
+module.exports = {
+  SPLAT_BYTES, BUCKETS, SPLAT_CULL, CLOUD_FLOATS, KEEP_FRAMES,
+  native, Gpu, parseSplats, Splats, SplatScene, SplatPass, cloudUniform,
+};
