@@ -50,7 +50,18 @@ def main():
     ap.add_argument("--workload", default="slab-8m", choices=["plain-64k", "plain-1m", "slab-8m"])
     ap.add_argument("--cull", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="time a plain 512 MiB device copy instead (HBM yardstick for the traffic counters)")
     args = ap.parse_args()
+    if args.calibrate:
+        import winding_amd as wa
+        ctx = wa.Context(0)
+        nbytes = 512 << 20
+        ms = ctx.copy_ms(nbytes, reps=args.steps)
+        ctx.close()
+        print(json.dumps({"calibration": "device copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "ms_mean": ms, "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return 0
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not run (each box has one MI355X, "
               "and the multi-GPU split is a design note, not code)" % args.gpus, file=sys.stderr)

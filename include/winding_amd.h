@@ -187,6 +187,33 @@ int wd_profile(wd_context *ctx, int on);
 int wd_profile_read(wd_context *ctx, const char **names, double *ms,
                     uint64_t *launches, int capacity, int *n);
 
+/* ---- batches: many (cloud, view) sorts in one persistent launch ------- */
+
+/* One sort of a batch: what wd_sort takes. No two jobs may share a state. */
+typedef struct wd_job {
+  const wd_cloud *cloud;
+  wd_view view;
+  wd_state *state;
+} wd_job;
+
+/*
+ * A batch replaces the reference's per-cloud loop of dispatches in
+ * _encodeSort (src/render/splats.js:676-697): every job's projection and both
+ * scatter passes run in ONE persistent kernel, taken tile by tile from a
+ * ticketed queue, replayed from a hipGraph.
+ */
+int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out);
+/*
+ * Sets the jobs (views may change between runs at no cost; other clouds or
+ * states re-capture the graph on the next run).
+ */
+int wd_batch_set(wd_batch *batch, const wd_job *jobs, uint32_t n);
+/* Enqueues one run of every job; results land in each job's state. */
+int wd_batch_run(wd_context *ctx, wd_batch *batch);
+/* Waits, then reports whether a bounded wait in the kernel ran out (a bug). */
+int wd_batch_stuck(wd_context *ctx, wd_batch *batch, uint32_t *stuck);
+int wd_batch_destroy(wd_batch *batch);
+
 /* ---- synthetic inputs: the workloads' seeded generator ---------------- */
 
 /*
@@ -212,6 +239,12 @@ int wd_debug_canonical(wd_context *ctx, const wd_cloud *cloud, const wd_view *vi
  */
 int wd_debug_selftest(wd_context *ctx, const double *args, const double *expect,
                       uint32_t count, uint32_t mismatch3[3]);
+/*
+ * A plain device-to-device copy of `bytes` (read once, written once), `reps`
+ * times; *ms_mean gets the mean duration from HIP events. What HBM sustains,
+ * and the yardstick the FETCH_SIZE / WRITE_SIZE counters are scaled by.
+ */
+int wd_debug_copy(wd_context *ctx, uint64_t bytes, int reps, double *ms_mean);
 
 #ifdef __cplusplus
 }

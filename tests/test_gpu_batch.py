@@ -1,0 +1,85 @@
+"""The batch path (one persistent launch over many (cloud, view) jobs, replayed from a hipGraph) against
+the CPU oracle, job by job: order bit-exact, survivor count, cov2d within 1e-5."""
+import numpy as np
+import pytest
+
+import oracle
+import winding_amd as wa
+from winding_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def check_job(state, recs, view, label):
+    order, cov, stats = state.read(n_cov=len(recs))
+    o_order, _, keep, o_cov = oracle.run(recs, view)
+    assert stats.count == len(o_order), label
+    assert np.array_equal(order, o_order), "%s: order differs" % label
+    if keep.any():
+        g, w = cov[keep, :3].astype(np.float64), o_cov[keep]
+        err = np.max(np.abs(g - w).max(axis=1) / np.maximum(np.abs(w[:, 0]), np.abs(w[:, 2])))
+        assert err <= 1e-5, "%s: cov2d error %g" % (label, err)
+
+
+def test_batch_uneven_jobs_and_replays(ctx):
+    sizes = [1, 100, 4096, 5000, 70001, 300000, 12289, 65536]
+    st = wl.Stream(11)
+    jobs, data = [], []
+    for j, n in enumerate(sizes):
+        recs = wa.synth_records(j % 2, 100 + j, n)
+        cloud = wa.Cloud(ctx, recs)
+        state = wa.State(ctx, n)
+        view = wl.view_for(recs, wl.orbit_pose(st), 1280, 720)
+        if view is None:
+            view = wl.view_for(recs, (0.0, 0.0, 10.0), 1280, 720)
+        jobs.append((cloud, view, state))
+        data.append(recs)
+    batch = wa.Batch(ctx, len(jobs))
+    try:
+        batch.set(jobs)
+        for replay in range(3):
+            if replay == 2:  # new views, same clouds and states: the captured graph is replayed
+                jobs = [(c, wl.view_for(r, wl.orbit_pose(st), 1920, 1080), s) for (c, _, s), r in zip(jobs, data)]
+                batch.set(jobs)
+            batch.run()
+            ctx.sync()
+            assert batch.stuck() == 0
+            print("replay %d ok" % replay, flush=True)
+            for (cloud, view, state), recs in zip(jobs, data):
+                check_job(state, recs, view, "replay %d n=%d" % (replay, len(recs)))
+        # a single-view sort on a state a batch used: epochs do not collide
+        cloud, view, state = jobs[4]
+        state.sort(cloud, view)
+        check_job(state, data[4], view, "single after batch")
+    finally:
+        batch.destroy()
+        for cloud, _, state in jobs:
+            cloud.destroy()
+            state.destroy()
+
+
+def test_batch_64_scenes_32m(ctx):
+    seed, sizes = wl.batch_sizes()
+    st = wl.Stream(seed)
+    jobs, data = [], []
+    try:
+        for j, n in enumerate(sizes):
+            recs = wa.synth_records(0, 1000 + j, n)
+            view = None
+            while view is None:
+                view = wl.view_for(recs, wl.orbit_pose(st), 1280, 720)
+            jobs.append((wa.Cloud(ctx, recs), view, wa.State(ctx, n)))
+            data.append(recs)
+        batch = wa.Batch(ctx, 64)
+        batch.set(jobs)
+        batch.run()
+        ctx.sync()
+        assert batch.stuck() == 0
+        for (cloud, view, state), recs in zip(jobs, data):
+            check_job(state, recs, view, "scene n=%d" % len(recs))
+        print("64 scenes, %d splats" % sum(sizes), flush=True)
+        batch.destroy()
+    finally:
+        for cloud, _, state in jobs:
+            cloud.destroy()
+            state.destroy()

@@ -26,7 +26,8 @@ EXPORTS = [
     "wd_cloud_create", "wd_cloud_create_device", "wd_cloud_info", "wd_cloud_records", "wd_cloud_destroy",
     "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
     "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_profile", "wd_profile_read",
-    "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest",
+    "wd_batch_create", "wd_batch_set", "wd_batch_run", "wd_batch_stuck", "wd_batch_destroy",
+    "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest", "wd_debug_copy",
 ]
 
 
@@ -43,6 +44,11 @@ class View(C.Structure):
 
     def as_array(self):
         return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+
+class Job(C.Structure):
+    """wd_job: one (cloud, view, state) of a batch."""
+    _fields_ = [("cloud", C.c_void_p), ("view", View), ("state", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -91,9 +97,15 @@ def lib():
             "wd_profile": (I, [P, I]),
             "wd_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(U64), I,
                                     C.POINTER(I)]),
+            "wd_batch_create": (I, [P, U32, C.POINTER(P)]),
+            "wd_batch_set": (I, [P, C.POINTER(Job), U32]),
+            "wd_batch_run": (I, [P, P]),
+            "wd_batch_stuck": (I, [P, P, C.POINTER(U32)]),
+            "wd_batch_destroy": (I, [P]),
             "wd_synth_records": (I, [I, U64, U64, P]),
             "wd_debug_canonical": (I, [P, P, C.POINTER(View), P]),
             "wd_debug_selftest": (I, [P, P, P, U32, P]),
+            "wd_debug_copy": (I, [P, U64, I, C.POINTER(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -195,6 +207,12 @@ class Context:
         check(lib().wd_profile_read(self.ptr, names, ms, launches, 16, C.byref(n)))
         return {names[i].decode(): (ms[i], launches[i]) for i in range(n.value) if launches[i]}
 
+    def copy_ms(self, nbytes, reps=20):
+        """Mean ms of a plain device copy of nbytes (one read stream, one write stream)."""
+        ms = C.c_double()
+        check(lib().wd_debug_copy(self.ptr, nbytes, reps, C.byref(ms)))
+        return ms.value
+
     def selftest(self, args, expect):
         args = np.ascontiguousarray(args, np.float64)
         expect = np.ascontiguousarray(expect, np.float64)
@@ -271,6 +289,12 @@ class State:
                                   cov.ctypes.data if n_cov else None, n_cov, C.byref(s)))
         return order[:s.count], cov[:n_cov], s
 
+    def buffers(self):
+        """Device addresses (order, count, cov2d)."""
+        o, c, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().wd_state_buffers(self.ptr, C.byref(o), C.byref(c), C.byref(v)))
+        return o.value, c.value, v.value
+
     def read_keys(self, n):
         keys = np.empty(n, np.uint16)
         mask = np.empty((n + 63) // 64, np.uint64)
@@ -280,3 +304,35 @@ class State:
 
 
 from . import workloads  # noqa: E402  (needs the names above)
+
+
+class Batch:
+    """Many (cloud, view, state) sorts in one persistent launch, replayed from a hipGraph (wd_batch_*)."""
+
+    def __init__(self, ctx, max_jobs):
+        p = C.c_void_p()
+        check(lib().wd_batch_create(ctx.ptr, max_jobs, C.byref(p)))
+        self.ctx = ctx
+        self.ptr = p
+
+    def set(self, jobs):
+        """jobs: [(Cloud, View, State)]."""
+        arr = (Job * len(jobs))()
+        for i, (cloud, view, state) in enumerate(jobs):
+            arr[i].cloud = cloud.ptr.value
+            arr[i].view = view
+            arr[i].state = state.ptr.value
+        check(lib().wd_batch_set(self.ptr, arr, len(jobs)))
+
+    def run(self):
+        check(lib().wd_batch_run(self.ctx.ptr, self.ptr))
+
+    def stuck(self):
+        s = C.c_uint32()
+        check(lib().wd_batch_stuck(self.ctx.ptr, self.ptr, C.byref(s)))
+        return s.value
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().wd_batch_destroy(self.ptr))
+            self.ptr = None

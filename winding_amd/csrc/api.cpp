@@ -341,9 +341,14 @@ int wd_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_stat
 int wd_state_stats(wd_context *ctx, wd_state *s, wd_stats *stats) {
   if (!ctx || !s || s->magic != 0x53544154 || !stats) return fail(WD_E_ARG, "stats: null argument or destroyed state");
   wd::Ctl head;
-  WD_HIP(hipMemcpyAsync(&head.count, &s->ctl->count, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+  WD_HIP(hipMemcpyAsync(&head.count, &s->ctl->count, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
          "stats: read");
   WD_HIP(hipStreamSynchronize(ctx->stream), "stats: sync");
+  if (head.stuck) {
+    const uint32_t zero = 0;
+    hipMemcpy(&s->ctl->stuck, &zero, sizeof zero, hipMemcpyHostToDevice);
+    return fail(WD_E_STATE, "sort: a look-back wait ran out; the result is not valid");
+  }
   stats->count = head.count;
   stats->recomputed = head.recomputed;
   stats->fallback = head.fallback;
@@ -420,6 +425,33 @@ int wd_debug_canonical(wd_context *ctx, const wd_cloud *cloud, const wd_view *vi
     if (e != hipSuccess) rc = wd::hip_fail(e, "canonical debug: read");
   }
   hipFree(dev);
+  return rc;
+}
+
+int wd_debug_copy(wd_context *ctx, uint64_t bytes, int reps, double *ms_mean) {
+  if (!ctx || !ms_mean || bytes < 16 || reps < 1) return fail(WD_E_ARG, "copy: bad argument");
+  void *src = nullptr, *dst = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipMalloc(&src, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dst, bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(src, 1, bytes, ctx->stream);
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  int rc = e == hipSuccess ? wd::launch_copy(ctx, src, dst, bytes) : wd::hip_fail(e, "copy: setup");
+  if (rc == WD_OK) {
+    e = hipEventRecord(a, ctx->stream);
+    for (int r = 0; r < reps && rc == WD_OK; ++r) rc = wd::launch_copy(ctx, src, dst, bytes);
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (e != hipSuccess && rc == WD_OK) rc = wd::hip_fail(e, "copy: timing");
+    *ms_mean = ms / reps;
+  }
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  hipFree(src);
+  hipFree(dst);
   return rc;
 }
 

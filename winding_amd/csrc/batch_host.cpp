@@ -1,0 +1,146 @@
+// Host side of batches (wd_batch_*): the job table, the ticket queue's item
+// counts, and the hipGraph the persistent batch kernel is replayed from.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "wd_internal.h"
+
+using wd::fail;
+
+extern "C" {
+
+int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out) {
+  if (!ctx || !out || max_jobs == 0) return fail(WD_E_ARG, "batch: null argument or no jobs");
+  *out = nullptr;
+  WD_HIP(hipSetDevice(ctx->device), "batch: set device");
+  wd_batch *b = new wd_batch();
+  b->ctx = ctx;
+  b->max_jobs = max_jobs;
+  const size_t ctl_bytes = sizeof(wd::BatchCtl) + 2 * sizeof(uint32_t) * max_jobs;
+  hipError_t e = hipMalloc(&b->jobs, sizeof(wd::BatchJob) * max_jobs);
+  if (e == hipSuccess) e = hipMalloc(&b->starts, sizeof(uint32_t) * 3 * (max_jobs + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->ctl, ctl_bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(b->ctl, 0, ctl_bytes, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    wd_batch_destroy(b);
+    return wd::hip_fail(e, "batch: allocation");
+  }
+  *out = b;
+  return WD_OK;
+}
+
+int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
+  if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
+  if (!jobs || n == 0 || n > b->max_jobs) return fail(WD_E_ARG, "batch: %u jobs, this batch holds 1 to %u", n, b->max_jobs);
+  std::vector<wd::BatchJob> hj(n);
+  std::vector<uint32_t> a(n + 1, 0), t(n + 1, 0);
+  std::vector<const void *> shape;
+  for (uint32_t i = 0; i < n; ++i) {
+    const wd_job &J = jobs[i];
+    if (!J.cloud || J.cloud->magic != 0x53504c54) return fail(WD_E_STATE, "batch: job %u: its splats were unloaded", i);
+    if (!J.state || J.state->magic != 0x53544154) return fail(WD_E_STATE, "batch: job %u: its sort state was destroyed", i);
+    if (J.cloud->ctx != b->ctx || J.state->ctx != b->ctx) return fail(WD_E_STATE, "batch: job %u is of another device", i);
+    if (J.cloud->n > J.state->capacity)
+      return fail(WD_E_CAPACITY, "batch: job %u: %llu splats are more than its state holds, %llu", i,
+                  (unsigned long long)J.cloud->n, (unsigned long long)J.state->capacity);
+    if (!(J.view.cull >= 0 && J.view.cull <= 3.4e38f)) return fail(WD_E_ARG, "splatCull must be 0 or more, got %g", (double)J.view.cull);
+    for (uint32_t k = 0; k < i; ++k)
+      if (jobs[k].state == J.state) return fail(WD_E_ARG, "batch: jobs %u and %u share a sort state", k, i);
+    wd::BatchJob &B = hj[i];
+    B.view = wd::make_viewk(J.view);
+    B.std_form = wd::is_std(J.view) ? 1u : 0u;
+    B.rec = J.cloud->records;
+    B.n = (uint32_t)J.cloud->n;
+    wd_state *s = J.state;
+    B.keys = s->keys;
+    B.mask = s->mask;
+    B.cov = s->cov;
+    B.packed = s->packed;
+    B.order = s->order;
+    B.ctl = s->ctl;
+    B.status_lo = s->status_lo;
+    B.status_hi = s->status_hi;
+    const uint32_t tiles256 = (B.n + 255u) / 256u;
+    B.a_items = (tiles256 + wd::kBatchATiles - 1) / wd::kBatchATiles;
+    B.tiles = (B.n + wd::kTile - 1) / wd::kTile;
+    a[i + 1] = a[i] + B.a_items;
+    t[i + 1] = t[i] + B.tiles;
+    s->last_n = B.n;
+    shape.push_back(J.cloud);
+    shape.push_back(J.state);
+  }
+  std::vector<uint32_t> starts(3 * (n + 1), 0);
+  std::copy(a.begin(), a.end(), starts.begin());
+  std::copy(t.begin(), t.end(), starts.begin() + (n + 1));
+  std::copy(t.begin(), t.end(), starts.begin() + 2 * (n + 1));
+  wd_context *ctx = b->ctx;
+  WD_HIP(hipSetDevice(ctx->device), "batch: set device");
+  // The table is read by the kernel at run time: a later set() with new views
+  // and the same clouds and states replays the same graph.
+  WD_HIP(hipMemcpyAsync(b->jobs, hj.data(), sizeof(wd::BatchJob) * n, hipMemcpyHostToDevice, ctx->stream),
+         "batch: job table");
+  WD_HIP(hipMemcpyAsync(b->starts, starts.data(), sizeof(uint32_t) * starts.size(), hipMemcpyHostToDevice,
+                        ctx->stream), "batch: item table");
+  WD_HIP(hipStreamSynchronize(ctx->stream), "batch: upload");
+  const uint32_t total = a[n] + 2 * t[n];
+  if (shape != b->shape || n != b->n || total != b->total) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  b->host_jobs.swap(hj);
+  b->host_starts.swap(starts);
+  b->shape.swap(shape);
+  b->n = n;
+  b->total = total;
+  return wd::batch_grid(ctx, total, &b->grid);
+}
+
+int wd_batch_run(wd_context *ctx, wd_batch *b) {
+  if (!ctx || !b || b->magic != 0x42415443 || b->ctx != ctx) return fail(WD_E_STATE, "batch: destroyed or of another device");
+  if (b->n == 0) return fail(WD_E_STATE, "batch: no jobs set");
+  WD_HIP(hipSetDevice(ctx->device), "batch: set device");
+  if (ctx->profiling) return wd::launch_batch(ctx, b);  // timed launches go straight to the stream
+  if (!b->exec) {
+    WD_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "batch: capture");
+    int rc = wd::launch_batch(ctx, b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    WD_HIP(e, "batch: end capture");
+    b->graph = g;
+    WD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "batch: instantiate");
+  }
+  WD_HIP(hipGraphLaunch(b->exec, ctx->stream), "batch: graph launch");
+  return WD_OK;
+}
+
+int wd_batch_stuck(wd_context *ctx, wd_batch *b, uint32_t *stuck) {
+  if (!ctx || !b || b->magic != 0x42415443 || !stuck) return fail(WD_E_ARG, "batch: null argument or destroyed");
+  WD_HIP(hipStreamSynchronize(ctx->stream), "batch: sync");
+  WD_HIP(hipMemcpy(stuck, &b->ctl->stuck, sizeof(uint32_t), hipMemcpyDeviceToHost), "batch: read");
+  return WD_OK;
+}
+
+int wd_batch_destroy(wd_batch *b) {
+  if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed twice");
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->jobs) hipFree(b->jobs);
+  if (b->starts) hipFree(b->starts);
+  if (b->ctl) hipFree(b->ctl);
+  b->magic = 0;
+  delete b;
+  return WD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,499 @@
+// Device code of the splat path shared by the single-view kernels (splat_sort.hip)
+// and the batch kernel (batch.hip): numerics, wave primitives, the projection
+// body and the stable scatter body.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "wd_internal.h"
+
+namespace wd {
+namespace {
+
+
+constexpr double kC13 = (double)1.3f;
+constexpr double kC03 = (double)0.3f;
+constexpr double kC2PI = (double)6.2831853f;
+constexpr float kU = 5.9604644775390625e-8f;  // 2^-24, unit roundoff of f32
+
+__device__ __forceinline__ float f32_of(uint32_t bits) { return __uint_as_float(bits); }
+
+// Lanes below this one whose bit is set in m.
+__device__ __forceinline__ uint32_t count_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Inclusive scan over the 64 lanes of a wave on DPP: Hillis-Steele within
+// each row of 16 (row_shr 1, 2, 4, 8), then row 0's total into row 1 and
+// row 2's into row 3 (row_bcast:15), then the first half's into the second
+// (row_bcast:31).
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);
+  return (uint32_t)x;
+}
+
+// Exclusive scan of one value per thread over a 256-thread workgroup.
+// s_w holds 4 words of LDS; *total gets the workgroup's sum.
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t *s_w, uint32_t *total) {
+  const uint32_t inc = wave_scan_incl(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 63) s_w[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t t = s_w[k];
+    off += (k < w) ? t : 0u;
+    sum += t;
+  }
+  *total = sum;
+  __syncthreads();
+  return off + inc - v;
+}
+
+// Lanes of this wave whose 8-bit digit equals this lane's, among `valid`.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
+  uint64_t m = valid;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    m &= bit ? bal : ~bal;
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint64_t load_status(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_status(uint64_t *p, uint32_t epoch, uint32_t flag, uint32_t value) {
+  const uint64_t w = ((uint64_t)((epoch << 2) | flag) << 32) | value;
+  __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// The FP64 definition, for the rare splat the f32 bound cannot decide.
+// put() (src/scene/splats.js:350-368), footprint() (src/render/splats.js:
+// 102-125) and the area test (:174-178), in the oracle's expression order.
+// Returns alpha * area.
+__device__ __forceinline__ double canonical_value(const ViewK &v, uint4 a, uint4 b, double vx, double vy, double vz) {
+  const double sx = f32_of(a.w), sy = f32_of(b.x), sz = f32_of(b.y);
+  const uint32_t q = b.w;
+  const double qw = ((int)(q & 255u) - 128) / 128.0, qx = ((int)((q >> 8) & 255u) - 128) / 128.0;
+  const double qy = ((int)((q >> 16) & 255u) - 128) / 128.0, qz = ((int)(q >> 24) - 128) / 128.0;
+  double length = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+  if (length == 0) length = 1;
+  const double w = qw / length, u = qx / length, vq = qy / length, t = qz / length;
+  const double r00 = 1 - 2 * (vq * vq + t * t), r01 = 2 * (u * vq - w * t), r02 = 2 * (u * t + w * vq);
+  const double r10 = 2 * (u * vq + w * t), r11 = 1 - 2 * (u * u + t * t), r12 = 2 * (vq * t - w * u);
+  const double r20 = 2 * (u * t - w * vq), r21 = 2 * (vq * t + w * u), r22 = 1 - 2 * (u * u + vq * vq);
+  const double m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
+  const double m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
+  const double m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
+  const double S[3][3] = {
+      {m00 * m00 + m01 * m01 + m02 * m02, m00 * m10 + m01 * m11 + m02 * m12, m00 * m20 + m01 * m21 + m02 * m22},
+      {m00 * m10 + m01 * m11 + m02 * m12, m10 * m10 + m11 * m11 + m12 * m12, m10 * m20 + m11 * m21 + m12 * m22},
+      {m00 * m20 + m01 * m21 + m02 * m22, m10 * m20 + m11 * m21 + m12 * m22, m20 * m20 + m21 * m21 + m22 * m22}};
+  const double p00 = v.dP[0], p11 = v.dP[5];
+  const double fx = p00 * (double)v.vpw * 0.5, fy = p11 * (double)v.vph * 0.5;
+  double j00 = fx, j11 = fy, j02 = 0, j12 = 0;
+  if (v.persp) {
+    const double d = -vz;
+    const double lx = kC13 / p00, ly = kC13 / p11;
+    double cx = vx / d, cy = vy / d;
+    if (cx < -lx) cx = -lx;
+    if (cx > lx) cx = lx;
+    if (cy < -ly) cy = -ly;
+    if (cy > ly) cy = ly;
+    const double x = cx * d, y = cy * d;
+    j00 = fx / d;
+    j11 = fy / d;
+    j02 = fx * x / (d * d);
+    j12 = fy * y / (d * d);
+  }
+  double t0[3], t1[3], u0[3], u1[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    t0[c] = j00 * v.dmv[c * 4 + 0] + j02 * v.dmv[c * 4 + 2];
+    t1[c] = j11 * v.dmv[c * 4 + 1] + j12 * v.dmv[c * 4 + 2];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    u0[c] = t0[0] * S[0][c] + t0[1] * S[1][c] + t0[2] * S[2][c];
+    u1[c] = t1[0] * S[0][c] + t1[1] * S[1][c] + t1[2] * S[2][c];
+  }
+  const double xx = u0[0] * t0[0] + u0[1] * t0[1] + u0[2] * t0[2];
+  const double xy = u0[0] * t1[0] + u0[1] * t1[1] + u0[2] * t1[2];
+  const double yy = u1[0] * t1[0] + u1[1] * t1[1] + u1[2] * t1[2];
+  const double det = (xx + kC03) * (yy + kC03) - xy * xy;
+  const double area = kC2PI * sqrt(det > 0 ? det : 0);
+  const double alpha = (double)(b.z >> 24) / 255.0;
+  return alpha * area;
+}
+
+// One splat through makeKeys. STD: the projection is the reverse-Z infinite
+// perspective (nonzero only at 0, 5, 8, 9, 11 = -1, 14) and the model-view is
+// affine, so the zero terms of the FP64 products are left out: adding an
+// exact zero changes at most the sign of a zero, which no test here sees.
+struct Projected {
+  bool keep;
+  bool recomputed;
+  uint32_t key;
+  float xx, xy, yy;
+};
+
+template <bool STD>
+__device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 b, float bn0w0, float bn1w1,
+                                                 float bw2) {
+  const double x = f32_of(a.x), y = f32_of(a.y), z = f32_of(a.z);
+  // view = modelView * (p, 1); each product is exact, so fma is the sum as written.
+  const double vx = fma(v.dmv[8], z, fma(v.dmv[4], y, v.dmv[0] * x)) + v.dmv[12];
+  const double vy = fma(v.dmv[9], z, fma(v.dmv[5], y, v.dmv[1] * x)) + v.dmv[13];
+  const double vz = fma(v.dmv[10], z, fma(v.dmv[6], y, v.dmv[2] * x)) + v.dmv[14];
+  double cx, cy, cz, cw;
+  if (STD) {
+    cx = v.dP[0] * vx + v.dP[8] * vz;
+    cy = v.dP[5] * vy + v.dP[9] * vz;
+    cz = v.dP[14];
+    cw = -vz;
+  } else {
+    const double vw = fma(v.dmv[11], z, fma(v.dmv[7], y, v.dmv[3] * x)) + v.dmv[15];
+    const double vv[4] = {vx, vy, vz, vw};
+    double c[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      c[r] = v.dP[r] * vv[0] + v.dP[4 + r] * vv[1] + v.dP[8 + r] * vv[2] + v.dP[12 + r] * vv[3];
+    cx = c[0];
+    cy = c[1];
+    cz = c[2];
+    cw = c[3];
+  }
+  const uint32_t abyte = b.z >> 24;
+  // Far first: the farthest key is the smallest (src/render/splats.js:179-181).
+  double t = (-vz - v.ddmin) * v.ddscale;
+  t = t < 0.0 ? 0.0 : t;
+  t = t > 65535.0 ? 65535.0 : t;
+  Projected o;
+  o.key = 65535u - (uint32_t)t;
+  o.recomputed = false;
+  o.xx = o.xy = o.yy = 0.f;
+  o.keep = !((vz >= 0) | (cz > cw) | (fabs(cx) > kC13 * cw) | (fabs(cy) > kC13 * cw) | (abyte <= 1u));
+  if (o.keep) {
+    // f32 from here to the decision: fusing multiply-adds only tightens the bound below.
+#pragma clang fp contract(fast)
+    // ---- footprint: cov = diag(j) G G^T diag(j), G = E R S ----
+    float j0 = v.fx, j1 = v.fy, xd = 0.f, yd = 0.f, l0 = 0.f, l1 = 0.f;
+    if (v.persp) {
+      const float id = __builtin_amdgcn_rcpf((float)(-vz));
+      const float rx = (float)vx * id, ry = (float)vy * id;
+      xd = fminf(fmaxf(rx, -v.limx), v.limx);
+      yd = fminf(fmaxf(ry, -v.limy), v.limy);
+      l0 = fmaxf(fabsf(rx), v.limx);
+      l1 = fmaxf(fabsf(ry), v.limy);
+      j0 *= id;
+      j1 *= id;
+    }
+    // E = rows of W (the model-view's 3x3, W[k][c] = mv[c*4+k]) with the
+    // perspective's third column folded in: T = diag(j0, j1) E.
+    const float e00 = xd * v.mv[2] + v.mv[0], e01 = xd * v.mv[6] + v.mv[4], e02 = xd * v.mv[10] + v.mv[8];
+    const float e10 = yd * v.mv[2] + v.mv[1], e11 = yd * v.mv[6] + v.mv[5], e12 = yd * v.mv[10] + v.mv[9];
+    // R from the byte quaternion, unnormalised: integer products are exact in f32.
+    const uint32_t qb = b.w;
+    const float qw = (float)(qb & 255u) - 128.f, qx = (float)((qb >> 8) & 255u) - 128.f;
+    const float qy = (float)((qb >> 16) & 255u) - 128.f, qz = (float)(qb >> 24) - 128.f;
+    const float xx2 = qx * qx, yy2 = qy * qy, zz2 = qz * qz;
+    const float n2 = qw * qw + xx2 + yy2 + zz2;
+    const float s = n2 > 0.f ? 2.f * __builtin_amdgcn_rcpf(n2) : 0.f;
+    const float r00 = 1.f - s * (yy2 + zz2), r01 = s * (qx * qy - qw * qz), r02 = s * (qx * qz + qw * qy);
+    const float r10 = s * (qx * qy + qw * qz), r11 = 1.f - s * (xx2 + zz2), r12 = s * (qy * qz - qw * qx);
+    const float r20 = s * (qx * qz - qw * qy), r21 = s * (qy * qz + qw * qx), r22 = 1.f - s * (xx2 + yy2);
+    const float sx = f32_of(a.w), sy = f32_of(b.x), sz = f32_of(b.y);
+    const float g00 = (e00 * r00 + e01 * r10 + e02 * r20) * sx;
+    const float g01 = (e00 * r01 + e01 * r11 + e02 * r21) * sy;
+    const float g02 = (e00 * r02 + e01 * r12 + e02 * r22) * sz;
+    const float g10 = (e10 * r00 + e11 * r10 + e12 * r20) * sx;
+    const float g11 = (e10 * r01 + e11 * r11 + e12 * r21) * sy;
+    const float g12 = (e10 * r02 + e11 * r12 + e12 * r22) * sz;
+    o.xx = (j0 * j0) * (g00 * g00 + g01 * g01 + g02 * g02);
+    o.xy = (j0 * j1) * (g00 * g10 + g01 * g11 + g02 * g12);
+    o.yy = (j1 * j1) * (g10 * g10 + g11 * g11 + g12 * g12);
+
+    if (v.cull > 0.f) {
+      // The f32 value and its bound against the FP64 one. With n_r the row
+      // norms of E (|W_r| + l_r |W_2|, l_r >= |x/d|), S2 = |s|^2 and u = 2^-24,
+      // |xx - xx64| <= 52 u (j0 n0)^2 S2, likewise yy and xy; 64 is used.
+      const float a0 = j0 * (l0 * bw2 + bn0w0), a1 = j1 * (l1 * bw2 + bn1w1);
+      const float c = (64.f * kU) * (sx * sx + sy * sy + sz * sz);
+      const float exx = c * a0 * a0, eyy = c * a1 * a1, exy = c * a0 * a1;
+      const float X = o.xx + 0.3f, Y = o.yy + 0.3f;
+      const float xy2 = o.xy * o.xy;
+      const float det = X * Y - xy2;
+      const float edet = 1.01f * (Y * exx + X * eyy + exx * eyy + 2.f * fabsf(o.xy) * exy + exy * exy +
+                                  4.f * kU * (X * Y + xy2));
+      const float root = __builtin_amdgcn_sqrtf(fmaxf(det, 0.f));
+      const float alpha2pi = (float)abyte * (6.2831853f / 255.f);
+      const float val = alpha2pi * root;
+      const float esq = det > 2.f * edet ? edet * __builtin_amdgcn_rsqf(det - edet) : __builtin_amdgcn_sqrtf(3.f * edet);
+      const float eval = 1.02f * alpha2pi * (esq + 8.f * kU * root) + 1e-30f;
+      if (!(val - v.cull > eval)) {
+        if (v.cull - val > eval) {
+          o.keep = false;
+        } else {
+          o.recomputed = true;
+          o.keep = !(canonical_value(v, a, b, vx, vy, vz) < v.dcull);
+        }
+      }
+    }
+  }
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// K1 body: project + key over 256-splat tiles first, first+stride, ... < end.
+// Shared by the single-view kernel (a grid striding over the cloud) and the
+// batch kernel (a work item of consecutive tiles).
+struct ProjectSmem {
+  uint32_t h[512];  // survivors per low key byte, then per high key byte
+  uint32_t w[4];
+  uint32_t recomp;
+  uint32_t last;
+};
+
+template <bool STD>
+__device__ __forceinline__ uint32_t project_tiles(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
+                                                  const ViewK &v, uint16_t *__restrict__ keys,
+                                                  uint64_t *__restrict__ mask, float4 *__restrict__ cov,
+                                                  uint32_t first, uint32_t stride, uint32_t end) {
+  const uint32_t tid = threadIdx.x;
+  // Per-view parts of the bound's row norms.
+  const float bw0 = fabsf(v.mv[0]) + fabsf(v.mv[4]) + fabsf(v.mv[8]);
+  const float bw1 = fabsf(v.mv[1]) + fabsf(v.mv[5]) + fabsf(v.mv[9]);
+  const float bw2 = fabsf(v.mv[2]) + fabsf(v.mv[6]) + fabsf(v.mv[10]);
+  uint32_t recomp = 0;
+  // Two register buffers: a tile's records are loaded while the previous
+  // tile is worked on, so every wave keeps a tile of loads in flight behind
+  // its arithmetic.
+  auto load = [&](uint32_t tile, uint4 &a, uint4 &b) {
+    const uint32_t i = (tile << 8) + tid;
+    if (tile < end && i < n) {
+      a = rec[2 * (size_t)i];
+      b = rec[2 * (size_t)i + 1];
+    }
+  };
+  auto work = [&](uint32_t tile, const uint4 &a, const uint4 &b) {
+    const uint32_t i = (tile << 8) + tid;
+    const bool in = i < n;
+    Projected p = project_one<STD>(v, a, b, bw0, bw1, bw2);
+    p.keep = p.keep && in;
+    const uint64_t bal = __ballot(p.keep);
+    if ((tid & 63u) == 0 && in) mask[i >> 6] = bal;
+    if (in) keys[i] = (uint16_t)p.key;
+    recomp += (p.recomputed && in) ? 1u : 0u;
+    if (p.keep) {
+      cov[i] = make_float4(p.xx, p.xy, p.yy, __uint_as_float(b.z));
+      atomicAdd(&sm.h[p.key & 255u], 1u);
+      atomicAdd(&sm.h[256u + (p.key >> 8)], 1u);
+    }
+  };
+  uint4 a0 = make_uint4(0, 0, 0, 0), b0 = a0, a1 = a0, b1 = a0;
+  load(first, a0, b0);
+  for (uint32_t tile = first; tile < end; tile += 2 * stride) {
+    load(tile + stride, a1, b1);
+    work(tile, a0, b0);
+    load(tile + 2 * stride, a0, b0);
+    if (tile + stride < end) work(tile + stride, a1, b1);
+  }
+  return recomp;
+}
+
+__device__ __forceinline__ void project_begin(ProjectSmem &sm) {
+  __syncthreads();  // the LDS may still be read by the previous work item
+  for (uint32_t k = threadIdx.x; k < 512; k += 256) sm.h[k] = 0;
+  if (threadIdx.x == 0) sm.recomp = 0;
+  __syncthreads();
+}
+
+// After a workgroup's tiles: its counts go to the cloud's histogram, and the
+// last of `expected` workgroups scans the histogram into the digit bases and
+// zeroes it for the next sort (the reference's scanCounts, :188-212). Returns
+// true in the workgroup that did so.
+__device__ __forceinline__ bool project_finish(ProjectSmem &sm, Ctl *__restrict__ ctl, uint32_t recomp,
+                                               uint32_t expected) {
+  const uint32_t tid = threadIdx.x;
+  if (recomp) atomicAdd(&sm.recomp, recomp);
+  __syncthreads();
+  for (uint32_t k = tid; k < 512; k += 256)
+    if (sm.h[k]) atomicAdd(&ctl->hist[k], sm.h[k]);
+  if (tid == 0 && sm.recomp) atomicAdd(&ctl->recomputed_acc, sm.recomp);
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) sm.last = (atomicAdd(&ctl->done, 1u) == expected - 1) ? 1u : 0u;
+  __syncthreads();
+  if (!sm.last) return false;
+  __threadfence();
+  const uint32_t c0 = __hip_atomic_load(&ctl->hist[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t c1 = __hip_atomic_load(&ctl->hist[256 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  ctl->hist[tid] = 0;
+  ctl->hist[256 + tid] = 0;
+  uint32_t total0, total1;
+  const uint32_t e0 = block_scan_excl(c0, sm.w, &total0);
+  const uint32_t e1 = block_scan_excl(c1, sm.w, &total1);
+  ctl->base[tid] = e0;
+  ctl->base[256 + tid] = e1;
+  if (tid == 0) {
+    ctl->count = total0;
+    ctl->done = 0;
+    ctl->ticket_lo = 0;
+    ctl->ticket_hi = 0;
+    ctl->recomputed = __hip_atomic_load(&ctl->recomputed_acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ctl->recomputed_acc = 0;
+  }
+  __threadfence();
+  __syncthreads();
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// K2/K3 body: one stable counting pass over an 8-bit digit on the 4096-element
+// tile t. HI = false: input is every splat in index order (keys + survivor
+// bits), digit = key & 255, output high byte << 24 | index. HI = true: input
+// is that output (count from the control block), digit = value >> 24, output
+// the index.
+struct ScatterSmem {
+  uint32_t val[kTile];
+  uint8_t dig[kTile];
+  uint32_t wh[4][256];  // per-wave running digit counts, then wave offsets
+  uint32_t lstart[256];
+  uint32_t gstart[256];
+  uint32_t w[4];
+  uint32_t tile;
+  uint32_t nvalid;
+};
+
+template <bool HI>
+__device__ __forceinline__ void scatter_tile(ScatterSmem &sm, uint32_t t, const uint16_t *__restrict__ keys,
+                                             const uint64_t *__restrict__ mask,
+                                             const uint32_t *__restrict__ in_packed, uint32_t n,
+                                             const Ctl *__restrict__ ctl, uint64_t *__restrict__ status,
+                                             uint32_t epoch, uint32_t *__restrict__ out, uint32_t *stuck) {
+  constexpr int KPT = kKeysPerThread;
+  constexpr int TILE = kTile;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t limit = HI ? ctl->count : n;
+  const uint32_t tile0 = t * (uint32_t)TILE;
+  if (tile0 >= limit) return;  // past the survivors (HI); never looked back to
+  for (uint32_t k = tid; k < 1024; k += 256) (&sm.wh[0][0])[k] = 0;
+  __syncthreads();
+  const uint32_t wbase = tile0 + w * (64u * KPT);
+  const uint32_t nwords = (n + 63u) >> 6;
+
+  // Every load of the wave's 16 rounds is issued before any is used.
+  uint32_t raw[KPT];
+  uint64_t vmask[KPT];
+  const uint32_t wave0 = tile0 + (uint32_t)__builtin_amdgcn_readfirstlane(w) * (64u * KPT);
+#pragma unroll
+  for (int r = 0; r < KPT; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+    if (HI) {
+      raw[r] = i < limit ? in_packed[i] : 0u;
+    } else {
+      raw[r] = i < n ? (uint32_t)keys[i] : 0u;
+      const uint32_t wi = (wave0 + (uint32_t)r * 64u) >> 6;
+      vmask[r] = wi < nwords ? mask[wi] : 0ull;
+    }
+  }
+  uint32_t dig[KPT], rank[KPT], val[KPT];
+  uint32_t okbits = 0;
+#pragma unroll
+  for (int r = 0; r < KPT; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+    uint64_t valid;
+    uint32_t d, value;
+    if (HI) {
+      valid = __ballot(i < limit);
+      d = raw[r] >> 24;
+      value = raw[r] & 0xFFFFFFu;
+    } else {
+      valid = vmask[r];
+      d = raw[r] & 255u;
+      value = ((raw[r] >> 8) << 24) | i;
+    }
+    const bool ok = (valid >> lane) & 1ull;
+    const uint64_t peers = match_digit(d, valid);
+    const uint32_t below = count_below(peers);
+    uint32_t before = 0;
+    if (ok) before = sm.wh[w][d];
+    if (ok && below == 0) sm.wh[w][d] = before + (uint32_t)__popcll(peers);
+    dig[r] = d;
+    rank[r] = before + below;
+    val[r] = value;
+    okbits |= (ok ? 1u : 0u) << r;
+  }
+  __syncthreads();
+  // Thread d owns digit d: wave offsets, the tile's count, the look-back.
+  const uint32_t d = tid;
+  const uint32_t c0 = sm.wh[0][d], c1 = sm.wh[1][d], c2 = sm.wh[2][d], c3 = sm.wh[3][d];
+  const uint32_t tot = c0 + c1 + c2 + c3;
+  uint64_t *st = status + (size_t)t * 256u + d;
+  uint32_t prefix = 0;
+  if (t == 0) {
+    store_status(st, epoch, kFlagInclusive, tot);
+  } else {
+    store_status(st, epoch, kFlagAggregate, tot);
+    int p = (int)t - 1;
+    uint32_t spins = 0;
+    while (p >= 0) {
+      const uint64_t s = load_status(status + (size_t)p * 256u + d);
+      const uint32_t tag = (uint32_t)(s >> 32);
+      if ((tag >> 2) != epoch || (tag & 3u) == 0u) {
+        // never, unless a bug: flag it and finish (and stop waiting once any wait has run out)
+        if (++spins == kSpinLimit || ((spins & 1023u) == 0 && __hip_atomic_load(stuck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          atomicOr(stuck, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      prefix += (uint32_t)s;
+      if ((tag & 3u) == kFlagInclusive) break;
+      --p;
+    }
+    store_status(st, epoch, kFlagInclusive, prefix + tot);
+  }
+  sm.wh[0][d] = 0;
+  sm.wh[1][d] = c0;
+  sm.wh[2][d] = c0 + c1;
+  sm.wh[3][d] = c0 + c1 + c2;
+  uint32_t nvalid;
+  const uint32_t lstart = block_scan_excl(tot, sm.w, &nvalid);
+  sm.lstart[d] = lstart;
+  sm.gstart[d] = ctl->base[(HI ? 256u : 0u) + d] + prefix;
+  if (tid == 0) sm.nvalid = nvalid;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < KPT; ++r) {
+    if ((okbits >> r) & 1u) {
+      const uint32_t pos = sm.lstart[dig[r]] + sm.wh[w][dig[r]] + rank[r];
+      if (pos < (uint32_t)TILE) {  // always, when the counts agree; a guard, not a branch taken
+        sm.val[pos] = val[r];
+        sm.dig[pos] = (uint8_t)dig[r];
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t nv = min(sm.nvalid, (uint32_t)TILE);
+  for (uint32_t p = tid; p < nv; p += 256) {
+    const uint32_t dd = sm.dig[p];
+    const uint32_t g = sm.gstart[dd] + (p - sm.lstart[dd]);
+    if (g < n) out[g] = sm.val[p];  // n bounds both outputs; a guard against a miscount
+  }
+}
+
+}  // namespace
+}  // namespace wd

@@ -30,6 +30,8 @@ constexpr int kKeysPerThread = 16;
 // Flags of a look-back status word: value in the low 32 bits, epoch << 2 | flag above.
 constexpr uint32_t kFlagAggregate = 1;
 constexpr uint32_t kFlagInclusive = 2;
+// Polls of a bounded wait before it gives up and flags the sort as failed.
+constexpr uint32_t kSpinLimit = 1u << 20;
 
 // Device control block of one sort state. The K1 histogram, its finish counter
 // and the tickets reset themselves (the last K1 workgroup zeroes them), so a
@@ -45,7 +47,8 @@ struct Ctl {
   uint32_t recomputed_acc;
   uint32_t fallback;      // learned sort: 1 when the exact path ran
   uint32_t overflow_bins;
-  uint32_t pad[56];
+  uint32_t stuck;         // a bounded wait ran out (a bug, reported, never a hang)
+  uint32_t pad[55];
 };
 static_assert(sizeof(Ctl) % 256 == 0, "Ctl keeps 256-byte alignment");
 
@@ -67,6 +70,35 @@ struct ViewK {
 
 // Builds ViewK from a wd_view (host).
 ViewK make_viewk(const wd_view &v);
+// The reverse-Z infinite perspective and an affine model-view: the kernels'
+// fast form (zero terms of the FP64 products left out).
+bool is_std(const wd_view &v);
+
+// One (cloud, view, state) of a batch, as the batch kernel reads it.
+struct BatchJob {
+  ViewK view;
+  const void *rec;  // uint4 records
+  uint16_t *keys;
+  uint64_t *mask;
+  float *cov;
+  uint32_t *packed;
+  uint32_t *order;
+  Ctl *ctl;
+  uint64_t *status_lo, *status_hi;
+  uint32_t n, std_form;
+  uint32_t a_items;  // projection work items: kBatchATiles tiles of 256 splats each
+  uint32_t tiles;    // scatter tiles of kTile
+};
+constexpr uint32_t kBatchATiles = 16;
+// Device control block of a batch: tickets, exits, the run counter, and per
+// job the projection-finished flag and the finished low-byte tiles.
+struct BatchCtl {
+  uint32_t ticket;
+  uint32_t exited;
+  uint32_t runs;
+  uint32_t stuck;
+  // followed by ready[max_jobs], lo_done[max_jobs]
+};
 
 }  // namespace wd
 
@@ -97,6 +129,21 @@ struct wd_cloud {
   uint32_t magic = 0x53504c54;  // cleared by destroy
 };
 
+struct wd_batch {
+  wd_context *ctx = nullptr;
+  uint32_t max_jobs = 0, n = 0;
+  wd::BatchJob *jobs = nullptr;   // device
+  uint32_t *starts = nullptr;     // device: 3 x (max_jobs + 1) item starts, then totals
+  wd::BatchCtl *ctl = nullptr;    // device
+  std::vector<wd::BatchJob> host_jobs;
+  std::vector<uint32_t> host_starts;
+  std::vector<const void *> shape;  // clouds and states of the captured graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t grid = 0, total = 0;
+  uint32_t magic = 0x42415443;
+};
+
 struct wd_state {
   wd_context *ctx = nullptr;
   uint64_t capacity = 0;
@@ -121,6 +168,9 @@ extern const char *const kKernelNames[kKernelCount];
 // Kernel launchers (splat_sort.hip). Enqueue on ctx->stream.
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8);
 int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_state *st);
+int launch_batch(wd_context *ctx, wd_batch *b);
+int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid);
+int launch_copy(wd_context *ctx, const void *src, void *dst, size_t bytes);
 int launch_canonical_debug(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, double *out5);
 
 }  // namespace wd
