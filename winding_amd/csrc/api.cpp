@@ -287,7 +287,8 @@ int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
   alloc((void **)&s->packed, capacity * sizeof(uint32_t));
   alloc((void **)&s->order, capacity * sizeof(uint32_t));
   alloc((void **)&s->cov, capacity * 4 * sizeof(float));
-  alloc((void **)&s->status_lo, s->tiles * 256 * sizeof(uint64_t));
+  alloc((void **)&s->tilehist, s->tiles * 512 * sizeof(uint16_t));
+  alloc((void **)&s->prefix, s->tiles * 256 * sizeof(uint32_t));
   alloc((void **)&s->status_hi, s->tiles * 256 * sizeof(uint64_t));
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
@@ -303,7 +304,7 @@ int wd_state_destroy(wd_state *s) {
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   for (void *p : {(void *)s->ctl, (void *)s->keys, (void *)s->mask, (void *)s->packed, (void *)s->order,
-                  (void *)s->cov, (void *)s->status_lo, (void *)s->status_hi})
+                  (void *)s->cov, (void *)s->tilehist, (void *)s->prefix, (void *)s->status_hi})
     if (p) hipFree(p);
   s->magic = 0;
   delete s;

@@ -1,12 +1,12 @@
 // The batch kernel: many (cloud, view) sorts in one persistent launch.
 //
-// Work is a single ticketed queue of items in three runs:
-//   [projection items of every job][low-byte scatter tiles][high-byte tiles]
+// Work is a single ticketed queue of items in four runs:
+//   [projection chunks of every job][scan items][low-byte tiles][high-byte tiles]
 // A workgroup takes the next ticket, does that item, and takes another, until
 // the queue is empty. An item only ever waits on items with smaller tickets:
-// a low-byte tile waits for its job's projection (the job's digit bases), its
-// look-back waits on earlier tiles of the same job, a high-byte tile waits for
-// all of its job's low-byte tiles. Smaller tickets were taken by workgroups
+// a job's scan items wait for its projection chunks, its low-byte tiles for
+// its scan (the digit offsets), its high-byte tiles for all its low-byte tiles,
+// and a high-byte look-back on earlier tiles of the same job. Smaller tickets were taken by workgroups
 // already running, so every wait ends, whatever the grid size. Jobs of 10k and
 // 2M splats share the machine tile by tile instead of launch by launch.
 //
@@ -49,12 +49,14 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
     ProjectSmem p;
     ScatterSmem s;
   } sm;
-  __shared__ uint32_t s_item, s_runs;
-  uint32_t *ready = reinterpret_cast<uint32_t *>(bc + 1);
+  __shared__ uint32_t s_item, s_runs, s_w[4], s_last;
+  uint32_t *proj_done = reinterpret_cast<uint32_t *>(bc + 1);
+  uint32_t *ready = proj_done + max_jobs;
   uint32_t *lo_done = ready + max_jobs;
-  const uint32_t *a_start = starts, *b_start = starts + (nj + 1);
-  const uint32_t a_total = a_start[nj], b_total = b_start[nj];
-  const uint32_t total = a_total + 2 * b_total;
+  // starts: tile starts (chunks = low tiles = high tiles) then scan-item starts, nj + 1 each
+  const uint32_t *t_start = starts, *s_start = starts + (nj + 1);
+  const uint32_t t_total = t_start[nj], s_total = s_start[nj];
+  const uint32_t e_proj = t_total, e_scan = e_proj + s_total, e_lo = e_scan + t_total, total = e_lo + t_total;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_runs = __hip_atomic_load(&bc->runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
@@ -65,37 +67,46 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
     __syncthreads();
     const uint32_t item = s_item;
     if (item >= total) break;
-    if (item < a_total) {
-      const uint32_t j = find_job(a_start, nj, item);
+    if (item < e_proj) {
+      const uint32_t j = find_job(t_start, nj, item);
       const BatchJob &J = jobs[j];
-      const uint32_t k = item - a_start[j];
-      const uint32_t ntiles = (J.n + 255u) >> 8;
-      const uint32_t first = k * kBatchATiles, end = min(first + kBatchATiles, ntiles);
-      project_begin(sm.p);
-      const uint32_t r = J.std_form
-          ? project_tiles<true>(sm.p, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, first, 1, end)
-          : project_tiles<false>(sm.p, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, first, 1, end);
-      if (project_finish(sm.p, J.ctl, r, J.a_items) && tid == 0)
+      if (J.std_form)
+        project_chunk<true>(sm.p, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
+                            &J.ctl->recomputed_acc, item - t_start[j]);
+      else
+        project_chunk<false>(sm.p, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
+                             &J.ctl->recomputed_acc, item - t_start[j]);
+      __threadfence();
+      __syncthreads();
+      if (tid == 0) atomicAdd(&proj_done[j], 1u);
+    } else if (item < e_scan) {
+      const uint32_t it = item - e_proj;
+      const uint32_t j = find_job(s_start, nj, it);
+      const BatchJob &J = jobs[j];
+      if (tid == 0) wait_value(&proj_done[j], J.tiles, &bc->stuck);
+      __syncthreads();
+      if (scan_digits<kBatchScanDigits>(it - s_start[j], 256 / kBatchScanDigits, J.tiles, J.tilehist, J.prefix,
+                                        J.ctl, s_w, &s_last) && tid == 0)
         __hip_atomic_store(&ready[j], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else if (item < a_total + b_total) {
-      const uint32_t it = item - a_total;
-      const uint32_t j = find_job(b_start, nj, it);
+    } else if (item < e_lo) {
+      const uint32_t it = item - e_scan;
+      const uint32_t j = find_job(t_start, nj, it);
       const BatchJob &J = jobs[j];
       if (tid == 0) wait_value(&ready[j], 1u, &bc->stuck);
       __syncthreads();
-      scatter_tile<false>(sm.s, it - b_start[j], J.keys, J.mask, nullptr, J.n, J.ctl, J.status_lo, epoch, J.packed,
-                          &bc->stuck);
+      scatter_tile<false>(sm.s, it - t_start[j], J.keys, J.mask, nullptr, J.n, J.tiles, J.prefix, J.ctl, nullptr,
+                          epoch, J.packed, &bc->stuck);
       __threadfence();
       __syncthreads();
       if (tid == 0) atomicAdd(&lo_done[j], 1u);
     } else {
-      const uint32_t it = item - a_total - b_total;
-      const uint32_t j = find_job(b_start, nj, it);
+      const uint32_t it = item - e_lo;
+      const uint32_t j = find_job(t_start, nj, it);
       const BatchJob &J = jobs[j];
       if (tid == 0) wait_value(&lo_done[j], J.tiles, &bc->stuck);
       __syncthreads();
-      scatter_tile<true>(sm.s, it - b_start[j], nullptr, nullptr, J.packed, J.n, J.ctl, J.status_hi, epoch, J.order,
-                         &bc->stuck);
+      scatter_tile<true>(sm.s, it - t_start[j], nullptr, nullptr, J.packed, J.n, J.tiles, nullptr, J.ctl, J.status_hi,
+                         epoch, J.order, &bc->stuck);
     }
   }
   // The last workgroup out resets the queue for the next run.
@@ -104,6 +115,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
     if (atomicAdd(&bc->exited, 1u) == gridDim.x - 1) {
       __threadfence();
       for (uint32_t j = 0; j < nj; ++j) {
+        proj_done[j] = 0;
         ready[j] = 0;
         lo_done[j] = 0;
       }

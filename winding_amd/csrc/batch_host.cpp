@@ -18,9 +18,9 @@ int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out) {
   wd_batch *b = new wd_batch();
   b->ctx = ctx;
   b->max_jobs = max_jobs;
-  const size_t ctl_bytes = sizeof(wd::BatchCtl) + 2 * sizeof(uint32_t) * max_jobs;
+  const size_t ctl_bytes = sizeof(wd::BatchCtl) + 3 * sizeof(uint32_t) * max_jobs;
   hipError_t e = hipMalloc(&b->jobs, sizeof(wd::BatchJob) * max_jobs);
-  if (e == hipSuccess) e = hipMalloc(&b->starts, sizeof(uint32_t) * 3 * (max_jobs + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->starts, sizeof(uint32_t) * 2 * (max_jobs + 1));
   if (e == hipSuccess) e = hipMalloc(&b->ctl, ctl_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(b->ctl, 0, ctl_bytes, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -36,7 +36,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
   if (!jobs || n == 0 || n > b->max_jobs) return fail(WD_E_ARG, "batch: %u jobs, this batch holds 1 to %u", n, b->max_jobs);
   std::vector<wd::BatchJob> hj(n);
-  std::vector<uint32_t> a(n + 1, 0), t(n + 1, 0);
+  std::vector<uint32_t> a(n + 1, 0), t(n + 1, 0);  // scan items, tiles
   std::vector<const void *> shape;
   for (uint32_t i = 0; i < n; ++i) {
     const wd_job &J = jobs[i];
@@ -61,21 +61,20 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     B.packed = s->packed;
     B.order = s->order;
     B.ctl = s->ctl;
-    B.status_lo = s->status_lo;
+    B.tilehist = s->tilehist;
+    B.prefix = s->prefix;
     B.status_hi = s->status_hi;
-    const uint32_t tiles256 = (B.n + 255u) / 256u;
-    B.a_items = (tiles256 + wd::kBatchATiles - 1) / wd::kBatchATiles;
     B.tiles = (B.n + wd::kTile - 1) / wd::kTile;
-    a[i + 1] = a[i] + B.a_items;
+    B.pad = 0;
     t[i + 1] = t[i] + B.tiles;
+    a[i + 1] = a[i] + 256 / wd::kBatchScanDigits;
     s->last_n = B.n;
     shape.push_back(J.cloud);
     shape.push_back(J.state);
   }
-  std::vector<uint32_t> starts(3 * (n + 1), 0);
-  std::copy(a.begin(), a.end(), starts.begin());
-  std::copy(t.begin(), t.end(), starts.begin() + (n + 1));
-  std::copy(t.begin(), t.end(), starts.begin() + 2 * (n + 1));
+  std::vector<uint32_t> starts(2 * (n + 1), 0);
+  std::copy(t.begin(), t.end(), starts.begin());
+  std::copy(a.begin(), a.end(), starts.begin() + (n + 1));
   wd_context *ctx = b->ctx;
   WD_HIP(hipSetDevice(ctx->device), "batch: set device");
   // The table is read by the kernel at run time: a later set() with new views
@@ -85,7 +84,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   WD_HIP(hipMemcpyAsync(b->starts, starts.data(), sizeof(uint32_t) * starts.size(), hipMemcpyHostToDevice,
                         ctx->stream), "batch: item table");
   WD_HIP(hipStreamSynchronize(ctx->stream), "batch: upload");
-  const uint32_t total = a[n] + 2 * t[n];
+  const uint32_t total = 3 * t[n] + a[n];
   if (shape != b->shape || n != b->n || total != b->total) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);

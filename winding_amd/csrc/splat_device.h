@@ -255,27 +255,33 @@ __device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 
 }
 
 // ---------------------------------------------------------------------------
-// K1 body: project + key over 256-splat tiles first, first+stride, ... < end.
-// Shared by the single-view kernel (a grid striding over the cloud) and the
-// batch kernel (a work item of consecutive tiles).
+// Pass 1 body: project + key over one chunk of kTile splats (16 tiles of 256),
+// the chunk the low-byte scatter later takes as one tile. Writes every splat's
+// key, survivor bit and cov2d slot (culled splats get zeros, so every cache
+// line is written whole, by one wave), and the chunk's two 256-bin digit
+// histograms: no global atomics.
 struct ProjectSmem {
   uint32_t h[512];  // survivors per low key byte, then per high key byte
-  uint32_t w[4];
   uint32_t recomp;
-  uint32_t last;
 };
 
 template <bool STD>
-__device__ __forceinline__ uint32_t project_tiles(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
-                                                  const ViewK &v, uint16_t *__restrict__ keys,
-                                                  uint64_t *__restrict__ mask, float4 *__restrict__ cov,
-                                                  uint32_t first, uint32_t stride, uint32_t end) {
+__device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
+                                              const ViewK &v, uint16_t *__restrict__ keys,
+                                              uint64_t *__restrict__ mask, float4 *__restrict__ cov,
+                                              uint16_t *__restrict__ tilehist, uint32_t *__restrict__ recomp_acc,
+                                              uint32_t chunk) {
   const uint32_t tid = threadIdx.x;
+  __syncthreads();  // the LDS may still be read by the previous work item
+  for (uint32_t k = tid; k < 512; k += 256) sm.h[k] = 0;
+  if (tid == 0) sm.recomp = 0;
+  __syncthreads();
   // Per-view parts of the bound's row norms.
   const float bw0 = fabsf(v.mv[0]) + fabsf(v.mv[4]) + fabsf(v.mv[8]);
   const float bw1 = fabsf(v.mv[1]) + fabsf(v.mv[5]) + fabsf(v.mv[9]);
   const float bw2 = fabsf(v.mv[2]) + fabsf(v.mv[6]) + fabsf(v.mv[10]);
   uint32_t recomp = 0;
+  const uint32_t first = chunk * (kTile / 256), end = min(first + kTile / 256, (n + 255u) >> 8);
   // Two register buffers: a tile's records are loaded while the previous
   // tile is worked on, so every wave keeps a tile of loads in flight behind
   // its arithmetic.
@@ -293,63 +299,97 @@ __device__ __forceinline__ uint32_t project_tiles(ProjectSmem &sm, const uint4 *
     p.keep = p.keep && in;
     const uint64_t bal = __ballot(p.keep);
     if ((tid & 63u) == 0 && in) mask[i >> 6] = bal;
-    if (in) keys[i] = (uint16_t)p.key;
+    if (in) {
+      keys[i] = (uint16_t)p.key;
+      cov[i] = p.keep ? make_float4(p.xx, p.xy, p.yy, __uint_as_float(b.z)) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     recomp += (p.recomputed && in) ? 1u : 0u;
     if (p.keep) {
-      cov[i] = make_float4(p.xx, p.xy, p.yy, __uint_as_float(b.z));
       atomicAdd(&sm.h[p.key & 255u], 1u);
       atomicAdd(&sm.h[256u + (p.key >> 8)], 1u);
     }
   };
   uint4 a0 = make_uint4(0, 0, 0, 0), b0 = a0, a1 = a0, b1 = a0;
   load(first, a0, b0);
-  for (uint32_t tile = first; tile < end; tile += 2 * stride) {
-    load(tile + stride, a1, b1);
+  for (uint32_t tile = first; tile < end; tile += 2) {
+    load(tile + 1, a1, b1);
     work(tile, a0, b0);
-    load(tile + 2 * stride, a0, b0);
-    if (tile + stride < end) work(tile + stride, a1, b1);
+    load(tile + 2, a0, b0);
+    if (tile + 1 < end) work(tile + 1, a1, b1);
   }
-  return recomp;
-}
-
-__device__ __forceinline__ void project_begin(ProjectSmem &sm) {
-  __syncthreads();  // the LDS may still be read by the previous work item
-  for (uint32_t k = threadIdx.x; k < 512; k += 256) sm.h[k] = 0;
-  if (threadIdx.x == 0) sm.recomp = 0;
-  __syncthreads();
-}
-
-// After a workgroup's tiles: its counts go to the cloud's histogram, and the
-// last of `expected` workgroups scans the histogram into the digit bases and
-// zeroes it for the next sort (the reference's scanCounts, :188-212). Returns
-// true in the workgroup that did so.
-__device__ __forceinline__ bool project_finish(ProjectSmem &sm, Ctl *__restrict__ ctl, uint32_t recomp,
-                                               uint32_t expected) {
-  const uint32_t tid = threadIdx.x;
   if (recomp) atomicAdd(&sm.recomp, recomp);
   __syncthreads();
-  for (uint32_t k = tid; k < 512; k += 256)
-    if (sm.h[k]) atomicAdd(&ctl->hist[k], sm.h[k]);
-  if (tid == 0 && sm.recomp) atomicAdd(&ctl->recomputed_acc, sm.recomp);
+  for (uint32_t k = tid; k < 512; k += 256) tilehist[(size_t)chunk * 512 + k] = (uint16_t)sm.h[k];
+  if (tid == 0 && sm.recomp) atomicAdd(recomp_acc, sm.recomp);
+}
+
+// ---------------------------------------------------------------------------
+// Between the passes: per-tile offsets for the low-byte pass and digit bases
+// for both, from the chunk histograms (reduce-then-scan: the low-byte pass
+// then needs no look-back). A workgroup takes DPW digits; 256 / DPW threads a
+// digit split its tiles into runs, scan the runs' sums, and write each tile's
+// exclusive prefix, digit-major, prefix[d * T + t]. The last workgroup of a
+// cloud turns the digit totals into bases (the reference's scanCounts,
+// src/render/splats.js:188-212) and resets the counters for the next sort.
+template <int DPW>
+__device__ __forceinline__ bool scan_digits(uint32_t g, uint32_t groups, uint32_t T,
+                                            const uint16_t *__restrict__ tilehist, uint32_t *__restrict__ prefix,
+                                            Ctl *__restrict__ ctl, uint32_t *s_w, uint32_t *s_last) {
+  constexpr uint32_t TPD = 256 / DPW;  // threads a digit
+  const uint32_t tid = threadIdx.x, j = tid % TPD, d = g * DPW + tid / TPD;
+  const uint32_t per = (T + TPD - 1) / TPD;
+  const uint32_t t0 = min(j * per, T), t1 = min(t0 + per, T);
+  uint32_t lo = 0, hi = 0;
+  for (uint32_t t = t0; t < t1; ++t) {
+    lo += tilehist[(size_t)t * 512 + d];
+    hi += tilehist[(size_t)t * 512 + 256 + d];
+  }
+  uint32_t excl, tot_lo, tot_hi;
+  if (DPW == 1) {
+    excl = block_scan_excl(lo, s_w, &tot_lo);
+    block_scan_excl(hi, s_w, &tot_hi);
+  } else {
+    // 16 threads a digit: one DPP row; scan within the row.
+    int x = (int)lo;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
+    excl = (uint32_t)x - lo;
+    const int lane = (int)(tid & 63u), row_last = (lane & ~15) + 15;
+    tot_lo = (uint32_t)__shfl(x, row_last);
+    int y = (int)hi;
+    y += __builtin_amdgcn_update_dpp(0, y, 0x111, 0xF, 0xF, true);
+    y += __builtin_amdgcn_update_dpp(0, y, 0x112, 0xF, 0xF, true);
+    y += __builtin_amdgcn_update_dpp(0, y, 0x114, 0xF, 0xF, true);
+    y += __builtin_amdgcn_update_dpp(0, y, 0x118, 0xF, 0xF, true);
+    tot_hi = (uint32_t)__shfl(y, row_last);
+  }
+  uint32_t run = excl;
+  for (uint32_t t = t0; t < t1; ++t) {
+    prefix[(size_t)d * T + t] = run;
+    run += tilehist[(size_t)t * 512 + d];
+  }
+  if (j == 0) {
+    ctl->hist[d] = tot_lo;
+    ctl->hist[256 + d] = tot_hi;
+  }
   __threadfence();
   __syncthreads();
-  if (tid == 0) sm.last = (atomicAdd(&ctl->done, 1u) == expected - 1) ? 1u : 0u;
+  if (tid == 0) *s_last = (atomicAdd(&ctl->done, 1u) == groups - 1) ? 1u : 0u;
   __syncthreads();
-  if (!sm.last) return false;
+  if (!*s_last) return false;
   __threadfence();
   const uint32_t c0 = __hip_atomic_load(&ctl->hist[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t c1 = __hip_atomic_load(&ctl->hist[256 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  ctl->hist[tid] = 0;
-  ctl->hist[256 + tid] = 0;
   uint32_t total0, total1;
-  const uint32_t e0 = block_scan_excl(c0, sm.w, &total0);
-  const uint32_t e1 = block_scan_excl(c1, sm.w, &total1);
+  const uint32_t e0 = block_scan_excl(c0, s_w, &total0);
+  const uint32_t e1 = block_scan_excl(c1, s_w, &total1);
   ctl->base[tid] = e0;
   ctl->base[256 + tid] = e1;
   if (tid == 0) {
     ctl->count = total0;
     ctl->done = 0;
-    ctl->ticket_lo = 0;
     ctl->ticket_hi = 0;
     ctl->recomputed = __hip_atomic_load(&ctl->recomputed_acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ctl->recomputed_acc = 0;
@@ -360,11 +400,12 @@ __device__ __forceinline__ bool project_finish(ProjectSmem &sm, Ctl *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// K2/K3 body: one stable counting pass over an 8-bit digit on the 4096-element
-// tile t. HI = false: input is every splat in index order (keys + survivor
-// bits), digit = key & 255, output high byte << 24 | index. HI = true: input
-// is that output (count from the control block), digit = value >> 24, output
-// the index.
+// Pass 2 and 3 body: one stable counting pass over an 8-bit digit on the
+// 4096-element tile t. HI = false: input is every splat in index order (keys +
+// survivor bits), digit = key & 255, tile offsets from the scan (prefix),
+// output high byte << 24 | index. HI = true: input is that output (count from
+// the control block), digit = value >> 24, tile offsets by decoupled look-back,
+// output the index.
 struct ScatterSmem {
   uint32_t val[kTile];
   uint8_t dig[kTile];
@@ -376,18 +417,55 @@ struct ScatterSmem {
   uint32_t nvalid;
 };
 
+// Inclusive prefix of digit d over the tiles before t, from their look-back
+// words: eight predecessors are read at once, so a wait costs one memory round
+// trip per eight tiles instead of per tile.
+__device__ __forceinline__ uint32_t look_back(const uint64_t *__restrict__ status, uint32_t t, uint32_t d,
+                                              uint32_t epoch, uint32_t *stuck) {
+  constexpr int W = 8;
+  uint32_t prefix = 0, spins = 0;
+  int p = (int)t - 1;
+  while (p >= 0) {
+    uint64_t s[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) s[k] = (p - k >= 0) ? load_status(status + (size_t)(p - k) * 256u + d) : 0ull;
+    // Sum from the nearest tile down to the first inclusive word; stop short
+    // at the first tile that has published nothing yet, and poll it again.
+    int k = 0;
+    for (; k < W && p - k >= 0; ++k) {
+      const uint32_t tag = (uint32_t)(s[k] >> 32);
+      if ((tag >> 2) != epoch || (tag & 3u) == 0u) break;
+      prefix += (uint32_t)s[k];
+      if ((tag & 3u) == kFlagInclusive) return prefix;
+    }
+    p -= k;
+    if (k == 0) {
+      // never, unless a bug: flag it and finish (and stop waiting once any wait has run out)
+      if (++spins == kSpinLimit ||
+          ((spins & 1023u) == 0 && __hip_atomic_load(stuck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        atomicOr(stuck, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return prefix;
+}
+
 template <bool HI>
 __device__ __forceinline__ void scatter_tile(ScatterSmem &sm, uint32_t t, const uint16_t *__restrict__ keys,
                                              const uint64_t *__restrict__ mask,
-                                             const uint32_t *__restrict__ in_packed, uint32_t n,
-                                             const Ctl *__restrict__ ctl, uint64_t *__restrict__ status,
-                                             uint32_t epoch, uint32_t *__restrict__ out, uint32_t *stuck) {
+                                             const uint32_t *__restrict__ in_packed, uint32_t n, uint32_t T,
+                                             const uint32_t *__restrict__ prefix, const Ctl *__restrict__ ctl,
+                                             uint64_t *__restrict__ status, uint32_t epoch,
+                                             uint32_t *__restrict__ out, uint32_t *stuck) {
   constexpr int KPT = kKeysPerThread;
   constexpr int TILE = kTile;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t limit = HI ? ctl->count : n;
   const uint32_t tile0 = t * (uint32_t)TILE;
   if (tile0 >= limit) return;  // past the survivors (HI); never looked back to
+  __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 1024; k += 256) (&sm.wh[0][0])[k] = 0;
   __syncthreads();
   const uint32_t wbase = tile0 + w * (64u * KPT);
@@ -436,35 +514,23 @@ __device__ __forceinline__ void scatter_tile(ScatterSmem &sm, uint32_t t, const 
     okbits |= (ok ? 1u : 0u) << r;
   }
   __syncthreads();
-  // Thread d owns digit d: wave offsets, the tile's count, the look-back.
+  // Thread d owns digit d: wave offsets, the tile's count, its global offset.
   const uint32_t d = tid;
   const uint32_t c0 = sm.wh[0][d], c1 = sm.wh[1][d], c2 = sm.wh[2][d], c3 = sm.wh[3][d];
   const uint32_t tot = c0 + c1 + c2 + c3;
-  uint64_t *st = status + (size_t)t * 256u + d;
-  uint32_t prefix = 0;
-  if (t == 0) {
-    store_status(st, epoch, kFlagInclusive, tot);
-  } else {
-    store_status(st, epoch, kFlagAggregate, tot);
-    int p = (int)t - 1;
-    uint32_t spins = 0;
-    while (p >= 0) {
-      const uint64_t s = load_status(status + (size_t)p * 256u + d);
-      const uint32_t tag = (uint32_t)(s >> 32);
-      if ((tag >> 2) != epoch || (tag & 3u) == 0u) {
-        // never, unless a bug: flag it and finish (and stop waiting once any wait has run out)
-        if (++spins == kSpinLimit || ((spins & 1023u) == 0 && __hip_atomic_load(stuck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-          atomicOr(stuck, 1u);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      prefix += (uint32_t)s;
-      if ((tag & 3u) == kFlagInclusive) break;
-      --p;
+  uint32_t pre;
+  if (HI) {
+    uint64_t *st = status + (size_t)t * 256u + d;
+    if (t == 0) {
+      store_status(st, epoch, kFlagInclusive, tot);
+      pre = 0;
+    } else {
+      store_status(st, epoch, kFlagAggregate, tot);
+      pre = look_back(status, t, d, epoch, stuck);
+      store_status(st, epoch, kFlagInclusive, pre + tot);
     }
-    store_status(st, epoch, kFlagInclusive, prefix + tot);
+  } else {
+    pre = prefix[(size_t)d * T + t];
   }
   sm.wh[0][d] = 0;
   sm.wh[1][d] = c0;
@@ -473,7 +539,7 @@ __device__ __forceinline__ void scatter_tile(ScatterSmem &sm, uint32_t t, const 
   uint32_t nvalid;
   const uint32_t lstart = block_scan_excl(tot, sm.w, &nvalid);
   sm.lstart[d] = lstart;
-  sm.gstart[d] = ctl->base[(HI ? 256u : 0u) + d] + prefix;
+  sm.gstart[d] = ctl->base[(HI ? 256u : 0u) + d] + pre;
   if (tid == 0) sm.nvalid = nvalid;
   __syncthreads();
 #pragma unroll

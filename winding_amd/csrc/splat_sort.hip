@@ -34,30 +34,39 @@
 namespace wd {
 namespace {
 
-// K1: a grid of a few workgroups per CU strides over the cloud's tiles, so
-// the histogram flush is paid once per workgroup.
+// Pass 1: a workgroup a chunk of 4096 splats.
 template <bool STD>
 __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, uint32_t n, ViewK v,
                                                  uint16_t *__restrict__ keys, uint64_t *__restrict__ mask,
-                                                 float4 *__restrict__ cov, Ctl *__restrict__ ctl) {
+                                                 float4 *__restrict__ cov, uint16_t *__restrict__ tilehist,
+                                                 Ctl *__restrict__ ctl) {
   __shared__ ProjectSmem sm;
-  project_begin(sm);
-  const uint32_t r = project_tiles<STD>(sm, rec, n, v, keys, mask, cov, blockIdx.x, gridDim.x, (n + 255u) >> 8);
-  project_finish(sm, ctl, r, gridDim.x);
+  project_chunk<STD>(sm, rec, n, v, keys, mask, cov, tilehist, &ctl->recomputed_acc, blockIdx.x);
 }
 
+// Between the passes: a workgroup a digit.
+__global__ __launch_bounds__(256) void k_scan(uint32_t T, const uint16_t *__restrict__ tilehist,
+                                              uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl) {
+  __shared__ uint32_t s_w[4], s_last;
+  scan_digits<1>(blockIdx.x, gridDim.x, T, tilehist, prefix, ctl, s_w, &s_last);
+}
 
+// Pass 2: tile t is workgroup t (its offsets are known); pass 3: tiles are
+// taken in ticket order, so every tile a look-back waits on belongs to a
+// workgroup that is already running.
 template <bool HI>
 __global__ __launch_bounds__(256) void k_scatter(const uint16_t *__restrict__ keys, const uint64_t *__restrict__ mask,
-                                                 const uint32_t *__restrict__ in_packed, uint32_t n,
-                                                 Ctl *__restrict__ ctl, uint64_t *__restrict__ status,
-                                                 uint32_t epoch, uint32_t *__restrict__ out) {
+                                                 const uint32_t *__restrict__ in_packed, uint32_t n, uint32_t T,
+                                                 const uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl,
+                                                 uint64_t *__restrict__ status, uint32_t epoch,
+                                                 uint32_t *__restrict__ out) {
   __shared__ ScatterSmem sm;
-  // Tiles are taken in ticket order, so every tile a look-back waits on
-  // belongs to a workgroup that is already running.
-  if (threadIdx.x == 0) sm.tile = atomicAdd(HI ? &ctl->ticket_hi : &ctl->ticket_lo, 1u);
-  __syncthreads();
-  scatter_tile<HI>(sm, sm.tile, keys, mask, in_packed, n, ctl, status, epoch, out, &ctl->stuck);
+  if (HI) {
+    if (threadIdx.x == 0) sm.tile = atomicAdd(&ctl->ticket_hi, 1u);
+    __syncthreads();
+  }
+  const uint32_t t = HI ? sm.tile : blockIdx.x;
+  scatter_tile<HI>(sm, t, keys, mask, in_packed, n, T, prefix, ctl, status, epoch, out, &ctl->stuck);
 }
 
 // ---------------------------------------------------------------------------
@@ -190,8 +199,8 @@ ViewK make_viewk(const wd_view &v) {
   return k;
 }
 
-const char *const kKernelNames[kKernelCount] = {"k_bounds", "k_project", "k_scatter_lo", "k_scatter_hi",
-                                                "k_batch",  "k_mlp_fwd", "k_mlp_bwd",    "k_learned"};
+const char *const kKernelNames[kKernelCount] = {"k_bounds", "k_project", "k_scan", "k_scatter_lo",
+                                                "k_scatter_hi", "k_batch", "k_mlp", "k_learned"};
 
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8) {
   static const uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0, 0xFFFFFFFFu, 0};
@@ -209,33 +218,36 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
   const ViewK vk = make_viewk(view);
   const bool std_form = is_std(view);
   if (++st->epoch >= (1u << 29)) {
-    WD_HIP(hipMemsetAsync(st->status_lo, 0, st->tiles * 256 * sizeof(uint64_t), ctx->stream), "sort: status reset");
     WD_HIP(hipMemsetAsync(st->status_hi, 0, st->tiles * 256 * sizeof(uint64_t), ctx->stream), "sort: status reset");
     st->epoch = 1;
   }
   st->last_n = n;
-  const uint32_t tiles256 = (n + 255u) / 256u;
-  const uint32_t g1 = std::min<uint32_t>(tiles256, (uint32_t)ctx->cus * 4u);
   const uint32_t tiles = (n + kTile - 1) / kTile;
   const uint4 *rec = (const uint4 *)cloud->records;
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKProject, &ev)) return WD_E_HIP;
   if (std_form)
-    hipLaunchKernelGGL(k_project<true>, dim3(g1), dim3(256), 0, ctx->stream, rec, n, vk, st->keys, st->mask,
-                       (float4 *)st->cov, st->ctl);
+    hipLaunchKernelGGL(k_project<true>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk, st->keys, st->mask,
+                       (float4 *)st->cov, st->tilehist, st->ctl);
   else
-    hipLaunchKernelGGL(k_project<false>, dim3(g1), dim3(256), 0, ctx->stream, rec, n, vk, st->keys, st->mask,
-                       (float4 *)st->cov, st->ctl);
+    hipLaunchKernelGGL(k_project<false>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk, st->keys, st->mask,
+                       (float4 *)st->cov, st->tilehist, st->ctl);
   WD_HIP(hipGetLastError(), "sort: project launch");
   if (ctx->prof_end(kKProject, ev)) return WD_E_HIP;
+  if (ctx->prof_begin(kKScan, &ev)) return WD_E_HIP;
+  hipLaunchKernelGGL(k_scan, dim3(256), dim3(256), 0, ctx->stream, tiles, st->tilehist, st->prefix, st->ctl);
+  WD_HIP(hipGetLastError(), "sort: scan launch");
+  if (ctx->prof_end(kKScan, ev)) return WD_E_HIP;
   if (ctx->prof_begin(kKScatterLo, &ev)) return WD_E_HIP;
   hipLaunchKernelGGL(k_scatter<false>, dim3(tiles), dim3(256), 0, ctx->stream, st->keys, st->mask,
-                     (const uint32_t *)nullptr, n, st->ctl, st->status_lo, st->epoch, st->packed);
+                     (const uint32_t *)nullptr, n, tiles, st->prefix, st->ctl, (uint64_t *)nullptr, st->epoch,
+                     st->packed);
   WD_HIP(hipGetLastError(), "sort: low-byte scatter launch");
   if (ctx->prof_end(kKScatterLo, ev)) return WD_E_HIP;
   if (ctx->prof_begin(kKScatterHi, &ev)) return WD_E_HIP;
   hipLaunchKernelGGL(k_scatter<true>, dim3(tiles), dim3(256), 0, ctx->stream, (const uint16_t *)nullptr,
-                     (const uint64_t *)nullptr, st->packed, n, st->ctl, st->status_hi, st->epoch, st->order);
+                     (const uint64_t *)nullptr, st->packed, n, tiles, (const uint32_t *)nullptr, st->ctl,
+                     st->status_hi, st->epoch, st->order);
   WD_HIP(hipGetLastError(), "sort: high-byte scatter launch");
   return ctx->prof_end(kKScatterHi, ev);
 }

@@ -84,20 +84,44 @@ struct BatchJob {
   uint32_t *packed;
   uint32_t *order;
   Ctl *ctl;
-  uint64_t *status_lo, *status_hi;
+  uint16_t *tilehist;
+  uint32_t *prefix;
+  uint64_t *status_hi;
   uint32_t n, std_form;
-  uint32_t a_items;  // projection work items: kBatchATiles tiles of 256 splats each
-  uint32_t tiles;    // scatter tiles of kTile
+  uint32_t tiles;    // projection chunks = scatter tiles, of kTile splats
+  uint32_t pad;
 };
-constexpr uint32_t kBatchATiles = 16;
+// One GEMM of the predictor on the matrix cores: C = alpha * A B, element (m, k)
+// of A at A[m * sa_m + k * sa_k] (f32 or bf16), (k, n) of B likewise, then
+// + bias[n], relu, zero where mask <= 0, += C, and an optional bf16 copy.
+struct MlpGemm {
+  int M, N, K;
+  const void *A;
+  int a_bf16;
+  long sa_m, sa_k;
+  const void *B;
+  int b_bf16;
+  long sb_k, sb_n;
+  float *C;
+  long sc_m, sc_n;
+  const float *bias;
+  const float *mask;
+  int relu, accumulate;
+  float alpha;
+  uint16_t *C_bf16;
+};
+
+// Digits a scan work item of a batch takes (16 items a job).
+constexpr uint32_t kBatchScanDigits = 16;
 // Device control block of a batch: tickets, exits, the run counter, and per
-// job the projection-finished flag and the finished low-byte tiles.
+// job the finished projection chunks, the scanned flag and the finished
+// low-byte tiles.
 struct BatchCtl {
   uint32_t ticket;
   uint32_t exited;
   uint32_t runs;
   uint32_t stuck;
-  // followed by ready[max_jobs], lo_done[max_jobs]
+  // followed by proj_done[max_jobs], ready[max_jobs], lo_done[max_jobs]
 };
 
 }  // namespace wd
@@ -155,14 +179,16 @@ struct wd_state {
   uint32_t *packed = nullptr;  // low-digit pass output: high byte << 24 | index
   uint32_t *order = nullptr;   // the draw order
   float *cov = nullptr;        // n x 4 floats
-  uint64_t *status_lo = nullptr, *status_hi = nullptr;  // look-back words, tiles x 256
+  uint16_t *tilehist = nullptr;  // per 4096-splat chunk: 256 low-byte then 256 high-byte counts
+  uint32_t *prefix = nullptr;    // low-byte pass offsets, digit-major: [256][tiles]
+  uint64_t *status_hi = nullptr; // high-byte pass look-back words, tiles x 256
   uint64_t tiles = 0;
   uint32_t magic = 0x53544154;
 };
 
 namespace wd {
 
-enum KernelId { kKBounds = 0, kKProject, kKScatterLo, kKScatterHi, kKBatch, kKMlpFwd, kKMlpBwd, kKLearned, kKernelCount };
+enum KernelId { kKBounds = 0, kKProject, kKScan, kKScatterLo, kKScatterHi, kKBatch, kKMlp, kKLearned, kKernelCount };
 extern const char *const kKernelNames[kKernelCount];
 
 // Kernel launchers (splat_sort.hip). Enqueue on ctx->stream.
@@ -171,6 +197,14 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
 int launch_batch(wd_context *ctx, wd_batch *b);
 int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid);
 int launch_copy(wd_context *ctx, const void *src, void *dst, size_t bytes);
+// Predictor kernels (mlp.hip).
+int launch_gemm(wd_context *ctx, const MlpGemm &g);
+int launch_softmax_xent(wd_context *ctx, const float *z, const float *target, int rows, float scale, float *p, float *dz,
+                        float *loss);
+int launch_colsum(wd_context *ctx, const float *x, int rows, int cols, float *out);
+int launch_adam(wd_context *ctx, float *w, uint16_t *wb, const float *g, float *m, float *v, int n, float lr, float b1,
+                float b2, int step, float eps);
+int launch_to_bf16(wd_context *ctx, const float *w, uint16_t *wb, int n);
 int launch_canonical_debug(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, double *out5);
 
 }  // namespace wd
