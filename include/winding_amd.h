@@ -214,6 +214,39 @@ int wd_batch_run(wd_context *ctx, wd_batch *batch);
 int wd_batch_stuck(wd_context *ctx, wd_batch *batch, uint32_t *stuck);
 int wd_batch_destroy(wd_batch *batch);
 
+/* ---- the learned depth-CDF predictor (no counterpart in the reference) */
+
+/*
+ * A small MLP, 12 pose floats -> 128 -> 128 -> 256, bf16 weights on the
+ * matrix cores with fp32 accumulation. Its 256 outputs (a softmax) are the
+ * widths, in key space, of 256 equal-mass bins of the survivors' depth keys:
+ * the predicted depth CDF, inverted. Parameters (WD_MLP_PARAMS fp32, flat):
+ * W1[128][12] b1[128] W2[128][128] b2[128] W3[256][128] b3[256].
+ */
+#define WD_MLP_PARAMS 51200
+int wd_mlp_create(wd_context *ctx, uint64_t seed, wd_mlp **out);
+int wd_mlp_destroy(wd_mlp *mlp);
+int wd_mlp_set(wd_mlp *mlp, const float *params);
+int wd_mlp_get(wd_mlp *mlp, float *params);
+/* A saved predictor: "WDMLP1", the bf16 weights, the fp32 biases. */
+int wd_mlp_save(wd_mlp *mlp, void *bytes, uint64_t capacity, uint64_t *size);
+int wd_mlp_load(wd_mlp *mlp, const void *bytes, uint64_t size);
+/*
+ * One forward and backward pass on the GPU, no update: the mean cross-entropy
+ * against targets (n x 256 distributions) and every parameter's gradient.
+ */
+int wd_mlp_grad(wd_context *ctx, wd_mlp *mlp, const float *poses,
+                const float *targets, uint32_t n, float *grads, float *loss);
+/* Adam on the GPU over `epochs` shuffled passes of minibatches of `batch`. */
+int wd_mlp_train(wd_context *ctx, wd_mlp *mlp, const float *poses,
+                 const float *targets, uint32_t n, uint32_t epochs,
+                 uint32_t batch, float lr, uint64_t seed, float *loss);
+/* n x 256 predicted bin widths (probabilities). */
+int wd_mlp_predict(wd_context *ctx, wd_mlp *mlp, const float *poses, uint32_t n,
+                   float *probs);
+/* The 12 pose floats of a view: the model-view's top rows, translation / 10. */
+int wd_pose_features(const wd_view *view, float out[12]);
+
 /* ---- synthetic inputs: the workloads' seeded generator ---------------- */
 
 /*

@@ -27,6 +27,8 @@ EXPORTS = [
     "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
     "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_profile", "wd_profile_read",
     "wd_batch_create", "wd_batch_set", "wd_batch_run", "wd_batch_stuck", "wd_batch_destroy",
+    "wd_mlp_create", "wd_mlp_destroy", "wd_mlp_set", "wd_mlp_get", "wd_mlp_save", "wd_mlp_load",
+    "wd_mlp_grad", "wd_mlp_train", "wd_mlp_predict", "wd_pose_features",
     "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest", "wd_debug_copy",
 ]
 
@@ -102,6 +104,16 @@ def lib():
             "wd_batch_run": (I, [P, P]),
             "wd_batch_stuck": (I, [P, P, C.POINTER(U32)]),
             "wd_batch_destroy": (I, [P]),
+            "wd_mlp_create": (I, [P, U64, C.POINTER(P)]),
+            "wd_mlp_destroy": (I, [P]),
+            "wd_mlp_set": (I, [P, P]),
+            "wd_mlp_get": (I, [P, P]),
+            "wd_mlp_save": (I, [P, P, U64, C.POINTER(U64)]),
+            "wd_mlp_load": (I, [P, P, U64]),
+            "wd_mlp_grad": (I, [P, P, P, P, U32, P, C.POINTER(C.c_float)]),
+            "wd_mlp_train": (I, [P, P, P, P, U32, U32, U32, C.c_float, U64, C.POINTER(C.c_float)]),
+            "wd_mlp_predict": (I, [P, P, P, U32, P]),
+            "wd_pose_features": (I, [C.POINTER(View), P]),
             "wd_synth_records": (I, [I, U64, U64, P]),
             "wd_debug_canonical": (I, [P, P, C.POINTER(View), P]),
             "wd_debug_selftest": (I, [P, P, P, U32, P]),
@@ -336,3 +348,71 @@ class Batch:
         if self.ptr:
             check(lib().wd_batch_destroy(self.ptr))
             self.ptr = None
+
+
+WD_MLP_PARAMS = 51200
+
+
+def pose_features(view):
+    """The predictor's 12 inputs for a view."""
+    out = np.zeros(WD_POSE_FLOATS, np.float32)
+    check(lib().wd_pose_features(C.byref(view), out.ctypes.data))
+    return out
+
+
+class Mlp:
+    """The learned depth-CDF predictor (wd_mlp_*): 12 -> 128 -> 128 -> 256, bf16 matrix cores."""
+
+    def __init__(self, ctx, seed=0):
+        p = C.c_void_p()
+        check(lib().wd_mlp_create(ctx.ptr, seed, C.byref(p)))
+        self.ctx = ctx
+        self.ptr = p
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().wd_mlp_destroy(self.ptr))
+            self.ptr = None
+
+    def params(self):
+        out = np.zeros(WD_MLP_PARAMS, np.float32)
+        check(lib().wd_mlp_get(self.ptr, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(lib().wd_mlp_set(self.ptr, p.ctypes.data))
+
+    def save(self):
+        size = C.c_uint64()
+        check(lib().wd_mlp_save(self.ptr, None, 0, C.byref(size)))
+        buf = np.zeros(size.value, np.uint8)
+        check(lib().wd_mlp_save(self.ptr, buf.ctypes.data, size.value, C.byref(size)))
+        return buf.tobytes()
+
+    def load(self, data):
+        buf = np.frombuffer(data, np.uint8).copy()
+        check(lib().wd_mlp_load(self.ptr, buf.ctypes.data, len(buf)))
+
+    def grad(self, poses, targets):
+        poses = np.ascontiguousarray(poses, np.float32)
+        targets = np.ascontiguousarray(targets, np.float32)
+        g = np.zeros(WD_MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        check(lib().wd_mlp_grad(self.ctx.ptr, self.ptr, poses.ctypes.data, targets.ctypes.data, len(poses),
+                                g.ctypes.data, C.byref(loss)))
+        return loss.value, g
+
+    def train(self, poses, targets, epochs, batch=256, lr=1e-3, seed=0):
+        poses = np.ascontiguousarray(poses, np.float32)
+        targets = np.ascontiguousarray(targets, np.float32)
+        loss = C.c_float()
+        check(lib().wd_mlp_train(self.ctx.ptr, self.ptr, poses.ctypes.data, targets.ctypes.data, len(poses), epochs,
+                                 batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def predict(self, poses):
+        poses = np.ascontiguousarray(poses, np.float32)
+        out = np.zeros((len(poses), 256), np.float32)
+        check(lib().wd_mlp_predict(self.ctx.ptr, self.ptr, poses.ctypes.data, len(poses), out.ctypes.data))
+        return out

@@ -287,9 +287,14 @@ int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
   alloc((void **)&s->packed, capacity * sizeof(uint32_t));
   alloc((void **)&s->order, capacity * sizeof(uint32_t));
   alloc((void **)&s->cov, capacity * 4 * sizeof(float));
+  s->blocks = (s->tiles + wd::kScanRows - 1) / wd::kScanRows;
   alloc((void **)&s->tilehist, s->tiles * 512 * sizeof(uint16_t));
-  alloc((void **)&s->prefix, s->tiles * 256 * sizeof(uint32_t));
-  alloc((void **)&s->status_hi, s->tiles * 256 * sizeof(uint64_t));
+  alloc((void **)&s->tilehist3, s->tiles * 256 * sizeof(uint16_t));
+  alloc((void **)&s->prefix_lo, s->tiles * 256 * sizeof(uint32_t));
+  alloc((void **)&s->prefix_hi, s->tiles * 256 * sizeof(uint32_t));
+  alloc((void **)&s->bsum_lo, s->blocks * 256 * sizeof(uint32_t));
+  alloc((void **)&s->bsum_hi, s->blocks * 256 * sizeof(uint32_t));
+  alloc((void **)&s->bsum3, s->blocks * 256 * sizeof(uint32_t));
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     wd_state_destroy(s);
@@ -304,7 +309,8 @@ int wd_state_destroy(wd_state *s) {
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   for (void *p : {(void *)s->ctl, (void *)s->keys, (void *)s->mask, (void *)s->packed, (void *)s->order,
-                  (void *)s->cov, (void *)s->tilehist, (void *)s->prefix, (void *)s->status_hi})
+                  (void *)s->cov, (void *)s->tilehist, (void *)s->tilehist3, (void *)s->prefix_lo,
+                  (void *)s->prefix_hi, (void *)s->bsum_lo, (void *)s->bsum_hi, (void *)s->bsum3})
     if (p) hipFree(p);
   s->magic = 0;
   delete s;

@@ -18,9 +18,9 @@ int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out) {
   wd_batch *b = new wd_batch();
   b->ctx = ctx;
   b->max_jobs = max_jobs;
-  const size_t ctl_bytes = sizeof(wd::BatchCtl) + 3 * sizeof(uint32_t) * max_jobs;
+  const size_t ctl_bytes = sizeof(wd::BatchCtl) + wd::kBatchCounters * sizeof(uint32_t) * max_jobs;
   hipError_t e = hipMalloc(&b->jobs, sizeof(wd::BatchJob) * max_jobs);
-  if (e == hipSuccess) e = hipMalloc(&b->starts, sizeof(uint32_t) * 2 * (max_jobs + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->starts, sizeof(uint32_t) * 3 * (max_jobs + 1));
   if (e == hipSuccess) e = hipMalloc(&b->ctl, ctl_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(b->ctl, 0, ctl_bytes, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -36,7 +36,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
   if (!jobs || n == 0 || n > b->max_jobs) return fail(WD_E_ARG, "batch: %u jobs, this batch holds 1 to %u", n, b->max_jobs);
   std::vector<wd::BatchJob> hj(n);
-  std::vector<uint32_t> a(n + 1, 0), t(n + 1, 0);  // scan items, tiles
+  std::vector<uint32_t> t(n + 1, 0), a(n + 1, 0), u(n + 1, 0);  // item starts: tiles, scan blocks, super-tiles
   std::vector<const void *> shape;
   for (uint32_t i = 0; i < n; ++i) {
     const wd_job &J = jobs[i];
@@ -62,19 +62,25 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     B.order = s->order;
     B.ctl = s->ctl;
     B.tilehist = s->tilehist;
-    B.prefix = s->prefix;
-    B.status_hi = s->status_hi;
+    B.tilehist3 = s->tilehist3;
+    B.prefix_lo = s->prefix_lo;
+    B.prefix_hi = s->prefix_hi;
+    B.bsum_lo = s->bsum_lo;
+    B.bsum_hi = s->bsum_hi;
+    B.bsum3 = s->bsum3;
     B.tiles = (B.n + wd::kTile - 1) / wd::kTile;
-    B.pad = 0;
+    B.blocks = (B.tiles + wd::kScanRows - 1) / wd::kScanRows;
     t[i + 1] = t[i] + B.tiles;
-    a[i + 1] = a[i] + 256 / wd::kBatchScanDigits;
+    a[i + 1] = a[i] + B.blocks;
+    u[i + 1] = u[i] + (B.tiles + wd::kSuper - 1) / wd::kSuper;
     s->last_n = B.n;
     shape.push_back(J.cloud);
     shape.push_back(J.state);
   }
-  std::vector<uint32_t> starts(2 * (n + 1), 0);
+  std::vector<uint32_t> starts(3 * (n + 1), 0);
   std::copy(t.begin(), t.end(), starts.begin());
   std::copy(a.begin(), a.end(), starts.begin() + (n + 1));
+  std::copy(u.begin(), u.end(), starts.begin() + 2 * (n + 1));
   wd_context *ctx = b->ctx;
   WD_HIP(hipSetDevice(ctx->device), "batch: set device");
   // The table is read by the kernel at run time: a later set() with new views
@@ -84,7 +90,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   WD_HIP(hipMemcpyAsync(b->starts, starts.data(), sizeof(uint32_t) * starts.size(), hipMemcpyHostToDevice,
                         ctx->stream), "batch: item table");
   WD_HIP(hipStreamSynchronize(ctx->stream), "batch: upload");
-  const uint32_t total = 3 * t[n] + a[n];
+  const uint32_t total = 2 * t[n] + 2 * a[n] + 2 * u[n];
   if (shape != b->shape || n != b->n || total != b->total) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);

@@ -324,73 +324,80 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
 }
 
 // ---------------------------------------------------------------------------
-// Between the passes: per-tile offsets for the low-byte pass and digit bases
-// for both, from the chunk histograms (reduce-then-scan: the low-byte pass
-// then needs no look-back). A workgroup takes DPW digits; 256 / DPW threads a
-// digit split its tiles into runs, scan the runs' sums, and write each tile's
-// exclusive prefix, digit-major, prefix[d * T + t]. The last workgroup of a
-// cloud turns the digit totals into bases (the reference's scanCounts,
-// src/render/splats.js:188-212) and resets the counters for the next sort.
-template <int DPW>
-__device__ __forceinline__ bool scan_digits(uint32_t g, uint32_t groups, uint32_t T,
-                                            const uint16_t *__restrict__ tilehist, uint32_t *__restrict__ prefix,
-                                            Ctl *__restrict__ ctl, uint32_t *s_w, uint32_t *s_last) {
-  constexpr uint32_t TPD = 256 / DPW;  // threads a digit
-  const uint32_t tid = threadIdx.x, j = tid % TPD, d = g * DPW + tid / TPD;
-  const uint32_t per = (T + TPD - 1) / TPD;
-  const uint32_t t0 = min(j * per, T), t1 = min(t0 + per, T);
-  uint32_t lo = 0, hi = 0;
-  for (uint32_t t = t0; t < t1; ++t) {
-    lo += tilehist[(size_t)t * 512 + d];
-    hi += tilehist[(size_t)t * 512 + 256 + d];
+// Scans between the passes, reduce-then-scan: neither scatter pass waits on
+// another tile. Level 1, block g of kScanRows tiles: thread d walks the
+// block's rows of a [T][stride] u16 histogram at column off + d (rows are read
+// whole, coalesced), writing each tile's exclusive prefix within the block,
+// prefix[t * 256 + d], and the block's total, bsum[g * 256 + d]. Level 2, once
+// all of a cloud's blocks are in: thread d turns column d of bsum into
+// exclusive block offsets in place and returns the digit's total.
+
+__device__ __forceinline__ void scan_block(uint32_t g, uint32_t T, const uint16_t *__restrict__ hist, uint32_t stride,
+                                           uint32_t off, uint32_t *__restrict__ prefix, uint32_t *__restrict__ bsum) {
+  const uint32_t d = threadIdx.x;
+  const uint32_t t0 = g * kScanRows, t1 = min(t0 + kScanRows, T);
+  uint32_t c[kScanRows];
+#pragma unroll
+  for (uint32_t k = 0; k < kScanRows; ++k) c[k] = (t0 + k < t1) ? hist[(size_t)(t0 + k) * stride + off + d] : 0u;
+  uint32_t run = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanRows; ++k) {
+    if (prefix && t0 + k < t1) prefix[(size_t)(t0 + k) * 256 + d] = run;
+    run += c[k];
   }
-  uint32_t excl, tot_lo, tot_hi;
-  if (DPW == 1) {
-    excl = block_scan_excl(lo, s_w, &tot_lo);
-    block_scan_excl(hi, s_w, &tot_hi);
-  } else {
-    // 16 threads a digit: one DPP row; scan within the row.
-    int x = (int)lo;
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
-    excl = (uint32_t)x - lo;
-    const int lane = (int)(tid & 63u), row_last = (lane & ~15) + 15;
-    tot_lo = (uint32_t)__shfl(x, row_last);
-    int y = (int)hi;
-    y += __builtin_amdgcn_update_dpp(0, y, 0x111, 0xF, 0xF, true);
-    y += __builtin_amdgcn_update_dpp(0, y, 0x112, 0xF, 0xF, true);
-    y += __builtin_amdgcn_update_dpp(0, y, 0x114, 0xF, 0xF, true);
-    y += __builtin_amdgcn_update_dpp(0, y, 0x118, 0xF, 0xF, true);
-    tot_hi = (uint32_t)__shfl(y, row_last);
+  bsum[(size_t)g * 256 + d] = run;
+}
+
+__device__ __forceinline__ uint32_t scan_blocks(uint32_t G, uint32_t *__restrict__ bsum) {
+  const uint32_t d = threadIdx.x;
+  uint32_t run = 0;
+  for (uint32_t g0 = 0; g0 < G; g0 += 16) {
+    uint32_t c[16];
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) c[k] = (g0 + k < G) ? bsum[(size_t)(g0 + k) * 256 + d] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) {
+      if (g0 + k < G) bsum[(size_t)(g0 + k) * 256 + d] = run;
+      run += c[k];
+    }
   }
-  uint32_t run = excl;
-  for (uint32_t t = t0; t < t1; ++t) {
-    prefix[(size_t)d * T + t] = run;
-    run += tilehist[(size_t)t * 512 + d];
-  }
-  if (j == 0) {
-    ctl->hist[d] = tot_lo;
-    ctl->hist[256 + d] = tot_hi;
-  }
+  return run;
+}
+
+// Completion of a scan: the workgroup that finishes last of `expected` (by
+// the counter *done) sees every block's total. Returns true there.
+__device__ __forceinline__ bool last_of(uint32_t *done, uint32_t expected, uint32_t *s_last) {
   __threadfence();
   __syncthreads();
-  if (tid == 0) *s_last = (atomicAdd(&ctl->done, 1u) == groups - 1) ? 1u : 0u;
+  if (threadIdx.x == 0) *s_last = (atomicAdd(done, 1u) == expected - 1) ? 1u : 0u;
   __syncthreads();
   if (!*s_last) return false;
   __threadfence();
-  const uint32_t c0 = __hip_atomic_load(&ctl->hist[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t c1 = __hip_atomic_load(&ctl->hist[256 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Scan after pass 1: low-byte tile offsets and both digits' bases (the
+// reference's scanCounts, src/render/splats.js:188-212), from the chunk
+// histograms. The last block also zeroes the counter for the next sort.
+__device__ __forceinline__ bool scan_after_project(uint32_t g, uint32_t G, uint32_t T, const uint16_t *__restrict__ tilehist,
+                                                   uint32_t *__restrict__ prefix_lo, uint32_t *__restrict__ bsum_lo,
+                                                   uint32_t *__restrict__ bsum_hi, Ctl *__restrict__ ctl, uint32_t *s_w,
+                                                   uint32_t *s_last) {
+  scan_block(g, T, tilehist, 512, 0, prefix_lo, bsum_lo);
+  scan_block(g, T, tilehist, 512, 256, nullptr, bsum_hi);
+  if (!last_of(&ctl->done, G, s_last)) return false;
+  const uint32_t d = threadIdx.x;
+  const uint32_t tot_lo = scan_blocks(G, bsum_lo);
+  uint32_t tot_hi = 0;
+  for (uint32_t k = 0; k < G; ++k) tot_hi += bsum_hi[(size_t)k * 256 + d];
   uint32_t total0, total1;
-  const uint32_t e0 = block_scan_excl(c0, s_w, &total0);
-  const uint32_t e1 = block_scan_excl(c1, s_w, &total1);
-  ctl->base[tid] = e0;
-  ctl->base[256 + tid] = e1;
-  if (tid == 0) {
+  const uint32_t e0 = block_scan_excl(tot_lo, s_w, &total0);
+  const uint32_t e1 = block_scan_excl(tot_hi, s_w, &total1);
+  ctl->base[d] = e0;
+  ctl->base[256 + d] = e1;
+  if (d == 0) {
     ctl->count = total0;
     ctl->done = 0;
-    ctl->ticket_hi = 0;
     ctl->recomputed = __hip_atomic_load(&ctl->recomputed_acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ctl->recomputed_acc = 0;
   }
@@ -399,161 +406,177 @@ __device__ __forceinline__ bool scan_digits(uint32_t g, uint32_t groups, uint32_
   return true;
 }
 
+// Scan after the high-byte histograms: high-byte tile offsets.
+__device__ __forceinline__ bool scan_after_hist(uint32_t g, uint32_t G, const uint16_t *__restrict__ tilehist3,
+                                                uint32_t *__restrict__ prefix_hi, uint32_t *__restrict__ bsum3,
+                                                Ctl *__restrict__ ctl, uint32_t *s_last) {
+  const uint32_t T3 = (ctl->count + kTile - 1) / kTile;
+  scan_block(g, T3, tilehist3, 256, 0, prefix_hi, bsum3);
+  if (!last_of(&ctl->done2, G, s_last)) return false;
+  scan_blocks(G, bsum3);
+  if (threadIdx.x == 0) ctl->done2 = 0;
+  __threadfence();
+  __syncthreads();
+  return true;
+}
+
 // ---------------------------------------------------------------------------
-// Pass 2 and 3 body: one stable counting pass over an 8-bit digit on the
-// 4096-element tile t. HI = false: input is every splat in index order (keys +
-// survivor bits), digit = key & 255, tile offsets from the scan (prefix),
-// output high byte << 24 | index. HI = true: input is that output (count from
-// the control block), digit = value >> 24, tile offsets by decoupled look-back,
-// output the index.
+// The high-byte histogram of tile t of the low-byte pass's output (positions
+// t * kTile ...), the upsweep of the high-byte pass. Lanes of a wave with one
+// digit are counted once, by a ballot match, so a few hot digits (a slab seen
+// edge on) do not serialise the LDS counters.
+struct HistSmem {
+  uint32_t h[256];
+};
+
+__device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32_t *__restrict__ packed,
+                                          const Ctl *__restrict__ ctl, uint16_t *__restrict__ tilehist3) {
+  constexpr int KPT = kKeysPerThread;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t limit = ctl->count, tile0 = t * (uint32_t)kTile;
+  if (tile0 >= limit) return;
+  __syncthreads();
+  sm.h[tid] = 0;
+  __syncthreads();
+  uint32_t raw[KPT];
+  const uint32_t wbase = tile0 + w * (64u * KPT);
+#pragma unroll
+  for (int r = 0; r < KPT; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+    raw[r] = i < limit ? packed[i] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < KPT; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+    const uint64_t valid = __ballot(i < limit);
+    const uint32_t d = raw[r] >> 24;
+    const uint64_t peers = match_digit(d, valid);
+    if (i < limit && count_below(peers) == 0) atomicAdd(&sm.h[d], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  tilehist3[(size_t)t * 256 + tid] = (uint16_t)sm.h[tid];
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2 and 3 body: one stable counting pass over an 8-bit digit on super-
+// tile st, S consecutive tiles of kTile. HI = false: input is every splat in
+// index order (keys + survivor bits), digit = key & 255, output high byte
+// << 24 | index. HI = true: input is that output (V from the control block),
+// digit = value >> 24, output the index. Each tile is ranked within its waves
+// by a ballot match; the super-tile's counts per digit (known beforehand from
+// the tile histograms) place every element in LDS, and the super-tile is then
+// written out in digit runs S times longer than one tile's.
+template <int S>
 struct ScatterSmem {
-  uint32_t val[kTile];
-  uint8_t dig[kTile];
+  uint32_t val[S * kTile];
+  uint8_t dig[S * kTile];
   uint32_t wh[4][256];  // per-wave running digit counts, then wave offsets
+  uint32_t sub[S][256]; // each tile's start within the super-tile's digit run
   uint32_t lstart[256];
   uint32_t gstart[256];
   uint32_t w[4];
-  uint32_t tile;
   uint32_t nvalid;
 };
 
-// Inclusive prefix of digit d over the tiles before t, from their look-back
-// words: eight predecessors are read at once, so a wait costs one memory round
-// trip per eight tiles instead of per tile.
-__device__ __forceinline__ uint32_t look_back(const uint64_t *__restrict__ status, uint32_t t, uint32_t d,
-                                              uint32_t epoch, uint32_t *stuck) {
-  constexpr int W = 8;
-  uint32_t prefix = 0, spins = 0;
-  int p = (int)t - 1;
-  while (p >= 0) {
-    uint64_t s[W];
-#pragma unroll
-    for (int k = 0; k < W; ++k) s[k] = (p - k >= 0) ? load_status(status + (size_t)(p - k) * 256u + d) : 0ull;
-    // Sum from the nearest tile down to the first inclusive word; stop short
-    // at the first tile that has published nothing yet, and poll it again.
-    int k = 0;
-    for (; k < W && p - k >= 0; ++k) {
-      const uint32_t tag = (uint32_t)(s[k] >> 32);
-      if ((tag >> 2) != epoch || (tag & 3u) == 0u) break;
-      prefix += (uint32_t)s[k];
-      if ((tag & 3u) == kFlagInclusive) return prefix;
-    }
-    p -= k;
-    if (k == 0) {
-      // never, unless a bug: flag it and finish (and stop waiting once any wait has run out)
-      if (++spins == kSpinLimit ||
-          ((spins & 1023u) == 0 && __hip_atomic_load(stuck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-        atomicOr(stuck, 1u);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-  }
-  return prefix;
-}
-
-template <bool HI>
-__device__ __forceinline__ void scatter_tile(ScatterSmem &sm, uint32_t t, const uint16_t *__restrict__ keys,
-                                             const uint64_t *__restrict__ mask,
-                                             const uint32_t *__restrict__ in_packed, uint32_t n, uint32_t T,
-                                             const uint32_t *__restrict__ prefix, const Ctl *__restrict__ ctl,
-                                             uint64_t *__restrict__ status, uint32_t epoch,
-                                             uint32_t *__restrict__ out, uint32_t *stuck) {
+template <bool HI, int S>
+__device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, const uint16_t *__restrict__ keys,
+                                              const uint64_t *__restrict__ mask,
+                                              const uint32_t *__restrict__ in_packed, uint32_t n,
+                                              const uint16_t *__restrict__ hist, uint32_t hstride, uint32_t hoff,
+                                              const uint32_t *__restrict__ prefix, const uint32_t *__restrict__ boff,
+                                              const Ctl *__restrict__ ctl, uint32_t *__restrict__ out) {
   constexpr int KPT = kKeysPerThread;
-  constexpr int TILE = kTile;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t limit = HI ? ctl->count : n;
-  const uint32_t tile0 = t * (uint32_t)TILE;
-  if (tile0 >= limit) return;  // past the survivors (HI); never looked back to
-  __syncthreads();  // the LDS may still be read by the previous work item
-  for (uint32_t k = tid; k < 1024; k += 256) (&sm.wh[0][0])[k] = 0;
-  __syncthreads();
-  const uint32_t wbase = tile0 + w * (64u * KPT);
-  const uint32_t nwords = (n + 63u) >> 6;
-
-  // Every load of the wave's 16 rounds is issued before any is used.
-  uint32_t raw[KPT];
-  uint64_t vmask[KPT];
-  const uint32_t wave0 = tile0 + (uint32_t)__builtin_amdgcn_readfirstlane(w) * (64u * KPT);
-#pragma unroll
-  for (int r = 0; r < KPT; ++r) {
-    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-    if (HI) {
-      raw[r] = i < limit ? in_packed[i] : 0u;
-    } else {
-      raw[r] = i < n ? (uint32_t)keys[i] : 0u;
-      const uint32_t wi = (wave0 + (uint32_t)r * 64u) >> 6;
-      vmask[r] = wi < nwords ? mask[wi] : 0ull;
-    }
-  }
-  uint32_t dig[KPT], rank[KPT], val[KPT];
-  uint32_t okbits = 0;
-#pragma unroll
-  for (int r = 0; r < KPT; ++r) {
-    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-    uint64_t valid;
-    uint32_t d, value;
-    if (HI) {
-      valid = __ballot(i < limit);
-      d = raw[r] >> 24;
-      value = raw[r] & 0xFFFFFFu;
-    } else {
-      valid = vmask[r];
-      d = raw[r] & 255u;
-      value = ((raw[r] >> 8) << 24) | i;
-    }
-    const bool ok = (valid >> lane) & 1ull;
-    const uint64_t peers = match_digit(d, valid);
-    const uint32_t below = count_below(peers);
-    uint32_t before = 0;
-    if (ok) before = sm.wh[w][d];
-    if (ok && below == 0) sm.wh[w][d] = before + (uint32_t)__popcll(peers);
-    dig[r] = d;
-    rank[r] = before + below;
-    val[r] = value;
-    okbits |= (ok ? 1u : 0u) << r;
-  }
-  __syncthreads();
-  // Thread d owns digit d: wave offsets, the tile's count, its global offset.
+  const uint32_t T = (limit + kTile - 1) / kTile;
+  const uint32_t t0 = st * (uint32_t)S;
+  if (t0 >= T) return;  // past the survivors (HI)
+  const uint32_t ns = min((uint32_t)S, T - t0);
+  // Thread d: the super-tile's counts of digit d, its tiles' starts, its global start.
   const uint32_t d = tid;
-  const uint32_t c0 = sm.wh[0][d], c1 = sm.wh[1][d], c2 = sm.wh[2][d], c3 = sm.wh[3][d];
-  const uint32_t tot = c0 + c1 + c2 + c3;
-  uint32_t pre;
-  if (HI) {
-    uint64_t *st = status + (size_t)t * 256u + d;
-    if (t == 0) {
-      store_status(st, epoch, kFlagInclusive, tot);
-      pre = 0;
-    } else {
-      store_status(st, epoch, kFlagAggregate, tot);
-      pre = look_back(status, t, d, epoch, stuck);
-      store_status(st, epoch, kFlagInclusive, pre + tot);
-    }
-  } else {
-    pre = prefix[(size_t)d * T + t];
+  __syncthreads();  // the LDS may still be read by the previous work item
+  uint32_t tot = 0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    sm.sub[s][d] = tot;
+    if ((uint32_t)s < ns) tot += hist[(size_t)(t0 + s) * hstride + hoff + d];
   }
-  sm.wh[0][d] = 0;
-  sm.wh[1][d] = c0;
-  sm.wh[2][d] = c0 + c1;
-  sm.wh[3][d] = c0 + c1 + c2;
   uint32_t nvalid;
   const uint32_t lstart = block_scan_excl(tot, sm.w, &nvalid);
   sm.lstart[d] = lstart;
-  sm.gstart[d] = ctl->base[(HI ? 256u : 0u) + d] + pre;
+  sm.gstart[d] = ctl->base[(HI ? 256u : 0u) + d] + boff[(size_t)(t0 / kScanRows) * 256 + d] + prefix[(size_t)t0 * 256 + d];
   if (tid == 0) sm.nvalid = nvalid;
-  __syncthreads();
+  const uint32_t nwords = (n + 63u) >> 6;
+  for (uint32_t s = 0; s < ns; ++s) {
+    const uint32_t tile0 = (t0 + s) * (uint32_t)kTile;
+    for (uint32_t k = tid; k < 1024; k += 256) (&sm.wh[0][0])[k] = 0;
+    __syncthreads();
+    const uint32_t wbase = tile0 + w * (64u * KPT);
+    uint32_t raw[KPT];
+    uint64_t vmask[KPT];
+    const uint32_t wave0 = tile0 + (uint32_t)__builtin_amdgcn_readfirstlane(w) * (64u * KPT);
 #pragma unroll
-  for (int r = 0; r < KPT; ++r) {
-    if ((okbits >> r) & 1u) {
-      const uint32_t pos = sm.lstart[dig[r]] + sm.wh[w][dig[r]] + rank[r];
-      if (pos < (uint32_t)TILE) {  // always, when the counts agree; a guard, not a branch taken
-        sm.val[pos] = val[r];
-        sm.dig[pos] = (uint8_t)dig[r];
+    for (int r = 0; r < KPT; ++r) {
+      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+      if (HI) {
+        raw[r] = i < limit ? in_packed[i] : 0u;
+      } else {
+        raw[r] = i < n ? (uint32_t)keys[i] : 0u;
+        const uint32_t wi = (wave0 + (uint32_t)r * 64u) >> 6;
+        vmask[r] = wi < nwords ? mask[wi] : 0ull;
       }
     }
+    uint32_t dig[KPT], rank[KPT], val[KPT];
+    uint32_t okbits = 0;
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+      uint64_t valid;
+      uint32_t dd, value;
+      if (HI) {
+        valid = __ballot(i < limit);
+        dd = raw[r] >> 24;
+        value = raw[r] & 0xFFFFFFu;
+      } else {
+        valid = vmask[r];
+        dd = raw[r] & 255u;
+        value = ((raw[r] >> 8) << 24) | i;
+      }
+      const bool ok = (valid >> lane) & 1ull;
+      const uint64_t peers = match_digit(dd, valid);
+      const uint32_t below = count_below(peers);
+      uint32_t before = 0;
+      if (ok) before = sm.wh[w][dd];
+      if (ok && below == 0) sm.wh[w][dd] = before + (uint32_t)__popcll(peers);
+      dig[r] = dd;
+      rank[r] = before + below;
+      val[r] = value;
+      okbits |= (ok ? 1u : 0u) << r;
+    }
+    __syncthreads();
+    {
+      // wave offsets of digit d within this tile, plus the tile's start
+      const uint32_t c0 = sm.wh[0][d], c1 = sm.wh[1][d], c2 = sm.wh[2][d];
+      const uint32_t base = sm.lstart[d] + sm.sub[s][d];
+      sm.wh[0][d] = base;
+      sm.wh[1][d] = base + c0;
+      sm.wh[2][d] = base + c0 + c1;
+      sm.wh[3][d] = base + c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+      if ((okbits >> r) & 1u) {
+        const uint32_t pos = sm.wh[w][dig[r]] + rank[r];
+        if (pos < (uint32_t)(S * kTile)) {  // always, when the counts agree; a guard, not a branch taken
+          sm.val[pos] = val[r];
+          sm.dig[pos] = (uint8_t)dig[r];
+        }
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const uint32_t nv = min(sm.nvalid, (uint32_t)TILE);
+  const uint32_t nv = min(sm.nvalid, (uint32_t)(S * kTile));
   for (uint32_t p = tid; p < nv; p += 256) {
     const uint32_t dd = sm.dig[p];
     const uint32_t g = sm.gstart[dd] + (p - sm.lstart[dd]);

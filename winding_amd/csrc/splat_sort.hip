@@ -44,29 +44,37 @@ __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, 
   project_chunk<STD>(sm, rec, n, v, keys, mask, cov, tilehist, &ctl->recomputed_acc, blockIdx.x);
 }
 
-// Between the passes: a workgroup a digit.
+// Scan after pass 1: a workgroup a block of kScanRows tiles.
 __global__ __launch_bounds__(256) void k_scan(uint32_t T, const uint16_t *__restrict__ tilehist,
-                                              uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl) {
+                                              uint32_t *__restrict__ prefix_lo, uint32_t *__restrict__ bsum_lo,
+                                              uint32_t *__restrict__ bsum_hi, Ctl *__restrict__ ctl) {
   __shared__ uint32_t s_w[4], s_last;
-  scan_digits<1>(blockIdx.x, gridDim.x, T, tilehist, prefix, ctl, s_w, &s_last);
+  scan_after_project(blockIdx.x, gridDim.x, T, tilehist, prefix_lo, bsum_lo, bsum_hi, ctl, s_w, &s_last);
 }
 
-// Pass 2: tile t is workgroup t (its offsets are known); pass 3: tiles are
-// taken in ticket order, so every tile a look-back waits on belongs to a
-// workgroup that is already running.
+// Pass 2 and 3: a workgroup a super-tile.
 template <bool HI>
 __global__ __launch_bounds__(256) void k_scatter(const uint16_t *__restrict__ keys, const uint64_t *__restrict__ mask,
-                                                 const uint32_t *__restrict__ in_packed, uint32_t n, uint32_t T,
-                                                 const uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl,
-                                                 uint64_t *__restrict__ status, uint32_t epoch,
-                                                 uint32_t *__restrict__ out) {
-  __shared__ ScatterSmem sm;
-  if (HI) {
-    if (threadIdx.x == 0) sm.tile = atomicAdd(&ctl->ticket_hi, 1u);
-    __syncthreads();
-  }
-  const uint32_t t = HI ? sm.tile : blockIdx.x;
-  scatter_tile<HI>(sm, t, keys, mask, in_packed, n, T, prefix, ctl, status, epoch, out, &ctl->stuck);
+                                                 const uint32_t *__restrict__ in_packed, uint32_t n,
+                                                 const uint16_t *__restrict__ hist, uint32_t hstride,
+                                                 const uint32_t *__restrict__ prefix, const uint32_t *__restrict__ boff,
+                                                 const Ctl *__restrict__ ctl, uint32_t *__restrict__ out) {
+  __shared__ ScatterSmem<kSuper> sm;
+  scatter_super<HI, kSuper>(sm, blockIdx.x, keys, mask, in_packed, n, hist, hstride, 0, prefix, boff, ctl, out);
+}
+
+// Upsweep of pass 3: a workgroup a tile of pass 2's output.
+__global__ __launch_bounds__(256) void k_hist(const uint32_t *__restrict__ packed, const Ctl *__restrict__ ctl,
+                                              uint16_t *__restrict__ tilehist3) {
+  __shared__ HistSmem sm;
+  hist_tile(sm, blockIdx.x, packed, ctl, tilehist3);
+}
+
+// Scan after the upsweep.
+__global__ __launch_bounds__(256) void k_scan2(const uint16_t *__restrict__ tilehist3, uint32_t *__restrict__ prefix_hi,
+                                               uint32_t *__restrict__ bsum3, Ctl *__restrict__ ctl) {
+  __shared__ uint32_t s_last;
+  scan_after_hist(blockIdx.x, gridDim.x, tilehist3, prefix_hi, bsum3, ctl, &s_last);
 }
 
 // ---------------------------------------------------------------------------
@@ -199,8 +207,8 @@ ViewK make_viewk(const wd_view &v) {
   return k;
 }
 
-const char *const kKernelNames[kKernelCount] = {"k_bounds", "k_project", "k_scan", "k_scatter_lo",
-                                                "k_scatter_hi", "k_batch", "k_mlp", "k_learned"};
+const char *const kKernelNames[kKernelCount] = {"k_bounds", "k_project",    "k_scan",  "k_scatter_lo", "k_hist",
+                                                "k_scan2",  "k_scatter_hi", "k_batch", "k_mlp",        "k_learned"};
 
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8) {
   static const uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0, 0xFFFFFFFFu, 0};
@@ -217,39 +225,39 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
   const uint32_t n = (uint32_t)cloud->n;
   const ViewK vk = make_viewk(view);
   const bool std_form = is_std(view);
-  if (++st->epoch >= (1u << 29)) {
-    WD_HIP(hipMemsetAsync(st->status_hi, 0, st->tiles * 256 * sizeof(uint64_t), ctx->stream), "sort: status reset");
-    st->epoch = 1;
-  }
   st->last_n = n;
   const uint32_t tiles = (n + kTile - 1) / kTile;
+  const uint32_t blocks = (tiles + kScanRows - 1) / kScanRows;
+  const uint32_t supers = (tiles + kSuper - 1) / kSuper;
   const uint4 *rec = (const uint4 *)cloud->records;
   hipEvent_t ev = nullptr;
-  if (ctx->prof_begin(kKProject, &ev)) return WD_E_HIP;
+#define WD_LAUNCH(id, what, ...)                  \
+  do {                                            \
+    if (ctx->prof_begin(id, &ev)) return WD_E_HIP; \
+    hipLaunchKernelGGL(__VA_ARGS__);              \
+    WD_HIP(hipGetLastError(), what);              \
+    if (ctx->prof_end(id, ev)) return WD_E_HIP;   \
+  } while (0)
   if (std_form)
-    hipLaunchKernelGGL(k_project<true>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk, st->keys, st->mask,
-                       (float4 *)st->cov, st->tilehist, st->ctl);
+    WD_LAUNCH(kKProject, "sort: project launch", k_project<true>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk,
+              st->keys, st->mask, (float4 *)st->cov, st->tilehist, st->ctl);
   else
-    hipLaunchKernelGGL(k_project<false>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk, st->keys, st->mask,
-                       (float4 *)st->cov, st->tilehist, st->ctl);
-  WD_HIP(hipGetLastError(), "sort: project launch");
-  if (ctx->prof_end(kKProject, ev)) return WD_E_HIP;
-  if (ctx->prof_begin(kKScan, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(k_scan, dim3(256), dim3(256), 0, ctx->stream, tiles, st->tilehist, st->prefix, st->ctl);
-  WD_HIP(hipGetLastError(), "sort: scan launch");
-  if (ctx->prof_end(kKScan, ev)) return WD_E_HIP;
-  if (ctx->prof_begin(kKScatterLo, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(k_scatter<false>, dim3(tiles), dim3(256), 0, ctx->stream, st->keys, st->mask,
-                     (const uint32_t *)nullptr, n, tiles, st->prefix, st->ctl, (uint64_t *)nullptr, st->epoch,
-                     st->packed);
-  WD_HIP(hipGetLastError(), "sort: low-byte scatter launch");
-  if (ctx->prof_end(kKScatterLo, ev)) return WD_E_HIP;
-  if (ctx->prof_begin(kKScatterHi, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(k_scatter<true>, dim3(tiles), dim3(256), 0, ctx->stream, (const uint16_t *)nullptr,
-                     (const uint64_t *)nullptr, st->packed, n, tiles, (const uint32_t *)nullptr, st->ctl,
-                     st->status_hi, st->epoch, st->order);
-  WD_HIP(hipGetLastError(), "sort: high-byte scatter launch");
-  return ctx->prof_end(kKScatterHi, ev);
+    WD_LAUNCH(kKProject, "sort: project launch", k_project<false>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk,
+              st->keys, st->mask, (float4 *)st->cov, st->tilehist, st->ctl);
+  WD_LAUNCH(kKScan, "sort: scan launch", k_scan, dim3(blocks), dim3(256), 0, ctx->stream, tiles, st->tilehist,
+            st->prefix_lo, st->bsum_lo, st->bsum_hi, st->ctl);
+  WD_LAUNCH(kKScatterLo, "sort: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0, ctx->stream,
+            st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->bsum_lo, st->ctl,
+            st->packed);
+  WD_LAUNCH(kKHist, "sort: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, st->packed, st->ctl,
+            st->tilehist3);
+  WD_LAUNCH(kKScan2, "sort: second scan launch", k_scan2, dim3(blocks), dim3(256), 0, ctx->stream, st->tilehist3,
+            st->prefix_hi, st->bsum3, st->ctl);
+  WD_LAUNCH(kKScatterHi, "sort: high-byte pass launch", k_scatter<true>, dim3(supers), dim3(256), 0, ctx->stream,
+            (const uint16_t *)nullptr, (const uint64_t *)nullptr, st->packed, n, st->tilehist3, 256u, st->prefix_hi,
+            st->bsum3, st->ctl, st->order);
+#undef WD_LAUNCH
+  return WD_OK;
 }
 
 int launch_copy(wd_context *ctx, const void *src, void *dst, size_t bytes) {

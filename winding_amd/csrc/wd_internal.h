@@ -26,6 +26,7 @@ int hip_fail(hipError_t e, const char *what);
 constexpr int kThreads = 256;        // workgroup size of every kernel here
 constexpr int kTile = 4096;          // splats per scatter tile: 256 threads x 16
 constexpr int kKeysPerThread = 16;
+constexpr uint32_t kScanRows = 32;   // tiles a first-level scan block takes
 
 // Flags of a look-back status word: value in the low 32 bits, epoch << 2 | flag above.
 constexpr uint32_t kFlagAggregate = 1;
@@ -41,8 +42,8 @@ struct Ctl {
   uint32_t base[512];     // exclusive prefix of hist: where each digit starts
   uint32_t count;         // V, survivors of the last sort
   uint32_t done;          // projection workgroups finished
-  uint32_t ticket_lo;     // tile tickets of the two scatter passes
-  uint32_t ticket_hi;
+  uint32_t done2;         // high-byte scan blocks finished
+  uint32_t spare;
   uint32_t recomputed;    // FP64 area decisions in the last sort
   uint32_t recomputed_acc;
   uint32_t fallback;      // learned sort: 1 when the exact path ran
@@ -84,13 +85,14 @@ struct BatchJob {
   uint32_t *packed;
   uint32_t *order;
   Ctl *ctl;
-  uint16_t *tilehist;
-  uint32_t *prefix;
-  uint64_t *status_hi;
+  uint16_t *tilehist, *tilehist3;
+  uint32_t *prefix_lo, *prefix_hi, *bsum_lo, *bsum_hi, *bsum3;
   uint32_t n, std_form;
   uint32_t tiles;    // projection chunks = scatter tiles, of kTile splats
-  uint32_t pad;
+  uint32_t blocks;   // scan blocks of kScanRows tiles
 };
+// Tiles a scatter work item takes (a super-tile).
+constexpr int kSuper = 2;
 // One GEMM of the predictor on the matrix cores: C = alpha * A B, element (m, k)
 // of A at A[m * sa_m + k * sa_k] (f32 or bf16), (k, n) of B likewise, then
 // + bias[n], relu, zero where mask <= 0, += C, and an optional bf16 copy.
@@ -121,8 +123,10 @@ struct BatchCtl {
   uint32_t exited;
   uint32_t runs;
   uint32_t stuck;
-  // followed by proj_done[max_jobs], ready[max_jobs], lo_done[max_jobs]
+  // followed by five counters a job: projection chunks done, first scan done,
+  // low-byte super-tiles done, high-byte histograms done, second scan done
 };
+constexpr int kBatchCounters = 5;
 
 }  // namespace wd
 
@@ -179,16 +183,19 @@ struct wd_state {
   uint32_t *packed = nullptr;  // low-digit pass output: high byte << 24 | index
   uint32_t *order = nullptr;   // the draw order
   float *cov = nullptr;        // n x 4 floats
-  uint16_t *tilehist = nullptr;  // per 4096-splat chunk: 256 low-byte then 256 high-byte counts
-  uint32_t *prefix = nullptr;    // low-byte pass offsets, digit-major: [256][tiles]
-  uint64_t *status_hi = nullptr; // high-byte pass look-back words, tiles x 256
-  uint64_t tiles = 0;
+  uint16_t *tilehist = nullptr;   // per 4096-splat chunk: 256 low-byte then 256 high-byte counts
+  uint16_t *tilehist3 = nullptr;  // per tile of the low-byte pass's output: 256 high-byte counts
+  uint32_t *prefix_lo = nullptr, *prefix_hi = nullptr;  // [tiles][256] offsets within a scan block
+  uint32_t *bsum_lo = nullptr, *bsum_hi = nullptr, *bsum3 = nullptr;  // [blocks][256]
+  uint64_t tiles = 0, blocks = 0;
   uint32_t magic = 0x53544154;
 };
 
 namespace wd {
 
-enum KernelId { kKBounds = 0, kKProject, kKScan, kKScatterLo, kKScatterHi, kKBatch, kKMlp, kKLearned, kKernelCount };
+enum KernelId {
+  kKBounds = 0, kKProject, kKScan, kKScatterLo, kKHist, kKScan2, kKScatterHi, kKBatch, kKMlp, kKLearned, kKernelCount
+};
 extern const char *const kKernelNames[kKernelCount];
 
 // Kernel launchers (splat_sort.hip). Enqueue on ctx->stream.
