@@ -23,11 +23,30 @@ HBM_PEAK_GBS = 8000.0
 
 
 def cpu_baseline(recs, view, budget_s=15.0):
-    """The oracle (a one-thread C restatement of the reference's path) on a prefix of the cloud."""
+    """The JavaScript CPU path (oracle/splat_oracle.js, a one-thread restatement of the reference's
+    put + makeKeys + stable counting sort) in Node on this host, on a prefix of the cloud; the C
+    restatement when Node is absent."""
+    import shutil
+    import subprocess
+    import tempfile
     import numpy as np
-    import oracle
     n = min(len(recs), 1 << 20)
     sample = np.ascontiguousarray(recs[:n])
+    node = shutil.which("node")
+    if node:
+        with tempfile.TemporaryDirectory() as tmp:
+            rp, vp = os.path.join(tmp, "records.bin"), os.path.join(tmp, "view.bin")
+            sample.tofile(rp)
+            view.as_array().tofile(vp)
+            out = subprocess.run([node, os.path.join(ROOT, "oracle", "run_js.js"), rp, vp, "--time", str(budget_s)],
+                                 capture_output=True, text=True, timeout=budget_s * 4 + 60)
+        if out.returncode == 0 and out.stdout.strip():
+            r = json.loads(out.stdout.strip().splitlines()[-1])
+            return {"value": r["splats_per_s"] / 1e6, "unit": "M splats/s", "cores": r["threads"], "kind": "port",
+                    "sample": "first %d splats of the same cloud and view; median of %d runs of the JS CPU path "
+                              "(oracle/splat_oracle.js) in Node %s, 1 thread, on a %d-CPU host (%s)"
+                              % (n, r["runs"], r["node"], r["host_cpus"], r["cpu_model"].strip())}
+    import oracle
     t0 = time.perf_counter()
     oracle.run(sample, view)
     t = time.perf_counter() - t0
