@@ -61,12 +61,79 @@ def cpu_baseline(recs, view, budget_s=15.0):
             "sample": "first %d splats of the same cloud and view, median of %d runs, oracle/splat_oracle.c" % (n, len(times))}
 
 
+def bench_batch(args):
+    """64 scenes of log-uniform size (about 32M splats), one random orbit view each: one hipGraph launch
+    of the batch (schedule picked by the autotuner) against the same 64 sorts as single-view launches."""
+    import winding_amd as wa
+    wl = wa.workloads
+    seed, sizes = wl.batch_sizes()
+    st = wl.Stream(seed)
+    ctx = wa.Context(0)
+    jobs = []
+    for j, n in enumerate(sizes):
+        recs = wa.synth_records(0, 1000 + j, n)
+        view = None
+        while view is None:
+            view = wl.view_for(recs, wl.orbit_pose(st), 1280, 720, cull=args.cull)
+        jobs.append((wa.Cloud(ctx, recs), view, wa.State(ctx, n)))
+        del recs
+    total = sum(sizes)
+    batch = wa.Batch(ctx, len(jobs))
+    batch.set(jobs)
+    sched, ms_phased, ms_persistent = batch.tune(reps=5)
+    for _ in range(args.warmup):
+        batch.run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    v = sum(state.stats().count for _, _, state in jobs)
+    # the same 64 sorts, one launch sequence each
+    for _ in range(max(1, args.warmup // 4)):
+        for cloud, view, state in jobs:
+            state.sort(cloud, view)
+    ctx.sync()
+    reps = max(3, args.steps // 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for cloud, view, state in jobs:
+            state.sort(cloud, view)
+    ctx.sync()
+    ms_plain = (time.perf_counter() - t0) * 1e3 / reps
+    path_bytes = 32.0 * total + 20.0 * v
+    line = {
+        "metric": "M splats projected+sorted/s and % HBM peak; batched vs plain",
+        "value": total / (ms * 1e-3) / 1e6,
+        "unit": "M splats/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded generator, winding_amd/workloads.py)",
+        "config": {"workload": "batch-64", "scenes": len(sizes), "splats": total, "survivors": v,
+                   "width": 1280, "height": 720, "cull": args.cull, "size_seed": seed,
+                   "parallelism": "single GPU"},
+        "batch": {"schedule": ["phased", "persistent"][sched], "tuned_ms_phased": ms_phased,
+                  "tuned_ms_persistent": ms_persistent, "plain_ms": ms_plain,
+                  "plain_value": total / (ms_plain * 1e-3) / 1e6, "speedup_vs_plain": ms_plain / ms},
+        "path": {"bytes_per_step": path_bytes, "achieved_gbs": path_bytes / (ms * 1e-3) / 1e9,
+                 "frac_of_peak": path_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    batch.destroy()
+    for cloud, _, state in jobs:
+        cloud.destroy()
+        state.destroy()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="slab-8m", choices=["plain-64k", "plain-1m", "slab-8m"])
+    ap.add_argument("--workload", default="slab-8m", choices=["plain-64k", "plain-1m", "slab-8m", "batch-64"])
     ap.add_argument("--cull", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -87,6 +154,8 @@ def main():
         return 2
 
     import winding_amd as wa
+    if args.workload == "batch-64":
+        return bench_batch(args)
     recs, view, spec = wa.workloads.plain(args.workload, cull=args.cull)
     n = len(recs)
     ctx = wa.Context(0)

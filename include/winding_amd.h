@@ -187,7 +187,7 @@ int wd_profile(wd_context *ctx, int on);
 int wd_profile_read(wd_context *ctx, const char **names, double *ms,
                     uint64_t *launches, int capacity, int *n);
 
-/* ---- batches: many (cloud, view) sorts in one persistent launch ------- */
+/* ---- batches: many (cloud, view) sorts in one graph launch ------------ */
 
 /* One sort of a batch: what wd_sort takes. No two jobs may share a state. */
 typedef struct wd_job {
@@ -199,10 +199,23 @@ typedef struct wd_job {
 /*
  * A batch replaces the reference's per-cloud loop of dispatches in
  * _encodeSort (src/render/splats.js:676-697): every job's projection and both
- * scatter passes run in ONE persistent kernel, taken tile by tile from a
- * ticketed queue, replayed from a hipGraph.
+ * scatter passes run from one hipGraph launch, on one of two schedules:
+ *   WD_SCHEDULE_PERSISTENT  ONE persistent kernel taking every job's work
+ *                           tile by tile from a ticketed queue;
+ *   WD_SCHEDULE_PHASED      six kernels, each over every job's tiles.
+ * A new batch runs phased; wd_batch_tune times both and keeps the faster.
  */
+#define WD_SCHEDULE_PHASED 0
+#define WD_SCHEDULE_PERSISTENT 1
 int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out);
+/* Picks the schedule (re-captures the graph on the next run). */
+int wd_batch_schedule(wd_batch *batch, int schedule);
+/*
+ * The autotuner: runs the set jobs `reps` times on each schedule, keeps the
+ * faster, and reports it and both mean times (milliseconds a run).
+ */
+int wd_batch_tune(wd_context *ctx, wd_batch *batch, uint32_t reps,
+                  int *schedule, double *ms_phased, double *ms_persistent);
 /*
  * Sets the jobs (views may change between runs at no cost; other clouds or
  * states re-capture the graph on the next run).
@@ -246,6 +259,33 @@ int wd_mlp_predict(wd_context *ctx, wd_mlp *mlp, const float *poses, uint32_t n,
                    float *probs);
 /* The 12 pose floats of a view: the model-view's top rows, translation / 10. */
 int wd_pose_features(const wd_view *view, float out[12]);
+
+/* ---- the learned sort (no counterpart in the reference) --------------- */
+
+typedef struct wd_learned wd_learned;
+/* Buffers for learned runs of up to max_views (1..8) views of one cloud. */
+int wd_learned_create(wd_context *ctx, uint32_t max_views, wd_learned **out);
+/*
+ * Sorts `cloud` for n_views views into states[0..n_views) with the predictor
+ * `mlp`: its depth CDF cuts each view's key line into bins, every record is
+ * read once for all the views, and each bin is sorted in LDS. The result is
+ * the exact order (what wd_sort gives), bit for bit: each view's bins are
+ * checked on the GPU (no bin above `cap` survivors; 0 = 8 x n / 256 + 4096),
+ * and when any view fails, the exact path re-sorts the run's views in the
+ * same enqueue; wd_state_stats then reports fallback = 1 and the bins over
+ * the cap for that view. Enqueued on the context's stream, no host wait.
+ * Clouds of up to 2^23 splats.
+ */
+int wd_learned_run(wd_context *ctx, wd_learned *learned, wd_mlp *mlp,
+                   const wd_cloud *cloud, const wd_view *views,
+                   wd_state *const *states, uint32_t n_views, uint32_t cap);
+int wd_learned_destroy(wd_learned *learned);
+/*
+ * Training targets from an exact sort in `state`: edges[k] = the key at rank
+ * floor(k V / 256) (k = 1..255), edges[0] = 0, edges[256] = 65536. The
+ * predictor's target widths are (edges[k+1] - edges[k]) / 65536.
+ */
+int wd_state_quantiles(wd_context *ctx, wd_state *state, uint32_t edges[257]);
 
 /* ---- synthetic inputs: the workloads' seeded generator ---------------- */
 

@@ -1,5 +1,5 @@
-"""The batch path (one persistent launch over many (cloud, view) jobs, replayed from a hipGraph) against
-the CPU oracle, job by job: order bit-exact, survivor count, cov2d within 1e-5."""
+"""The batch path (many (cloud, view) jobs in one hipGraph launch, on the phased and the persistent
+schedule) against the CPU oracle, job by job: order bit-exact, survivor count, cov2d within 1e-5."""
 import numpy as np
 import pytest
 
@@ -21,7 +21,8 @@ def check_job(state, recs, view, label):
         assert err <= 1e-5, "%s: cov2d error %g" % (label, err)
 
 
-def test_batch_uneven_jobs_and_replays(ctx):
+@pytest.mark.parametrize("schedule", [wa.WD_SCHEDULE_PHASED, wa.WD_SCHEDULE_PERSISTENT])
+def test_batch_uneven_jobs_and_replays(ctx, schedule):
     sizes = [1, 100, 4096, 5000, 70001, 300000, 12289, 65536]
     st = wl.Stream(11)
     jobs, data = [], []
@@ -36,6 +37,7 @@ def test_batch_uneven_jobs_and_replays(ctx):
         data.append(recs)
     batch = wa.Batch(ctx, len(jobs))
     try:
+        batch.schedule(schedule)
         batch.set(jobs)
         for replay in range(3):
             if replay == 2:  # new views, same clouds and states: the captured graph is replayed
@@ -72,12 +74,18 @@ def test_batch_64_scenes_32m(ctx):
             data.append(recs)
         batch = wa.Batch(ctx, 64)
         batch.set(jobs)
-        batch.run()
-        ctx.sync()
-        assert batch.stuck() == 0
-        for (cloud, view, state), recs in zip(jobs, data):
-            check_job(state, recs, view, "scene n=%d" % len(recs))
-        print("64 scenes, %d splats" % sum(sizes), flush=True)
+        sched, ms_phased, ms_persistent = batch.tune(reps=3)
+        print("64 scenes, %d splats: phased %.3f ms, persistent %.3f ms, kept %d"
+              % (sum(sizes), ms_phased, ms_persistent, sched), flush=True)
+        for which in (wa.WD_SCHEDULE_PERSISTENT, wa.WD_SCHEDULE_PHASED):
+            batch.schedule(which)
+            for cloud, view, state in jobs:  # stale results must not pass for new ones
+                state.sort(cloud, wl.view_for(data[0], (0.0, 0.0, 50.0), 64, 64) or view)
+            batch.run()
+            ctx.sync()
+            assert batch.stuck() == 0
+            for (cloud, view, state), recs in zip(jobs, data):
+                check_job(state, recs, view, "schedule %d scene n=%d" % (which, len(recs)))
         batch.destroy()
     finally:
         for cloud, _, state in jobs:

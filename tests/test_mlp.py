@@ -81,9 +81,9 @@ def test_gpu_gradients_match_oracle(ctx):
         assert max(errs.values()) <= 2e-2, errs
         cos = float(g.astype(np.float64) @ w_g / (np.linalg.norm(g) * np.linalg.norm(w_g)))
         assert cos > 0.9999
-        # the same inputs give the same gradients, bit for bit
+        # the same inputs give the same gradients, bit for bit (the loss is summed by atomics: to rounding)
         loss2, g2 = mlp.grad(X.astype(np.float32), T.astype(np.float32))
-        assert loss2 == loss and np.array_equal(g2, g)
+        assert abs(loss2 - loss) <= 1e-5 * abs(loss) and np.array_equal(g2, g)
     finally:
         mlp.destroy()
 

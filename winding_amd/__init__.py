@@ -26,7 +26,8 @@ EXPORTS = [
     "wd_cloud_create", "wd_cloud_create_device", "wd_cloud_info", "wd_cloud_records", "wd_cloud_destroy",
     "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
     "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_profile", "wd_profile_read",
-    "wd_batch_create", "wd_batch_set", "wd_batch_run", "wd_batch_stuck", "wd_batch_destroy",
+    "wd_batch_create", "wd_batch_schedule", "wd_batch_tune", "wd_batch_set", "wd_batch_run", "wd_batch_stuck",
+    "wd_batch_destroy",
     "wd_mlp_create", "wd_mlp_destroy", "wd_mlp_set", "wd_mlp_get", "wd_mlp_save", "wd_mlp_load",
     "wd_mlp_grad", "wd_mlp_train", "wd_mlp_predict", "wd_pose_features",
     "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest", "wd_debug_copy",
@@ -318,8 +319,12 @@ class State:
 from . import workloads  # noqa: E402  (needs the names above)
 
 
+WD_SCHEDULE_PHASED, WD_SCHEDULE_PERSISTENT = 0, 1
+
+
 class Batch:
-    """Many (cloud, view, state) sorts in one persistent launch, replayed from a hipGraph (wd_batch_*)."""
+    """Many (cloud, view, state) sorts in one hipGraph launch (wd_batch_*): six phase kernels over
+    every job, or one persistent kernel over a ticketed queue."""
 
     def __init__(self, ctx, max_jobs):
         p = C.c_void_p()
@@ -338,6 +343,15 @@ class Batch:
 
     def run(self):
         check(lib().wd_batch_run(self.ctx.ptr, self.ptr))
+
+    def schedule(self, which):
+        check(lib().wd_batch_schedule(self.ptr, which))
+
+    def tune(self, reps=5):
+        """Times both schedules on the set jobs, keeps the faster: (schedule, ms phased, ms persistent)."""
+        sched, a, b = C.c_int(), C.c_double(), C.c_double()
+        check(lib().wd_batch_tune(self.ctx.ptr, self.ptr, reps, C.byref(sched), C.byref(a), C.byref(b)))
+        return sched.value, a.value, b.value
 
     def stuck(self):
         s = C.c_uint32()

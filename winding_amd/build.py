@@ -32,8 +32,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["splat_sort.hip", "batch.hip", "mlp.hip"]
-HOST_SOURCES = ["api.cpp", "synth.cpp", "batch_host.cpp", "mlp_host.cpp"]
+HIP_SOURCES = ["splat_sort.hip", "batch.hip", "mlp.hip", "learned.hip"]
+HOST_SOURCES = ["api.cpp", "synth.cpp", "batch_host.cpp", "mlp_host.cpp", "learned_host.cpp"]
 
 
 def _run(cmd, cwd=ROOT):

@@ -287,14 +287,10 @@ int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
   alloc((void **)&s->packed, capacity * sizeof(uint32_t));
   alloc((void **)&s->order, capacity * sizeof(uint32_t));
   alloc((void **)&s->cov, capacity * 4 * sizeof(float));
-  s->blocks = (s->tiles + wd::kScanRows - 1) / wd::kScanRows;
   alloc((void **)&s->tilehist, s->tiles * 512 * sizeof(uint16_t));
   alloc((void **)&s->tilehist3, s->tiles * 256 * sizeof(uint16_t));
   alloc((void **)&s->prefix_lo, s->tiles * 256 * sizeof(uint32_t));
   alloc((void **)&s->prefix_hi, s->tiles * 256 * sizeof(uint32_t));
-  alloc((void **)&s->bsum_lo, s->blocks * 256 * sizeof(uint32_t));
-  alloc((void **)&s->bsum_hi, s->blocks * 256 * sizeof(uint32_t));
-  alloc((void **)&s->bsum3, s->blocks * 256 * sizeof(uint32_t));
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     wd_state_destroy(s);
@@ -310,7 +306,7 @@ int wd_state_destroy(wd_state *s) {
   hipStreamSynchronize(s->ctx->stream);
   for (void *p : {(void *)s->ctl, (void *)s->keys, (void *)s->mask, (void *)s->packed, (void *)s->order,
                   (void *)s->cov, (void *)s->tilehist, (void *)s->tilehist3, (void *)s->prefix_lo,
-                  (void *)s->prefix_hi, (void *)s->bsum_lo, (void *)s->bsum_hi, (void *)s->bsum3})
+                  (void *)s->prefix_hi})
     if (p) hipFree(p);
   s->magic = 0;
   delete s;
@@ -348,13 +344,13 @@ int wd_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_stat
 int wd_state_stats(wd_context *ctx, wd_state *s, wd_stats *stats) {
   if (!ctx || !s || s->magic != 0x53544154 || !stats) return fail(WD_E_ARG, "stats: null argument or destroyed state");
   wd::Ctl head;
-  WD_HIP(hipMemcpyAsync(&head.count, &s->ctl->count, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+  WD_HIP(hipMemcpyAsync(&head.count, &s->ctl->count, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
          "stats: read");
   WD_HIP(hipStreamSynchronize(ctx->stream), "stats: sync");
   if (head.stuck) {
     const uint32_t zero = 0;
     hipMemcpy(&s->ctl->stuck, &zero, sizeof zero, hipMemcpyHostToDevice);
-    return fail(WD_E_STATE, "sort: a look-back wait ran out; the result is not valid");
+    return fail(WD_E_STATE, "sort: a bounded wait ran out; the result is not valid");
   }
   stats->count = head.count;
   stats->recomputed = head.recomputed;

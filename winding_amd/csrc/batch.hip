@@ -1,19 +1,25 @@
-// The batch kernel: many (cloud, view) sorts in one persistent launch.
+// Batches: many (cloud, view) sorts in one graph launch, two schedules.
 //
-// Work is a single ticketed queue of items in six runs, every job in each:
-//   [projection chunks][scans][low-byte super-tiles][high-byte histograms]
-//   [second scans][high-byte super-tiles]
-// A workgroup takes the next ticket, does that item, and takes another, until
-// the queue is empty. An item only ever waits on items with smaller tickets --
-// a job's scan on its chunks, its low-byte pass on its scan, its histograms on
-// its low-byte pass, its second scan on its histograms, its high-byte pass on
-// its second scan -- and smaller tickets were taken by workgroups already
-// running, so every wait ends, whatever the grid size. Jobs of 10k and 2M
-// splats share the machine tile by tile instead of launch by launch.
+// Persistent (one kernel): a single ticketed queue of items in six runs,
+// every job in each:
+//   [projection chunks][scan column groups][low-byte super-tiles]
+//   [high-byte histograms][second-scan column groups][high-byte super-tiles]
+// A workgroup takes the next ticket, does that item, and takes another,
+// until the queue is empty. An item only ever waits on items with smaller
+// tickets -- a job's scan on its chunks, its low-byte pass on its scan, its
+// histograms on its low-byte pass, its second scan on its histograms, its
+// high-byte pass on its second scan -- and smaller tickets were taken by
+// workgroups already running, so every wait ends, whatever the grid size.
+// Jobs of 10k and 2M splats share the machine tile by tile instead of launch
+// by launch. The price is a release fence after every item: on a chip whose
+// L2 is one per XCD, each is a write-back of this XCD's L2.
 //
-// Nothing is reset from the host between runs: the last workgroup to leave
-// zeroes the tickets and the per-job counters, so the same launch can be
-// replayed from a hipGraph.
+// Phased (six kernels): each of the six runs is one launch over every job's
+// items; the launch boundaries are the only barriers, so no fence runs.
+//
+// Nothing is reset from the host between runs: the persistent kernel's last
+// workgroup out zeroes the tickets and the per-job counters, so either
+// schedule replays from a hipGraph.
 #include "splat_device.h"
 
 namespace wd {
@@ -30,16 +36,18 @@ __device__ __forceinline__ uint32_t find_job(const uint32_t *__restrict__ starts
 }
 
 // Bounded: a wait that runs out flags the batch as failed and goes on, so a
-// bug shows as an error from the host, never as a hung GPU.
+// bug shows as an error from the host, never as a hung GPU. Polls relaxed;
+// one acquire fence once the count is in.
 __device__ __forceinline__ void wait_value(const uint32_t *p, uint32_t want, uint32_t *stuck) {
-  for (uint32_t spins = 0; __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want; ++spins) {
+  for (uint32_t spins = 0; __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want; ++spins) {
     if (spins == kSpinLimit ||
         ((spins & 1023u) == 1023u && __hip_atomic_load(stuck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
       atomicOr(stuck, 1u);
-      return;
+      break;
     }
     __builtin_amdgcn_s_sleep(2);
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 __device__ __forceinline__ void wait_for(const uint32_t *p, uint32_t want, uint32_t *stuck) {
@@ -47,30 +55,63 @@ __device__ __forceinline__ void wait_for(const uint32_t *p, uint32_t want, uint3
   __syncthreads();
 }
 
-// This workgroup's item is done: its writes are visible before the count is.
+// This workgroup's item is done: every thread's writes are released before
+// the count is.
 __device__ __forceinline__ void count_done(uint32_t *p) {
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) atomicAdd(p, 1u);
 }
 
+// Item bodies shared by both schedules.
+__device__ __forceinline__ void item_project(ProjectSmem &sm, const BatchJob &J, uint32_t t) {
+  if (J.std_form)
+    project_chunk<true>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
+                        &J.ctl->recomputed_acc, t);
+  else
+    project_chunk<false>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
+                         &J.ctl->recomputed_acc, t);
+}
+
+__device__ __forceinline__ void item_scan(const BatchJob &J, uint32_t cg, uint32_t (*s_part)[64]) {
+  scan_cols(cg * kScanCols, J.tiles, J.tilehist, 512, J.prefix_lo, 256, 256, J.ctl->total, s_part);
+  if (cg == 0 && threadIdx.x == 0) {
+    J.ctl->recomputed = J.ctl->recomputed_acc;
+    J.ctl->recomputed_acc = 0;
+  }
+}
+
+__device__ __forceinline__ void item_scan2(const BatchJob &J, uint32_t cg, uint32_t (*s_part)[64]) {
+  const uint32_t T3 = (J.ctl->count + kTile - 1) / kTile;
+  scan_cols(cg * kScanCols, T3, J.tilehist3, 256, J.prefix_hi, 256, 256, J.ctl->total3, s_part);
+}
+
+constexpr uint32_t kScanItems = 512 / kScanCols, kScan2Items = 256 / kScanCols;
+
+// gate: when given, the whole launch is skipped unless *gate is nonzero (the
+// learned sort's fallback: it runs, for every job, when any view failed its
+// check).
 __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs, const uint32_t *__restrict__ starts,
-                                               uint32_t nj, uint32_t max_jobs, BatchCtl *__restrict__ bc) {
+                                               uint32_t nj, uint32_t max_jobs, BatchCtl *__restrict__ bc,
+                                               const uint32_t *__restrict__ gate) {
+  if (gate && __hip_atomic_load(gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
   __shared__ union {
     ProjectSmem p;
     HistSmem h;
     ScatterSmem<kSuper> s;
+    uint32_t part[kScanCols][64];
   } sm;
-  __shared__ uint32_t s_item, s_w[4], s_last;
+  __shared__ uint32_t s_item;
   uint32_t *proj_done = reinterpret_cast<uint32_t *>(bc + 1);
   uint32_t *scan_done = proj_done + max_jobs;
   uint32_t *lo_done = scan_done + max_jobs;
   uint32_t *hist_done = lo_done + max_jobs;
   uint32_t *scan2_done = hist_done + max_jobs;
-  // starts: per job item starts of tiles, scan blocks and super-tiles, nj + 1 each
-  const uint32_t *t_start = starts, *b_start = starts + (nj + 1), *u_start = starts + 2 * (nj + 1);
-  const uint32_t nt = t_start[nj], nb = b_start[nj], nu = u_start[nj];
-  const uint32_t e0 = nt, e1 = e0 + nb, e2 = e1 + nu, e3 = e2 + nt, e4 = e3 + nb, total = e4 + nu;
+  // starts: per job item starts of tiles and super-tiles, nj + 1 each
+  const uint32_t *t_start = starts, *u_start = starts + (nj + 1);
+  const uint32_t nt = t_start[nj], nu = u_start[nj];
+  const uint32_t e0 = nt, e1 = e0 + kScanItems * nj, e2 = e1 + nu, e3 = e2 + nt, e4 = e3 + kScan2Items * nj,
+                 total = e4 + nu;
   const uint32_t tid = threadIdx.x;
   for (;;) {
     __syncthreads();
@@ -80,27 +121,20 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
     if (item >= total) break;
     if (item < e0) {
       const uint32_t j = find_job(t_start, nj, item);
-      const BatchJob &J = jobs[j];
-      if (J.std_form)
-        project_chunk<true>(sm.p, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
-                            &J.ctl->recomputed_acc, item - t_start[j]);
-      else
-        project_chunk<false>(sm.p, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
-                             &J.ctl->recomputed_acc, item - t_start[j]);
+      item_project(sm.p, jobs[j], item - t_start[j]);
       count_done(&proj_done[j]);
     } else if (item < e1) {
-      const uint32_t it = item - e0, j = find_job(b_start, nj, it);
+      const uint32_t j = (item - e0) / kScanItems;
       const BatchJob &J = jobs[j];
       wait_for(&proj_done[j], J.tiles, &bc->stuck);
-      if (scan_after_project(it - b_start[j], J.blocks, J.tiles, J.tilehist, J.prefix_lo, J.bsum_lo, J.bsum_hi, J.ctl,
-                             s_w, &s_last) && tid == 0)
-        __hip_atomic_store(&scan_done[j], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      item_scan(J, (item - e0) % kScanItems, sm.part);
+      count_done(&scan_done[j]);
     } else if (item < e2) {
       const uint32_t it = item - e1, j = find_job(u_start, nj, it);
       const BatchJob &J = jobs[j];
-      wait_for(&scan_done[j], 1u, &bc->stuck);
-      scatter_super<false, kSuper>(sm.s, it - u_start[j], J.keys, J.mask, nullptr, J.n, J.tilehist, 512, 0,
-                                   J.prefix_lo, J.bsum_lo, J.ctl, J.packed);
+      wait_for(&scan_done[j], kScanItems, &bc->stuck);
+      scatter_super<false, kSuper>(sm.s, it - u_start[j], J.keys, J.mask, nullptr, J.n, J.tilehist, 512, J.prefix_lo,
+                                   J.ctl->total, J.ctl, J.packed);
       count_done(&lo_done[j]);
     } else if (item < e3) {
       const uint32_t it = item - e2, j = find_job(t_start, nj, it);
@@ -109,17 +143,17 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
       hist_tile(sm.h, it - t_start[j], J.packed, J.ctl, J.tilehist3);
       count_done(&hist_done[j]);
     } else if (item < e4) {
-      const uint32_t it = item - e3, j = find_job(b_start, nj, it);
+      const uint32_t j = (item - e3) / kScan2Items;
       const BatchJob &J = jobs[j];
       wait_for(&hist_done[j], J.tiles, &bc->stuck);
-      if (scan_after_hist(it - b_start[j], J.blocks, J.tilehist3, J.prefix_hi, J.bsum3, J.ctl, &s_last) && tid == 0)
-        __hip_atomic_store(&scan2_done[j], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      item_scan2(J, (item - e3) % kScan2Items, sm.part);
+      count_done(&scan2_done[j]);
     } else {
       const uint32_t it = item - e4, j = find_job(u_start, nj, it);
       const BatchJob &J = jobs[j];
-      wait_for(&scan2_done[j], 1u, &bc->stuck);
-      scatter_super<true, kSuper>(sm.s, it - u_start[j], nullptr, nullptr, J.packed, J.n, J.tilehist3, 256, 0,
-                                  J.prefix_hi, J.bsum3, J.ctl, J.order);
+      wait_for(&scan2_done[j], kScan2Items, &bc->stuck);
+      scatter_super<true, kSuper>(sm.s, it - u_start[j], nullptr, nullptr, J.packed, J.n, J.tilehist3, 256,
+                                  J.prefix_hi, J.ctl->total + 256, J.ctl, J.order);
     }
   }
   // The last workgroup out resets the queue for the next run.
@@ -136,6 +170,47 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
   }
 }
 
+// The phased schedule: one launch a run, a workgroup an item.
+__global__ __launch_bounds__(256) void kb_project(const BatchJob *__restrict__ jobs, const uint32_t *__restrict__ starts,
+                                                  uint32_t nj) {
+  __shared__ ProjectSmem sm;
+  const uint32_t j = find_job(starts, nj, blockIdx.x);
+  item_project(sm, jobs[j], blockIdx.x - starts[j]);
+}
+
+__global__ __launch_bounds__(256) void kb_scan(const BatchJob *__restrict__ jobs) {
+  __shared__ uint32_t part[kScanCols][64];
+  item_scan(jobs[blockIdx.x / kScanItems], blockIdx.x % kScanItems, part);
+}
+
+template <bool HI>
+__global__ __launch_bounds__(256) void kb_scatter(const BatchJob *__restrict__ jobs, const uint32_t *__restrict__ starts,
+                                                  uint32_t nj) {
+  __shared__ ScatterSmem<kSuper> sm;
+  const uint32_t *u_start = starts + (nj + 1);
+  const uint32_t j = find_job(u_start, nj, blockIdx.x);
+  const BatchJob &J = jobs[j];
+  if (HI)
+    scatter_super<true, kSuper>(sm, blockIdx.x - u_start[j], nullptr, nullptr, J.packed, J.n, J.tilehist3, 256,
+                                J.prefix_hi, J.ctl->total + 256, J.ctl, J.order);
+  else
+    scatter_super<false, kSuper>(sm, blockIdx.x - u_start[j], J.keys, J.mask, nullptr, J.n, J.tilehist, 512,
+                                 J.prefix_lo, J.ctl->total, J.ctl, J.packed);
+}
+
+__global__ __launch_bounds__(256) void kb_hist(const BatchJob *__restrict__ jobs, const uint32_t *__restrict__ starts,
+                                               uint32_t nj) {
+  __shared__ HistSmem sm;
+  const uint32_t j = find_job(starts, nj, blockIdx.x);
+  const BatchJob &J = jobs[j];
+  hist_tile(sm, blockIdx.x - starts[j], J.packed, J.ctl, J.tilehist3);
+}
+
+__global__ __launch_bounds__(256) void kb_scan2(const BatchJob *__restrict__ jobs) {
+  __shared__ uint32_t part[kScanCols][64];
+  item_scan2(jobs[blockIdx.x / kScan2Items], blockIdx.x % kScan2Items, part);
+}
+
 }  // namespace
 
 int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid) {
@@ -146,11 +221,27 @@ int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid) {
   return WD_OK;
 }
 
-int launch_batch(wd_context *ctx, wd_batch *b) {
+int launch_persistent(wd_context *ctx, const BatchJob *jobs, const uint32_t *starts, uint32_t nj, uint32_t max_jobs,
+                      BatchCtl *bctl, uint32_t grid, const uint32_t *gate) {
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKBatch, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(256), 0, ctx->stream, b->jobs, b->starts, b->n, b->max_jobs,
-                     b->ctl);
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), 0, ctx->stream, jobs, starts, nj, max_jobs, bctl, gate);
+  WD_HIP(hipGetLastError(), "batch: launch");
+  return ctx->prof_end(kKBatch, ev);
+}
+
+int launch_batch(wd_context *ctx, wd_batch *b) {
+  const uint32_t nj = b->n, nt = b->host_starts[nj], nu = b->host_starts[2 * nj + 1];
+  if (b->schedule == WD_SCHEDULE_PERSISTENT)
+    return launch_persistent(ctx, b->jobs, b->starts, nj, b->max_jobs, b->ctl, b->grid, nullptr);
+  hipEvent_t ev = nullptr;
+  if (ctx->prof_begin(kKBatch, &ev)) return WD_E_HIP;
+  hipLaunchKernelGGL(kb_project, dim3(nt), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+  hipLaunchKernelGGL(kb_scan, dim3(kScanItems * nj), dim3(256), 0, ctx->stream, b->jobs);
+  hipLaunchKernelGGL(kb_scatter<false>, dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+  hipLaunchKernelGGL(kb_hist, dim3(nt), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+  hipLaunchKernelGGL(kb_scan2, dim3(kScan2Items * nj), dim3(256), 0, ctx->stream, b->jobs);
+  hipLaunchKernelGGL(kb_scatter<true>, dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   WD_HIP(hipGetLastError(), "batch: launch");
   return ctx->prof_end(kKBatch, ev);
 }

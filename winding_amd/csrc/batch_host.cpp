@@ -20,7 +20,7 @@ int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out) {
   b->max_jobs = max_jobs;
   const size_t ctl_bytes = sizeof(wd::BatchCtl) + wd::kBatchCounters * sizeof(uint32_t) * max_jobs;
   hipError_t e = hipMalloc(&b->jobs, sizeof(wd::BatchJob) * max_jobs);
-  if (e == hipSuccess) e = hipMalloc(&b->starts, sizeof(uint32_t) * 3 * (max_jobs + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->starts, sizeof(uint32_t) * 2 * (max_jobs + 1));
   if (e == hipSuccess) e = hipMalloc(&b->ctl, ctl_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(b->ctl, 0, ctl_bytes, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -36,7 +36,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
   if (!jobs || n == 0 || n > b->max_jobs) return fail(WD_E_ARG, "batch: %u jobs, this batch holds 1 to %u", n, b->max_jobs);
   std::vector<wd::BatchJob> hj(n);
-  std::vector<uint32_t> t(n + 1, 0), a(n + 1, 0), u(n + 1, 0);  // item starts: tiles, scan blocks, super-tiles
+  std::vector<uint32_t> t(n + 1, 0), u(n + 1, 0);  // item starts: tiles, super-tiles
   std::vector<const void *> shape;
   for (uint32_t i = 0; i < n; ++i) {
     const wd_job &J = jobs[i];
@@ -65,22 +65,16 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     B.tilehist3 = s->tilehist3;
     B.prefix_lo = s->prefix_lo;
     B.prefix_hi = s->prefix_hi;
-    B.bsum_lo = s->bsum_lo;
-    B.bsum_hi = s->bsum_hi;
-    B.bsum3 = s->bsum3;
     B.tiles = (B.n + wd::kTile - 1) / wd::kTile;
-    B.blocks = (B.tiles + wd::kScanRows - 1) / wd::kScanRows;
     t[i + 1] = t[i] + B.tiles;
-    a[i + 1] = a[i] + B.blocks;
     u[i + 1] = u[i] + (B.tiles + wd::kSuper - 1) / wd::kSuper;
     s->last_n = B.n;
     shape.push_back(J.cloud);
     shape.push_back(J.state);
   }
-  std::vector<uint32_t> starts(3 * (n + 1), 0);
+  std::vector<uint32_t> starts(2 * (n + 1), 0);
   std::copy(t.begin(), t.end(), starts.begin());
-  std::copy(a.begin(), a.end(), starts.begin() + (n + 1));
-  std::copy(u.begin(), u.end(), starts.begin() + 2 * (n + 1));
+  std::copy(u.begin(), u.end(), starts.begin() + (n + 1));
   wd_context *ctx = b->ctx;
   WD_HIP(hipSetDevice(ctx->device), "batch: set device");
   // The table is read by the kernel at run time: a later set() with new views
@@ -90,7 +84,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   WD_HIP(hipMemcpyAsync(b->starts, starts.data(), sizeof(uint32_t) * starts.size(), hipMemcpyHostToDevice,
                         ctx->stream), "batch: item table");
   WD_HIP(hipStreamSynchronize(ctx->stream), "batch: upload");
-  const uint32_t total = 2 * t[n] + 2 * a[n] + 2 * u[n];
+  const uint32_t total = 2 * t[n] + 2 * u[n] + (512 / wd::kScanCols + 256 / wd::kScanCols) * n;
   if (shape != b->shape || n != b->n || total != b->total) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
@@ -124,6 +118,54 @@ int wd_batch_run(wd_context *ctx, wd_batch *b) {
     WD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "batch: instantiate");
   }
   WD_HIP(hipGraphLaunch(b->exec, ctx->stream), "batch: graph launch");
+  return WD_OK;
+}
+
+int wd_batch_schedule(wd_batch *b, int schedule) {
+  if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
+  if (schedule != WD_SCHEDULE_PHASED && schedule != WD_SCHEDULE_PERSISTENT)
+    return fail(WD_E_ARG, "batch: schedule %d is neither phased (0) nor persistent (1)", schedule);
+  if (schedule != b->schedule) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->schedule = schedule;
+  }
+  return WD_OK;
+}
+
+int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, int *schedule, double *ms_phased,
+                  double *ms_persistent) {
+  if (!ctx || !b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
+  if (b->n == 0) return fail(WD_E_STATE, "batch: no jobs set");
+  if (ctx->profiling) return fail(WD_E_STATE, "batch: tuning times whole runs; switch kernel profiling off first");
+  reps = std::max(reps, 1u);
+  double ms[2] = {0, 0};
+  hipEvent_t a = nullptr, z = nullptr;
+  WD_HIP(hipEventCreate(&a), "batch: tune events");
+  hipError_t e = hipEventCreate(&z);
+  int rc = WD_OK;
+  for (int sched = 0; sched < 2 && e == hipSuccess && rc == WD_OK; ++sched) {
+    rc = wd_batch_schedule(b, sched);
+    if (!rc) rc = wd_batch_run(ctx, b);  // capture and warm
+    if (!rc) e = hipEventRecord(a, ctx->stream);
+    for (uint32_t r = 0; r < reps && !rc && e == hipSuccess; ++r) rc = wd_batch_run(ctx, b);
+    if (!rc && e == hipSuccess) e = hipEventRecord(z, ctx->stream);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(z);
+    float t = 0.f;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&t, a, z);
+    ms[sched] = (double)t / reps;
+  }
+  hipEventDestroy(a);
+  if (z) hipEventDestroy(z);
+  if (rc) return rc;
+  if (e != hipSuccess) return wd::hip_fail(e, "batch: tune");
+  const int best = ms[1] < ms[0] ? WD_SCHEDULE_PERSISTENT : WD_SCHEDULE_PHASED;
+  if ((rc = wd_batch_schedule(b, best))) return rc;
+  if (schedule) *schedule = best;
+  if (ms_phased) *ms_phased = ms[0];
+  if (ms_persistent) *ms_persistent = ms[1];
   return WD_OK;
 }
 

@@ -86,8 +86,8 @@ wd::MlpGemm gemm(int M, int N, int K, const void *A, int abf, long sam, long sak
 
 // Forward over `rows` rows already in m->X; logits into Z, probabilities into P,
 // and with targets in T: dZ = (P - T) / rows and the mean loss added to *m->loss.
-int forward(wd_context *ctx, wd_mlp *m, int rows, bool with_targets) {
-  wd::MlpGemm g1 = gemm(rows, kH, kIn, m->X, 0, kIn, 1, m->wb + oW1, 1, 1, kIn, m->H1, kH, 1);
+int forward(wd_context *ctx, wd_mlp *m, int rows, bool with_targets, const float *X = nullptr) {
+  wd::MlpGemm g1 = gemm(rows, kH, kIn, X ? X : m->X, 0, kIn, 1, m->wb + oW1, 1, 1, kIn, m->H1, kH, 1);
   g1.bias = m->w + oB1;
   g1.relu = 1;
   g1.C_bf16 = m->H1b;
@@ -126,6 +126,21 @@ int backward(wd_context *ctx, wd_mlp *m, int rows) {
 bool alive(const wd_mlp *m) { return m && m->magic == 0x4d4c5031; }
 
 }  // namespace
+
+namespace wd {
+int mlp_infer(wd_context *ctx, wd_mlp *m, const float *poses_dev, uint32_t rows, const float **probs) {
+  if (!ctx || !alive(m) || !poses_dev || rows == 0) return fail(WD_E_ARG, "mlp: bad argument");
+  if (m->ctx != ctx) return fail(WD_E_STATE, "mlp: of another device");
+  int rc = ensure_rows(m, rows);
+  if (rc) return rc;
+  hipEvent_t ev = nullptr;
+  if (ctx->prof_begin(kKMlp, &ev)) return WD_E_HIP;
+  if ((rc = forward(ctx, m, (int)rows, false, poses_dev))) return rc;
+  if (ctx->prof_end(kKMlp, ev)) return WD_E_HIP;
+  *probs = m->P;
+  return WD_OK;
+}
+}  // namespace wd
 
 extern "C" {
 

@@ -69,14 +69,6 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
   return m;
 }
 
-__device__ __forceinline__ uint64_t load_status(const uint64_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_status(uint64_t *p, uint32_t epoch, uint32_t flag, uint32_t value) {
-  const uint64_t w = ((uint64_t)((epoch << 2) | flag) << 32) | value;
-  __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // ---------------------------------------------------------------------------
 // The FP64 definition, for the rare splat the f32 bound cannot decide.
 // put() (src/scene/splats.js:350-368), footprint() (src/render/splats.js:
@@ -324,109 +316,66 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
 }
 
 // ---------------------------------------------------------------------------
-// Scans between the passes, reduce-then-scan: neither scatter pass waits on
-// another tile. Level 1, block g of kScanRows tiles: thread d walks the
-// block's rows of a [T][stride] u16 histogram at column off + d (rows are read
-// whole, coalesced), writing each tile's exclusive prefix within the block,
-// prefix[t * 256 + d], and the block's total, bsum[g * 256 + d]. Level 2, once
-// all of a cloud's blocks are in: thread d turns column d of bsum into
-// exclusive block offsets in place and returns the digit's total.
-
-__device__ __forceinline__ void scan_block(uint32_t g, uint32_t T, const uint16_t *__restrict__ hist, uint32_t stride,
-                                           uint32_t off, uint32_t *__restrict__ prefix, uint32_t *__restrict__ bsum) {
-  const uint32_t d = threadIdx.x;
-  const uint32_t t0 = g * kScanRows, t1 = min(t0 + kScanRows, T);
-  uint32_t c[kScanRows];
+// Scans between the passes. A workgroup takes 4 columns (digits) of a
+// [T][stride] u16 histogram with 64 row lanes: lane r sums rows
+// [r R, (r + 1) R), R = ceil(T / 64), the 64 partial sums of a column are
+// scanned by one wave, and the lane walks its rows again writing the
+// exclusive prefix down the column, prefix[t * pstride + c], for columns
+// below pcols; every column's sum goes to total[c]. No workgroup waits on
+// another: a scatter workgroup turns the totals into digit bases itself
+// (digit_bases), so no fence, look-back or last-block step is needed.
+__device__ __forceinline__ void scan_cols(uint32_t c0, uint32_t T, const uint16_t *__restrict__ hist, uint32_t stride,
+                                          uint32_t *__restrict__ prefix, uint32_t pstride, uint32_t pcols,
+                                          uint32_t *__restrict__ total, uint32_t (*s_part)[64]) {
+  const uint32_t tid = threadIdx.x, cl = tid & 3u, rl = tid >> 2, c = c0 + cl;
+  const uint32_t R = (T + 63u) >> 6, r0 = min(rl * R, T), r1 = min(r0 + R, T);
+  uint32_t sum = 0;
+  for (uint32_t t0 = r0; t0 < r1; t0 += 32) {
+    uint32_t v[32];
 #pragma unroll
-  for (uint32_t k = 0; k < kScanRows; ++k) c[k] = (t0 + k < t1) ? hist[(size_t)(t0 + k) * stride + off + d] : 0u;
-  uint32_t run = 0;
+    for (uint32_t k = 0; k < 32; ++k) v[k] = (t0 + k < r1) ? hist[(size_t)(t0 + k) * stride + c] : 0u;
 #pragma unroll
-  for (uint32_t k = 0; k < kScanRows; ++k) {
-    if (prefix && t0 + k < t1) prefix[(size_t)(t0 + k) * 256 + d] = run;
-    run += c[k];
+    for (uint32_t k = 0; k < 32; ++k) sum += v[k];
   }
-  bsum[(size_t)g * 256 + d] = run;
-}
-
-__device__ __forceinline__ uint32_t scan_blocks(uint32_t G, uint32_t *__restrict__ bsum) {
-  const uint32_t d = threadIdx.x;
-  uint32_t run = 0;
-  for (uint32_t g0 = 0; g0 < G; g0 += 16) {
-    uint32_t c[16];
+  s_part[cl][rl] = sum;
+  __syncthreads();
+  {
+    // wave w scans column w's 64 partial sums
+    const uint32_t w = tid >> 6, lane = tid & 63u;
+    const uint32_t x = s_part[w][lane];
+    const uint32_t inc = wave_scan_incl(x);
+    s_part[w][lane] = inc - x;
+    if (lane == 63) total[c0 + w] = inc;
+  }
+  __syncthreads();
+  if (!prefix || c >= pcols) return;
+  uint32_t run = s_part[cl][rl];
+  for (uint32_t t0 = r0; t0 < r1; t0 += 32) {
+    uint32_t v[32];
 #pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) c[k] = (g0 + k < G) ? bsum[(size_t)(g0 + k) * 256 + d] : 0u;
+    for (uint32_t k = 0; k < 32; ++k) v[k] = (t0 + k < r1) ? hist[(size_t)(t0 + k) * stride + c] : 0u;
 #pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) {
-      if (g0 + k < G) bsum[(size_t)(g0 + k) * 256 + d] = run;
-      run += c[k];
+    for (uint32_t k = 0; k < 32; ++k) {
+      if (t0 + k < r1) prefix[(size_t)(t0 + k) * pstride + c] = run;
+      run += v[k];
     }
   }
-  return run;
 }
 
-// Completion of a scan: the workgroup that finishes last of `expected` (by
-// the counter *done) sees every block's total. Returns true there.
-__device__ __forceinline__ bool last_of(uint32_t *done, uint32_t expected, uint32_t *s_last) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) *s_last = (atomicAdd(done, 1u) == expected - 1) ? 1u : 0u;
-  __syncthreads();
-  if (!*s_last) return false;
-  __threadfence();
-  return true;
-}
-
-// Scan after pass 1: low-byte tile offsets and both digits' bases (the
-// reference's scanCounts, src/render/splats.js:188-212), from the chunk
-// histograms. The last block also zeroes the counter for the next sort.
-__device__ __forceinline__ bool scan_after_project(uint32_t g, uint32_t G, uint32_t T, const uint16_t *__restrict__ tilehist,
-                                                   uint32_t *__restrict__ prefix_lo, uint32_t *__restrict__ bsum_lo,
-                                                   uint32_t *__restrict__ bsum_hi, Ctl *__restrict__ ctl, uint32_t *s_w,
-                                                   uint32_t *s_last) {
-  scan_block(g, T, tilehist, 512, 0, prefix_lo, bsum_lo);
-  scan_block(g, T, tilehist, 512, 256, nullptr, bsum_hi);
-  if (!last_of(&ctl->done, G, s_last)) return false;
-  const uint32_t d = threadIdx.x;
-  const uint32_t tot_lo = scan_blocks(G, bsum_lo);
-  uint32_t tot_hi = 0;
-  for (uint32_t k = 0; k < G; ++k) tot_hi += bsum_hi[(size_t)k * 256 + d];
-  uint32_t total0, total1;
-  const uint32_t e0 = block_scan_excl(tot_lo, s_w, &total0);
-  const uint32_t e1 = block_scan_excl(tot_hi, s_w, &total1);
-  ctl->base[d] = e0;
-  ctl->base[256 + d] = e1;
-  if (d == 0) {
-    ctl->count = total0;
-    ctl->done = 0;
-    ctl->recomputed = __hip_atomic_load(&ctl->recomputed_acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ctl->recomputed_acc = 0;
-  }
-  __threadfence();
-  __syncthreads();
-  return true;
-}
-
-// Scan after the high-byte histograms: high-byte tile offsets.
-__device__ __forceinline__ bool scan_after_hist(uint32_t g, uint32_t G, const uint16_t *__restrict__ tilehist3,
-                                                uint32_t *__restrict__ prefix_hi, uint32_t *__restrict__ bsum3,
-                                                Ctl *__restrict__ ctl, uint32_t *s_last) {
-  const uint32_t T3 = (ctl->count + kTile - 1) / kTile;
-  scan_block(g, T3, tilehist3, 256, 0, prefix_hi, bsum3);
-  if (!last_of(&ctl->done2, G, s_last)) return false;
-  scan_blocks(G, bsum3);
-  if (threadIdx.x == 0) ctl->done2 = 0;
-  __threadfence();
-  __syncthreads();
-  return true;
+// Where digit d (this thread's) starts: the exclusive scan of the 256 digit
+// totals; *sum gets V.
+__device__ __forceinline__ uint32_t digit_bases(uint32_t total_d, uint32_t *s_w, uint32_t *sum) {
+  return block_scan_excl(total_d, s_w, sum);
 }
 
 // ---------------------------------------------------------------------------
 // The high-byte histogram of tile t of the low-byte pass's output (positions
-// t * kTile ...), the upsweep of the high-byte pass. Lanes of a wave with one
-// digit are counted once, by a ballot match, so a few hot digits (a slab seen
-// edge on) do not serialise the LDS counters.
+// t * kTile ...), the upsweep of the high-byte pass. Each wave keeps its own
+// row of counts and adds to it once per distinct digit of a 64-lane round (a
+// ballot match), so there are no LDS atomics: a slab seen edge on, whose
+// survivors share a few high bytes, does not serialise on a few counters.
 struct HistSmem {
-  uint32_t h[256];
+  uint32_t h[4][256];
 };
 
 __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32_t *__restrict__ packed,
@@ -435,9 +384,6 @@ __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t limit = ctl->count, tile0 = t * (uint32_t)kTile;
   if (tile0 >= limit) return;
-  __syncthreads();
-  sm.h[tid] = 0;
-  __syncthreads();
   uint32_t raw[KPT];
   const uint32_t wbase = tile0 + w * (64u * KPT);
 #pragma unroll
@@ -445,16 +391,19 @@ __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32
     const uint32_t i = wbase + (uint32_t)r * 64u + lane;
     raw[r] = i < limit ? packed[i] : 0u;
   }
+  __syncthreads();  // the LDS may still be read by the previous work item
+  for (uint32_t k = tid; k < 1024; k += 256) (&sm.h[0][0])[k] = 0;
+  __syncthreads();
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t i = wbase + (uint32_t)r * 64u + lane;
     const uint64_t valid = __ballot(i < limit);
     const uint32_t d = raw[r] >> 24;
     const uint64_t peers = match_digit(d, valid);
-    if (i < limit && count_below(peers) == 0) atomicAdd(&sm.h[d], (uint32_t)__popcll(peers));
+    if (i < limit && count_below(peers) == 0) sm.h[w][d] += (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  tilehist3[(size_t)t * 256 + tid] = (uint16_t)sm.h[tid];
+  tilehist3[(size_t)t * 256 + tid] = (uint16_t)(sm.h[0][tid] + sm.h[1][tid] + sm.h[2][tid] + sm.h[3][tid]);
 }
 
 // ---------------------------------------------------------------------------
@@ -466,14 +415,16 @@ __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32
 // by a ballot match; the super-tile's counts per digit (known beforehand from
 // the tile histograms) place every element in LDS, and the super-tile is then
 // written out in digit runs S times longer than one tile's.
+//
+// One LDS word an element: digit << 24 | payload. HI: the payload is the
+// index. LO: the high byte << kLocalBits | the index within the super-tile.
+// 40 KB at S = 2, four workgroups a CU.
 template <int S>
 struct ScatterSmem {
-  uint32_t val[S * kTile];
-  uint8_t dig[S * kTile];
-  uint32_t wh[4][256];  // per-wave running digit counts, then wave offsets
-  uint32_t sub[S][256]; // each tile's start within the super-tile's digit run
-  uint32_t lstart[256];
-  uint32_t gstart[256];
+  uint32_t ent[S * kTile];
+  uint32_t wh[4][256];   // per-wave running digit counts, then wave offsets
+  uint32_t sub[S][256];  // each tile's start of digit d in the super-tile's LDS order
+  uint32_t shift[256];   // global start - LDS start of digit d
   uint32_t w[4];
   uint32_t nvalid;
 };
@@ -482,65 +433,79 @@ template <bool HI, int S>
 __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, const uint16_t *__restrict__ keys,
                                               const uint64_t *__restrict__ mask,
                                               const uint32_t *__restrict__ in_packed, uint32_t n,
-                                              const uint16_t *__restrict__ hist, uint32_t hstride, uint32_t hoff,
-                                              const uint32_t *__restrict__ prefix, const uint32_t *__restrict__ boff,
-                                              const Ctl *__restrict__ ctl, uint32_t *__restrict__ out) {
+                                              const uint16_t *__restrict__ hist, uint32_t hstride,
+                                              const uint32_t *__restrict__ prefix, const uint32_t *__restrict__ total,
+                                              Ctl *__restrict__ ctl, uint32_t *__restrict__ out) {
   constexpr int KPT = kKeysPerThread;
+  constexpr uint32_t kLocalBits = S * kTile <= 8192 ? 13u : 14u;
+  static_assert(S * kTile <= 16384, "index within the super-tile in 14 bits");
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t limit = HI ? ctl->count : n;
   const uint32_t T = (limit + kTile - 1) / kTile;
   const uint32_t t0 = st * (uint32_t)S;
   if (t0 >= T) return;  // past the survivors (HI)
   const uint32_t ns = min((uint32_t)S, T - t0);
-  // Thread d: the super-tile's counts of digit d, its tiles' starts, its global start.
   const uint32_t d = tid;
-  __syncthreads();  // the LDS may still be read by the previous work item
-  uint32_t tot = 0;
+  const uint32_t nwords = (n + 63u) >> 6;
+  // Every global read of the item is issued here, before any is waited on:
+  // the tiles' elements (and survivor words), the digit counts, the starts.
+  uint32_t raw[S][KPT];
+  uint64_t vm[S];  // LO: lane k < 16 holds the survivor word of round k of this wave
+  uint32_t cnt[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    sm.sub[s][d] = tot;
-    if ((uint32_t)s < ns) tot += hist[(size_t)(t0 + s) * hstride + hoff + d];
-  }
-  uint32_t nvalid;
-  const uint32_t lstart = block_scan_excl(tot, sm.w, &nvalid);
-  sm.lstart[d] = lstart;
-  sm.gstart[d] = ctl->base[(HI ? 256u : 0u) + d] + boff[(size_t)(t0 / kScanRows) * 256 + d] + prefix[(size_t)t0 * 256 + d];
-  if (tid == 0) sm.nvalid = nvalid;
-  const uint32_t nwords = (n + 63u) >> 6;
-  for (uint32_t s = 0; s < ns; ++s) {
-    const uint32_t tile0 = (t0 + s) * (uint32_t)kTile;
-    for (uint32_t k = tid; k < 1024; k += 256) (&sm.wh[0][0])[k] = 0;
-    __syncthreads();
-    const uint32_t wbase = tile0 + w * (64u * KPT);
-    uint32_t raw[KPT];
-    uint64_t vmask[KPT];
-    const uint32_t wave0 = tile0 + (uint32_t)__builtin_amdgcn_readfirstlane(w) * (64u * KPT);
+    const uint32_t wbase = (t0 + s) * (uint32_t)kTile + w * (64u * KPT);
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
       const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-      if (HI) {
-        raw[r] = i < limit ? in_packed[i] : 0u;
-      } else {
-        raw[r] = i < n ? (uint32_t)keys[i] : 0u;
-        const uint32_t wi = (wave0 + (uint32_t)r * 64u) >> 6;
-        vmask[r] = wi < nwords ? mask[wi] : 0ull;
-      }
+      raw[s][r] = HI ? (i < limit ? in_packed[i] : 0u) : (i < n ? (uint32_t)keys[i] : 0u);
     }
-    uint32_t dig[KPT], rank[KPT], val[KPT];
+    if (!HI) {
+      const uint32_t wi = (wbase >> 6) + lane;
+      vm[s] = (lane < (uint32_t)KPT && wi < nwords && (uint32_t)s < ns) ? mask[wi] : 0ull;
+    }
+    cnt[s] = (uint32_t)s < ns ? (uint32_t)hist[(size_t)(t0 + s) * hstride + d] : 0u;
+  }
+  const uint32_t dtotal = total[d], dprefix = prefix[(size_t)t0 * 256 + d];
+  __syncthreads();  // the LDS may still be read by the previous work item
+  uint32_t V;
+  const uint32_t gbase = digit_bases(dtotal, sm.w, &V) + dprefix;
+  if (!HI && st == 0 && tid == 0) ctl->count = V;  // read by the passes after this one
+  uint32_t tot = 0;
+  uint32_t subs[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    subs[s] = tot;
+    tot += cnt[s];
+  }
+  uint32_t nvalid;
+  const uint32_t lstart = block_scan_excl(tot, sm.w, &nvalid);
+#pragma unroll
+  for (int s = 0; s < S; ++s) sm.sub[s][d] = lstart + subs[s];
+  sm.shift[d] = gbase - lstart;
+  if (tid == 0) sm.nvalid = nvalid;
+  const uint32_t sbase = t0 * (uint32_t)kTile;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if ((uint32_t)s >= ns) break;
+    const uint32_t wbase = (t0 + s) * (uint32_t)kTile + w * (64u * KPT);
+    for (uint32_t k = tid; k < 1024; k += 256) (&sm.wh[0][0])[k] = 0;
+    __syncthreads();
+    uint32_t rank[KPT];
     uint32_t okbits = 0;
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
       const uint32_t i = wbase + (uint32_t)r * 64u + lane;
       uint64_t valid;
-      uint32_t dd, value;
+      uint32_t dd;
       if (HI) {
         valid = __ballot(i < limit);
-        dd = raw[r] >> 24;
-        value = raw[r] & 0xFFFFFFu;
+        dd = raw[s][r] >> 24;
       } else {
-        valid = vmask[r];
-        dd = raw[r] & 255u;
-        value = ((raw[r] >> 8) << 24) | i;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)vm[s], r);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(vm[s] >> 32), r);
+        valid = ((uint64_t)hi << 32) | lo;
+        dd = raw[s][r] & 255u;
       }
       const bool ok = (valid >> lane) & 1ull;
       const uint64_t peers = match_digit(dd, valid);
@@ -548,16 +513,14 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
       uint32_t before = 0;
       if (ok) before = sm.wh[w][dd];
       if (ok && below == 0) sm.wh[w][dd] = before + (uint32_t)__popcll(peers);
-      dig[r] = dd;
       rank[r] = before + below;
-      val[r] = value;
       okbits |= (ok ? 1u : 0u) << r;
     }
     __syncthreads();
     {
       // wave offsets of digit d within this tile, plus the tile's start
       const uint32_t c0 = sm.wh[0][d], c1 = sm.wh[1][d], c2 = sm.wh[2][d];
-      const uint32_t base = sm.lstart[d] + sm.sub[s][d];
+      const uint32_t base = sm.sub[s][d];
       sm.wh[0][d] = base;
       sm.wh[1][d] = base + c0;
       sm.wh[2][d] = base + c0 + c1;
@@ -567,20 +530,22 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
       if ((okbits >> r) & 1u) {
-        const uint32_t pos = sm.wh[w][dig[r]] + rank[r];
-        if (pos < (uint32_t)(S * kTile)) {  // always, when the counts agree; a guard, not a branch taken
-          sm.val[pos] = val[r];
-          sm.dig[pos] = (uint8_t)dig[r];
-        }
+        const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+        const uint32_t dd = HI ? raw[s][r] >> 24 : raw[s][r] & 255u;
+        const uint32_t e = HI ? raw[s][r] : (dd << 24) | ((raw[s][r] >> 8) << kLocalBits) | (i - sbase);
+        const uint32_t pos = sm.wh[w][dd] + rank[r];
+        if (pos < (uint32_t)(S * kTile)) sm.ent[pos] = e;  // always, when the counts agree; a guard
       }
     }
     __syncthreads();
   }
   const uint32_t nv = min(sm.nvalid, (uint32_t)(S * kTile));
   for (uint32_t p = tid; p < nv; p += 256) {
-    const uint32_t dd = sm.dig[p];
-    const uint32_t g = sm.gstart[dd] + (p - sm.lstart[dd]);
-    if (g < n) out[g] = sm.val[p];  // n bounds both outputs; a guard against a miscount
+    const uint32_t e = sm.ent[p];
+    const uint32_t g = sm.shift[e >> 24] + p;
+    const uint32_t value =
+        HI ? (e & 0xFFFFFFu) : (((e >> kLocalBits) & 255u) << 24) | (sbase + (e & ((1u << kLocalBits) - 1u)));
+    if (g < n) out[g] = value;  // n bounds both outputs; a guard against a miscount
   }
 }
 

@@ -1,22 +1,25 @@
 // gfx950 kernels of the splat hot path: project + key, then a two-pass
-// stable LSD scatter over the 16-bit key with decoupled look-back.
+// stable LSD scatter over the 16-bit key, with reduce-then-scan offsets.
 //
-//   k_project   one thread a splat, reading its 32-byte record once: the
-//               reference's makeKeys (src/render/splats.js:155-186) with the
-//               footprint (:102-125) fused in. Writes the 16-bit key of every
-//               splat, a survivor bit per splat (a wave ballot, one u64 per 64
-//               splats), (xx, xy, yy, rgba) for survivors, and counts both key
-//               bytes in LDS. The last workgroup to finish scans the two
-//               256-bin histograms into digit bases: the reference's
-//               scanCounts (:188-212), which also zeroes the counts for the
-//               next sort, as this does.
-//   k_scatter   the reference's place (:214-219), made stable: tiles of 4096
-//               ranked within each wave by a ballot match on the 8-bit digit,
-//               tile offsets by decoupled look-back, staged through LDS and
-//               written out in digit runs. Run once on the low byte (splats in
-//               index order in, high byte << 24 | index out) and once on the
-//               high byte (order out). LSD passes are stable, so the result
-//               is key ascending, then index ascending.
+//   k_project   a workgroup a chunk of 4096 splats, each 32-byte record read
+//               once: the reference's makeKeys (src/render/splats.js:155-186)
+//               with the footprint (:102-125) fused in. Writes the 16-bit key
+//               of every splat, a survivor bit per splat (a wave ballot, one
+//               u64 per 64 splats), (xx, xy, yy, rgba) for every splat (zeros
+//               when culled), and the chunk's two 256-bin digit histograms.
+//   k_scan      the reference's scanCounts (:188-212) for pass 2: per digit
+//               column, each tile's exclusive prefix, and the digit totals
+//               (the scatter turns those into digit bases itself).
+//   k_scatter   the reference's place (:214-219), made stable: super-tiles of
+//               2 x 4096 ranked within each wave by a ballot match on the 8-bit
+//               digit, staged through LDS in digit order and written out in
+//               digit runs. Run once on the low byte (splats in index order in,
+//               high byte << 24 | index out) and once on the high byte (order
+//               out). LSD passes are stable: key ascending, then index.
+//   k_hist      the high-byte histograms of pass 2's output tiles (pass 3's
+//               upsweep), and k_scan2 their column prefixes.
+// No kernel waits on another workgroup of the same launch: the launches are
+// the only grid-wide barriers, so no fence or look-back is ever executed.
 //
 // Numerics. Every decision equals the FP64 evaluation of the reference's
 // formulas on the f32 inputs (the definition the CPU oracle restates):
@@ -34,7 +37,7 @@
 namespace wd {
 namespace {
 
-// Pass 1: a workgroup a chunk of 4096 splats.
+// Pass 1: a workgroup a chunk of kTile splats.
 template <bool STD>
 __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, uint32_t n, ViewK v,
                                                  uint16_t *__restrict__ keys, uint64_t *__restrict__ mask,
@@ -44,12 +47,16 @@ __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, 
   project_chunk<STD>(sm, rec, n, v, keys, mask, cov, tilehist, &ctl->recomputed_acc, blockIdx.x);
 }
 
-// Scan after pass 1: a workgroup a block of kScanRows tiles.
+// Scan of pass 2: low-byte tile prefixes, and both digits' totals. Also moves
+// the FP64 recompute count of pass 1 to where the host reads it.
 __global__ __launch_bounds__(256) void k_scan(uint32_t T, const uint16_t *__restrict__ tilehist,
-                                              uint32_t *__restrict__ prefix_lo, uint32_t *__restrict__ bsum_lo,
-                                              uint32_t *__restrict__ bsum_hi, Ctl *__restrict__ ctl) {
-  __shared__ uint32_t s_w[4], s_last;
-  scan_after_project(blockIdx.x, gridDim.x, T, tilehist, prefix_lo, bsum_lo, bsum_hi, ctl, s_w, &s_last);
+                                              uint32_t *__restrict__ prefix_lo, Ctl *__restrict__ ctl) {
+  __shared__ uint32_t s_part[kScanCols][64];
+  scan_cols(blockIdx.x * kScanCols, T, tilehist, 512, prefix_lo, 256, 256, ctl->total, s_part);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->recomputed = ctl->recomputed_acc;
+    ctl->recomputed_acc = 0;
+  }
 }
 
 // Pass 2 and 3: a workgroup a super-tile.
@@ -57,10 +64,11 @@ template <bool HI>
 __global__ __launch_bounds__(256) void k_scatter(const uint16_t *__restrict__ keys, const uint64_t *__restrict__ mask,
                                                  const uint32_t *__restrict__ in_packed, uint32_t n,
                                                  const uint16_t *__restrict__ hist, uint32_t hstride,
-                                                 const uint32_t *__restrict__ prefix, const uint32_t *__restrict__ boff,
-                                                 const Ctl *__restrict__ ctl, uint32_t *__restrict__ out) {
+                                                 const uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl,
+                                                 uint32_t *__restrict__ out) {
   __shared__ ScatterSmem<kSuper> sm;
-  scatter_super<HI, kSuper>(sm, blockIdx.x, keys, mask, in_packed, n, hist, hstride, 0, prefix, boff, ctl, out);
+  scatter_super<HI, kSuper>(sm, blockIdx.x, keys, mask, in_packed, n, hist, hstride, prefix,
+                            HI ? ctl->total + 256 : ctl->total, ctl, out);
 }
 
 // Upsweep of pass 3: a workgroup a tile of pass 2's output.
@@ -70,11 +78,12 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t *__restrict__ packe
   hist_tile(sm, blockIdx.x, packed, ctl, tilehist3);
 }
 
-// Scan after the upsweep.
+// Scan of pass 3: high-byte tile prefixes over the tiles of V.
 __global__ __launch_bounds__(256) void k_scan2(const uint16_t *__restrict__ tilehist3, uint32_t *__restrict__ prefix_hi,
-                                               uint32_t *__restrict__ bsum3, Ctl *__restrict__ ctl) {
-  __shared__ uint32_t s_last;
-  scan_after_hist(blockIdx.x, gridDim.x, tilehist3, prefix_hi, bsum3, ctl, &s_last);
+                                               Ctl *__restrict__ ctl) {
+  __shared__ uint32_t s_part[kScanCols][64];
+  const uint32_t T3 = (ctl->count + kTile - 1) / kTile;
+  scan_cols(blockIdx.x * kScanCols, T3, tilehist3, 256, prefix_hi, 256, 256, ctl->total3, s_part);
 }
 
 // ---------------------------------------------------------------------------
@@ -207,8 +216,9 @@ ViewK make_viewk(const wd_view &v) {
   return k;
 }
 
-const char *const kKernelNames[kKernelCount] = {"k_bounds", "k_project",    "k_scan",  "k_scatter_lo", "k_hist",
-                                                "k_scan2",  "k_scatter_hi", "k_batch", "k_mlp",        "k_learned"};
+const char *const kKernelNames[kKernelCount] = {
+    "k_bounds", "k_project", "k_scan", "k_scatter_lo", "k_hist",        "k_scan2",
+    "k_scatter_hi", "k_batch", "k_mlp", "k_bin_table", "k_bin_scatter", "k_bin_sort"};
 
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8) {
   static const uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0, 0xFFFFFFFFu, 0};
@@ -227,7 +237,6 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
   const bool std_form = is_std(view);
   st->last_n = n;
   const uint32_t tiles = (n + kTile - 1) / kTile;
-  const uint32_t blocks = (tiles + kScanRows - 1) / kScanRows;
   const uint32_t supers = (tiles + kSuper - 1) / kSuper;
   const uint4 *rec = (const uint4 *)cloud->records;
   hipEvent_t ev = nullptr;
@@ -244,18 +253,17 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
   else
     WD_LAUNCH(kKProject, "sort: project launch", k_project<false>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk,
               st->keys, st->mask, (float4 *)st->cov, st->tilehist, st->ctl);
-  WD_LAUNCH(kKScan, "sort: scan launch", k_scan, dim3(blocks), dim3(256), 0, ctx->stream, tiles, st->tilehist,
-            st->prefix_lo, st->bsum_lo, st->bsum_hi, st->ctl);
+  WD_LAUNCH(kKScan, "sort: scan launch", k_scan, dim3(512 / kScanCols), dim3(256), 0, ctx->stream, tiles,
+            st->tilehist, st->prefix_lo, st->ctl);
   WD_LAUNCH(kKScatterLo, "sort: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0, ctx->stream,
-            st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->bsum_lo, st->ctl,
-            st->packed);
+            st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->ctl, st->packed);
   WD_LAUNCH(kKHist, "sort: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, st->packed, st->ctl,
             st->tilehist3);
-  WD_LAUNCH(kKScan2, "sort: second scan launch", k_scan2, dim3(blocks), dim3(256), 0, ctx->stream, st->tilehist3,
-            st->prefix_hi, st->bsum3, st->ctl);
+  WD_LAUNCH(kKScan2, "sort: second scan launch", k_scan2, dim3(256 / kScanCols), dim3(256), 0, ctx->stream,
+            st->tilehist3, st->prefix_hi, st->ctl);
   WD_LAUNCH(kKScatterHi, "sort: high-byte pass launch", k_scatter<true>, dim3(supers), dim3(256), 0, ctx->stream,
             (const uint16_t *)nullptr, (const uint64_t *)nullptr, st->packed, n, st->tilehist3, 256u, st->prefix_hi,
-            st->bsum3, st->ctl, st->order);
+            st->ctl, st->order);
 #undef WD_LAUNCH
   return WD_OK;
 }

@@ -26,30 +26,23 @@ int hip_fail(hipError_t e, const char *what);
 constexpr int kThreads = 256;        // workgroup size of every kernel here
 constexpr int kTile = 4096;          // splats per scatter tile: 256 threads x 16
 constexpr int kKeysPerThread = 16;
-constexpr uint32_t kScanRows = 32;   // tiles a first-level scan block takes
+constexpr uint32_t kScanCols = 4;    // histogram columns a scan workgroup takes
 
-// Flags of a look-back status word: value in the low 32 bits, epoch << 2 | flag above.
-constexpr uint32_t kFlagAggregate = 1;
-constexpr uint32_t kFlagInclusive = 2;
 // Polls of a bounded wait before it gives up and flags the sort as failed.
 constexpr uint32_t kSpinLimit = 1u << 20;
 
-// Device control block of one sort state. The K1 histogram, its finish counter
-// and the tickets reset themselves (the last K1 workgroup zeroes them), so a
-// sort never needs a memset, in a graph or out of one.
+// Device control block of one sort state. Nothing in it needs clearing
+// between sorts: every field is written before it is read in each sort.
 struct Ctl {
-  uint32_t hist[512];     // survivors per low byte (0..255) and high byte (256..511)
-  uint32_t base[512];     // exclusive prefix of hist: where each digit starts
+  uint32_t total[512];    // survivors per digit: low byte (0..255), high byte (256..511)
+  uint32_t total3[256];   // high-byte totals again, from pass 3's own histograms
   uint32_t count;         // V, survivors of the last sort
-  uint32_t done;          // projection workgroups finished
-  uint32_t done2;         // high-byte scan blocks finished
-  uint32_t spare;
   uint32_t recomputed;    // FP64 area decisions in the last sort
   uint32_t recomputed_acc;
   uint32_t fallback;      // learned sort: 1 when the exact path ran
   uint32_t overflow_bins;
   uint32_t stuck;         // a bounded wait ran out (a bug, reported, never a hang)
-  uint32_t pad[55];
+  uint32_t pad[58];
 };
 static_assert(sizeof(Ctl) % 256 == 0, "Ctl keeps 256-byte alignment");
 
@@ -86,13 +79,25 @@ struct BatchJob {
   uint32_t *order;
   Ctl *ctl;
   uint16_t *tilehist, *tilehist3;
-  uint32_t *prefix_lo, *prefix_hi, *bsum_lo, *bsum_hi, *bsum3;
+  uint32_t *prefix_lo, *prefix_hi;
+  // the learned sort's (unused by the exact path)
+  uint16_t *lut;          // [65536] key -> bin
+  uint16_t *binlo;        // [kBins] lowest key of each bin
+  uint32_t *prefix_bin;   // [tiles][512] where a tile's run of each bin starts
+  const float *probs;     // [256] predicted bin widths (the predictor's output row)
+  uint32_t cap;           // most survivors a bin may hold; more and the view runs the exact path
   uint32_t n, std_form;
   uint32_t tiles;    // projection chunks = scatter tiles, of kTile splats
-  uint32_t blocks;   // scan blocks of kScanRows tiles
 };
 // Tiles a scatter work item takes (a super-tile).
 constexpr int kSuper = 2;
+// The learned sort: bins a view's key line is cut into (256 predicted, the
+// wide ones split), keys a bin may span (its offsets fit 9 bits), views one
+// projection launch takes, and the largest cloud (indices packed in 23 bits).
+constexpr uint32_t kBins = 512;
+constexpr uint32_t kBinWidth = 512;
+constexpr uint32_t kMaxViews = 8;
+constexpr uint32_t kLearnedMaxSplats = 1u << 23;
 // One GEMM of the predictor on the matrix cores: C = alpha * A B, element (m, k)
 // of A at A[m * sa_m + k * sa_k] (f32 or bf16), (k, n) of B likewise, then
 // + bias[n], relu, zero where mask <= 0, += C, and an optional bf16 copy.
@@ -113,18 +118,16 @@ struct MlpGemm {
   uint16_t *C_bf16;
 };
 
-// Digits a scan work item of a batch takes (16 items a job).
-constexpr uint32_t kBatchScanDigits = 16;
 // Device control block of a batch: tickets, exits, the run counter, and per
-// job the finished projection chunks, the scanned flag and the finished
-// low-byte tiles.
+// job five counters of the persistent schedule.
 struct BatchCtl {
   uint32_t ticket;
   uint32_t exited;
   uint32_t runs;
   uint32_t stuck;
-  // followed by five counters a job: projection chunks done, first scan done,
-  // low-byte super-tiles done, high-byte histograms done, second scan done
+  // followed by five counters a job: projection chunks done, scan column
+  // groups done, low-byte super-tiles done, histogram tiles done, second-scan
+  // column groups done
 };
 constexpr int kBatchCounters = 5;
 
@@ -161,7 +164,7 @@ struct wd_batch {
   wd_context *ctx = nullptr;
   uint32_t max_jobs = 0, n = 0;
   wd::BatchJob *jobs = nullptr;   // device
-  uint32_t *starts = nullptr;     // device: 3 x (max_jobs + 1) item starts, then totals
+  uint32_t *starts = nullptr;     // device: 2 x (max_jobs + 1) item starts (tiles, super-tiles)
   wd::BatchCtl *ctl = nullptr;    // device
   std::vector<wd::BatchJob> host_jobs;
   std::vector<uint32_t> host_starts;
@@ -169,6 +172,7 @@ struct wd_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t grid = 0, total = 0;
+  int schedule = WD_SCHEDULE_PHASED;
   uint32_t magic = 0x42415443;
 };
 
@@ -176,7 +180,6 @@ struct wd_state {
   wd_context *ctx = nullptr;
   uint64_t capacity = 0;
   uint64_t last_n = 0;
-  uint32_t epoch = 0;          // look-back epoch of the last sort
   wd::Ctl *ctl = nullptr;      // device
   uint16_t *keys = nullptr;    // n keys, every splat
   uint64_t *mask = nullptr;    // survivor bits, ceil(n/64) words
@@ -185,16 +188,18 @@ struct wd_state {
   float *cov = nullptr;        // n x 4 floats
   uint16_t *tilehist = nullptr;   // per 4096-splat chunk: 256 low-byte then 256 high-byte counts
   uint16_t *tilehist3 = nullptr;  // per tile of the low-byte pass's output: 256 high-byte counts
-  uint32_t *prefix_lo = nullptr, *prefix_hi = nullptr;  // [tiles][256] offsets within a scan block
-  uint32_t *bsum_lo = nullptr, *bsum_hi = nullptr, *bsum3 = nullptr;  // [blocks][256]
-  uint64_t tiles = 0, blocks = 0;
+  uint32_t *prefix_lo = nullptr, *prefix_hi = nullptr;  // [tiles][256]: where a tile's digit run starts
+  uint16_t *lut = nullptr, *binlo = nullptr;          // learned sort: key -> bin, bin -> lowest key
+  uint32_t *prefix_bin = nullptr;                      // learned sort: [tiles][512]
+  uint64_t tiles = 0;
   uint32_t magic = 0x53544154;
 };
 
 namespace wd {
 
 enum KernelId {
-  kKBounds = 0, kKProject, kKScan, kKScatterLo, kKHist, kKScan2, kKScatterHi, kKBatch, kKMlp, kKLearned, kKernelCount
+  kKBounds = 0, kKProject, kKScan, kKScatterLo, kKHist, kKScan2, kKScatterHi, kKBatch, kKMlp, kKBinTable, kKBinScatter,
+  kKBinSort, kKernelCount
 };
 extern const char *const kKernelNames[kKernelCount];
 
@@ -202,6 +207,16 @@ extern const char *const kKernelNames[kKernelCount];
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8);
 int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_state *st);
 int launch_batch(wd_context *ctx, wd_batch *b);
+// The persistent batch kernel over a job table; skipped unless *gate != 0 when gate is given.
+int launch_persistent(wd_context *ctx, const BatchJob *jobs, const uint32_t *starts, uint32_t nj, uint32_t max_jobs,
+                      BatchCtl *bctl, uint32_t grid, const uint32_t *gate);
+// The 257 equal-mass edges of a sorted state's keys (learned.hip).
+int launch_quantiles(wd_context *ctx, const wd_state *s, uint32_t *edges_dev);
+// The learned sort's kernels (learned.hip) over nv views of one n-splat cloud.
+int launch_learned(wd_context *ctx, const BatchJob *jobs, uint32_t nv, uint32_t n, bool std_form, uint32_t *gate);
+// The predictor's forward pass over `rows` poses already in device memory;
+// *probs gets the device address of the rows x 256 softmax (mlp_host.cpp).
+int mlp_infer(wd_context *ctx, wd_mlp *m, const float *poses_dev, uint32_t rows, const float **probs);
 int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid);
 int launch_copy(wd_context *ctx, const void *src, void *dst, size_t bytes);
 // Predictor kernels (mlp.hip).
