@@ -22,6 +22,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter runs (profiles/*_traffic.md:
+    FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a copy of known size), or None."""
+    import glob
+    for path in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md")):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 5 and f[0] == "traffic:" and f[1] == workload and f[2] == kernel:
+                return float(f[3]) + float(f[4])
+    return None
+
+
 def cpu_baseline(recs, view, budget_s=15.0):
     """The JavaScript CPU path (oracle/splat_oracle.js, a one-thread restatement of the reference's
     put + makeKeys + stable counting sort) in Node on this host, on a prefix of the cloud; the C
@@ -128,12 +140,109 @@ def bench_batch(args):
     return 0
 
 
+def bench_learned(args):
+    """4,194,304 splats, a 1024-pose orbit. The predictor is trained first (untimed) on the exact sort's
+    quantiles at 4096 random poses of the same family (a disjoint seed). Timed: the whole orbit sorted
+    (a) plain, one wd_sort a pose; (b) batched, 8 poses a phased batch launch; (c) batched + learned, 8 poses
+    a wd_learned_run (one read of the records for the 8, bins from the predictor, exact fallback gated)."""
+    import numpy as np
+    import winding_amd as wa
+    wl = wa.workloads
+    spec = wl.LEARNED
+    n, W, H, K = spec["n"], spec["width"], spec["height"], 8
+    ctx = wa.Context(0)
+    recs = wa.synth_records(spec["kind"], spec["seed"], n)
+    cloud = wa.Cloud(ctx, recs)
+    del recs
+    states = [wa.State(ctx, n) for _ in range(K)]
+    t0 = time.perf_counter()
+    X, Y = wl.training_set(ctx, cloud, states[0], wl.training_poses(spec["train_poses"], spec["train_seed"]), W, H,
+                           args.cull)
+    t_data = time.perf_counter() - t0
+    mlp = wa.Mlp(ctx, seed=1)
+    before, _ = mlp.grad(X, Y)
+    t0 = time.perf_counter()
+    mlp.train(X, Y, epochs=40, batch=256, lr=2e-3, seed=2)
+    t_train = time.perf_counter() - t0
+    after, _ = mlp.grad(X, Y)
+    entropy = float(-(Y * np.log(np.maximum(Y, 1e-30))).sum(axis=1).mean())
+    views = [wl.view_at(cloud, e, W, H, args.cull) for e in wl.orbit_sequence(1024)]
+    groups = [(views[i:i + K], states[:len(views[i:i + K])]) for i in range(0, len(views), K)]
+    learned = wa.Learned(ctx, K)
+    batch = wa.Batch(ctx, K)
+    batch.schedule(wa.WD_SCHEDULE_PHASED)
+
+    def orbit_plain():
+        for v in views:
+            states[0].sort(cloud, v)
+
+    def orbit_batched():
+        for vs, sts in groups:
+            batch.set([(cloud, v, s) for v, s in zip(vs, sts)])
+            batch.run()
+
+    def orbit_learned():
+        for vs, sts in groups:
+            learned.run(mlp, cloud, vs, sts)
+
+    def timed(fn, reps):
+        fn()
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t) / reps
+
+    reps = max(1, args.steps // 50)
+    t_plain = timed(orbit_plain, reps)
+    t_batched = timed(orbit_batched, reps)
+    t_learned = timed(orbit_learned, reps)
+    # fallbacks over one pass of the orbit, and the survivors of every pose
+    fell, v_total = 0, 0
+    for vs, sts in groups:
+        learned.run(mlp, cloud, vs, sts)
+        for s in sts:
+            st = s.stats()
+            fell += st.fallback
+            v_total += st.count
+    poses = len(views)
+    path_bytes = 32.0 * n * poses + 20.0 * v_total
+    line = {
+        "metric": "M splats projected+sorted/s and % HBM peak; batched+learned vs plain",
+        "value": n * poses / t_learned / 1e6,
+        "unit": "M splats/s",
+        "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": t_learned * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded generator, winding_amd/workloads.py)",
+        "config": {"workload": "learned-4m", "splats": n, "poses": poses, "survivors_total": v_total,
+                   "width": W, "height": H, "cull": args.cull, "views_per_run": K, "parallelism": "single GPU"},
+        "learned": {"plain_value": n * poses / t_plain / 1e6, "batched_value": n * poses / t_batched / 1e6,
+                    "speedup_vs_plain": t_plain / t_learned, "speedup_batched_exact_vs_plain": t_plain / t_batched,
+                    "fallback_views": fell, "train_poses": len(X), "train_loss": [before, after],
+                    "target_entropy": entropy, "train_data_s": t_data, "train_s": t_train,
+                    "ms_per_orbit": {"plain": t_plain * 1e3, "batched": t_batched * 1e3, "learned": t_learned * 1e3}},
+        "path": {"bytes_per_step": path_bytes, "achieved_gbs": path_bytes / t_learned / 1e9,
+                 "frac_of_peak": path_bytes / t_learned / 1e9 / HBM_PEAK_GBS},
+    }
+    learned.destroy()
+    batch.destroy()
+    mlp.destroy()
+    for s in states:
+        s.destroy()
+    cloud.destroy()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="slab-8m", choices=["plain-64k", "plain-1m", "slab-8m", "batch-64"])
+    ap.add_argument("--workload", default="slab-8m",
+                    choices=["plain-64k", "plain-1m", "slab-8m", "batch-64", "learned-4m"])
     ap.add_argument("--cull", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -156,6 +265,8 @@ def main():
     import winding_amd as wa
     if args.workload == "batch-64":
         return bench_batch(args)
+    if args.workload == "learned-4m":
+        return bench_learned(args)
     recs, view, spec = wa.workloads.plain(args.workload, cull=args.cull)
     n = len(recs)
     ctx = wa.Context(0)
@@ -205,7 +316,8 @@ def main():
         "config": {"workload": args.workload, "splats": n, "survivors": v, "width": spec["width"],
                    "height": spec["height"], "cull": args.cull, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_project",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, "k_project"),
+                     "kernel": "k_project",
                      "kernel_ms": proj_ms, "bytes_per_launch": alg_proj},
         "path": {"bytes_per_step": path_bytes, "achieved_gbs": path_gbs, "frac_of_peak": path_gbs / HBM_PEAK_GBS,
                  "recomputed_fp64": stats.recomputed, "profiled_ms_per_step": prof_elapsed * 1e3 / args.steps},

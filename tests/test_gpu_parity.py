@@ -3,7 +3,9 @@
 The bar: keys, survivor flags and the draw order bit-exact; cov2d within 1e-5 of the FP64 oracle,
 relative to max(|xx|, |yy|) per splat (the xy term can be near zero); the FP64 fallback bit-exact.
 """
+import glob
 import math
+import os
 import struct
 
 import numpy as np
@@ -56,6 +58,24 @@ def assert_parity(ctx, recs, view, label=""):
     colours = cov[keep, 3].view(np.uint32)
     assert np.array_equal(colours, recs[keep, 24:28].copy().view(np.uint32).ravel())
     return stats, err
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: os.path.basename(p)[:-4])
+def test_golden_fixtures(ctx, path):
+    """The committed fixtures (both oracles agreed on them): keys, flags, order bit-exact, cov2d to 1e-5."""
+    with np.load(path, allow_pickle=False) as z:
+        recs, fk, forder, fcov = z["records"], z["keys"], z["order"], z["cov"]
+        view = wa.View.from_buffer_copy(z["view"].astype(np.float32).tobytes())
+    order, keys, keep, cov, stats = run_gpu(ctx, recs, view)
+    assert np.array_equal(keep, fk >= 0)
+    assert np.array_equal(keys[keep].astype(np.int32), fk[keep])
+    assert stats.count == len(forder) and np.array_equal(order, forder)
+    err = cov_error(cov, np.nan_to_num(fcov), keep)
+    print("%s: V=%d cov2d max rel err %.3g" % (os.path.basename(path), stats.count, err), flush=True)
+    assert err <= COV_TOL
 
 
 def test_selftest_dpp_scan_and_f64_rounding(ctx):

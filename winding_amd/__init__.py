@@ -30,6 +30,7 @@ EXPORTS = [
     "wd_batch_destroy",
     "wd_mlp_create", "wd_mlp_destroy", "wd_mlp_set", "wd_mlp_get", "wd_mlp_save", "wd_mlp_load",
     "wd_mlp_grad", "wd_mlp_train", "wd_mlp_predict", "wd_pose_features",
+    "wd_learned_create", "wd_learned_run", "wd_learned_destroy", "wd_state_quantiles",
     "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest", "wd_debug_copy",
 ]
 
@@ -101,6 +102,8 @@ def lib():
             "wd_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(U64), I,
                                     C.POINTER(I)]),
             "wd_batch_create": (I, [P, U32, C.POINTER(P)]),
+            "wd_batch_schedule": (I, [P, I]),
+            "wd_batch_tune": (I, [P, P, U32, C.POINTER(I), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "wd_batch_set": (I, [P, C.POINTER(Job), U32]),
             "wd_batch_run": (I, [P, P]),
             "wd_batch_stuck": (I, [P, P, C.POINTER(U32)]),
@@ -115,11 +118,18 @@ def lib():
             "wd_mlp_train": (I, [P, P, P, P, U32, U32, U32, C.c_float, U64, C.POINTER(C.c_float)]),
             "wd_mlp_predict": (I, [P, P, P, U32, P]),
             "wd_pose_features": (I, [C.POINTER(View), P]),
+            "wd_learned_create": (I, [P, U32, C.POINTER(P)]),
+            "wd_learned_run": (I, [P, P, P, P, C.POINTER(View), C.POINTER(P), U32, U32]),
+            "wd_learned_destroy": (I, [P]),
+            "wd_state_quantiles": (I, [P, P, P]),
             "wd_synth_records": (I, [I, U64, U64, P]),
             "wd_debug_canonical": (I, [P, P, C.POINTER(View), P]),
             "wd_debug_selftest": (I, [P, P, P, U32, P]),
             "wd_debug_copy": (I, [P, U64, I, C.POINTER(C.c_double)]),
         }
+        missing = sorted(set(EXPORTS) - set(sig))
+        if missing:
+            raise ImportError("winding_amd: no ctypes signature for %s" % ", ".join(missing))
         for name, (res, args) in sig.items():
             f = getattr(L, name)
             f.restype = res
@@ -293,6 +303,12 @@ class State:
         check(lib().wd_state_stats(self.ctx.ptr, self.ptr, C.byref(s)))
         return s
 
+    def quantiles(self):
+        """The 257 equal-mass key edges of the last sort (the predictor's training target)."""
+        edges = np.zeros(257, np.uint32)
+        check(lib().wd_state_quantiles(self.ctx.ptr, self.ptr, edges.ctypes.data))
+        return edges
+
     def read(self, n_cov=0):
         """(order[:V], cov2d n_cov x 4 float32, stats)."""
         s = self.stats()
@@ -316,7 +332,6 @@ class State:
         return keys, keep
 
 
-from . import workloads  # noqa: E402  (needs the names above)
 
 
 WD_SCHEDULE_PHASED, WD_SCHEDULE_PERSISTENT = 0, 1
@@ -430,3 +445,35 @@ class Mlp:
         out = np.zeros((len(poses), 256), np.float32)
         check(lib().wd_mlp_predict(self.ctx.ptr, self.ptr, poses.ctypes.data, len(poses), out.ctypes.data))
         return out
+
+
+class Learned:
+    """The learned sort (wd_learned_*): up to 8 views of one cloud from one read of the records, binned by
+    the predictor's depth CDF, exact by construction (a failed check re-sorts on the exact path)."""
+
+    def __init__(self, ctx, max_views=8):
+        p = C.c_void_p()
+        check(lib().wd_learned_create(ctx.ptr, max_views, C.byref(p)))
+        self.ctx = ctx
+        self.ptr = p
+
+    def run(self, mlp, cloud, views, states, cap=0):
+        k = len(views)
+        varr = (View * k)(*views)
+        sarr = (C.c_void_p * k)(*[s.ptr.value for s in states])
+        assert len(states) == k
+        check(lib().wd_learned_run(self.ctx.ptr, self.ptr, mlp.ptr, cloud.ptr, varr, sarr, k, cap))
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().wd_learned_destroy(self.ptr))
+            self.ptr = None
+
+
+def widths_from_edges(edges):
+    """The predictor's target: bin widths in key space, (edges[k+1] - edges[k]) / 65536."""
+    e = np.asarray(edges, np.float64)
+    return (np.diff(e) / 65536.0).astype(np.float32)
+
+
+from . import workloads  # noqa: E402  (needs the names above)

@@ -191,6 +191,7 @@ static int cloud_from(wd_context *ctx, const void *src, uint64_t bytes, bool dev
   }
   if (min_out) std::memcpy(min_out, c->min, sizeof c->min);
   if (max_out) std::memcpy(max_out, c->max, sizeof c->max);
+  ctx->live.insert(c);
   *out = c;
   return WD_OK;
 }
@@ -225,6 +226,7 @@ int wd_cloud_destroy(wd_cloud *cloud) {
   hipSetDevice(cloud->ctx->device);
   hipStreamSynchronize(cloud->ctx->stream);
   hipFree(cloud->records);
+  cloud->ctx->live.erase(cloud);
   cloud->magic = 0;
   delete cloud;
   return WD_OK;
@@ -296,6 +298,7 @@ int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
     wd_state_destroy(s);
     return wd::hip_fail(e, "state: allocation");
   }
+  ctx->live.insert(s);
   *out = s;
   return WD_OK;
 }
@@ -306,8 +309,9 @@ int wd_state_destroy(wd_state *s) {
   hipStreamSynchronize(s->ctx->stream);
   for (void *p : {(void *)s->ctl, (void *)s->keys, (void *)s->mask, (void *)s->packed, (void *)s->order,
                   (void *)s->cov, (void *)s->tilehist, (void *)s->tilehist3, (void *)s->prefix_lo,
-                  (void *)s->prefix_hi})
+                  (void *)s->prefix_hi, (void *)s->lut, (void *)s->binlo, (void *)s->prefix_bin})
     if (p) hipFree(p);
+  s->ctx->live.erase(s);
   s->magic = 0;
   delete s;
   return WD_OK;
@@ -323,8 +327,9 @@ int wd_state_buffers(wd_state *s, void **order, void **count, void **cov2d) {
 
 static int check_sort_args(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_state *s) {
   if (!ctx || !view) return fail(WD_E_ARG, "sort: null argument");
-  if (!cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "addSplats: these splats were unloaded; remove the node first");
-  if (!s || s->magic != 0x53544154) return fail(WD_E_STATE, "sort: the sort state was destroyed");
+  if (!cloud || !ctx->live.count(cloud))
+    return fail(WD_E_STATE, "addSplats: these splats were unloaded; remove the node first");
+  if (!s || !ctx->live.count(s)) return fail(WD_E_STATE, "sort: the sort state was destroyed");
   if (cloud->ctx != ctx || s->ctx != ctx) return fail(WD_E_STATE, "sort: splats, state and context are of different devices");
   if (cloud->n > s->capacity)
     return fail(WD_E_CAPACITY, "sort: %llu splats are more than this state holds, %llu", (unsigned long long)cloud->n,

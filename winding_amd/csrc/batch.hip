@@ -188,13 +188,13 @@ __global__ __launch_bounds__(256) void kb_scatter(const BatchJob *__restrict__ j
                                                   uint32_t nj) {
   __shared__ ScatterSmem<kSuper> sm;
   const uint32_t *u_start = starts + (nj + 1);
-  const uint32_t j = find_job(u_start, nj, blockIdx.x);
+  const uint32_t item = xcd_item(blockIdx.x, gridDim.x), j = find_job(u_start, nj, item);
   const BatchJob &J = jobs[j];
   if (HI)
-    scatter_super<true, kSuper>(sm, blockIdx.x - u_start[j], nullptr, nullptr, J.packed, J.n, J.tilehist3, 256,
+    scatter_super<true, kSuper>(sm, item - u_start[j], nullptr, nullptr, J.packed, J.n, J.tilehist3, 256,
                                 J.prefix_hi, J.ctl->total + 256, J.ctl, J.order);
   else
-    scatter_super<false, kSuper>(sm, blockIdx.x - u_start[j], J.keys, J.mask, nullptr, J.n, J.tilehist, 512,
+    scatter_super<false, kSuper>(sm, item - u_start[j], J.keys, J.mask, nullptr, J.n, J.tilehist, 512,
                                  J.prefix_lo, J.ctl->total, J.ctl, J.packed);
 }
 

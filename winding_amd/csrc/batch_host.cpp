@@ -40,8 +40,8 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   std::vector<const void *> shape;
   for (uint32_t i = 0; i < n; ++i) {
     const wd_job &J = jobs[i];
-    if (!J.cloud || J.cloud->magic != 0x53504c54) return fail(WD_E_STATE, "batch: job %u: its splats were unloaded", i);
-    if (!J.state || J.state->magic != 0x53544154) return fail(WD_E_STATE, "batch: job %u: its sort state was destroyed", i);
+    if (!J.cloud || !b->ctx->live.count(J.cloud)) return fail(WD_E_STATE, "batch: job %u: its splats were unloaded", i);
+    if (!J.state || !b->ctx->live.count(J.state)) return fail(WD_E_STATE, "batch: job %u: its sort state was destroyed", i);
     if (J.cloud->ctx != b->ctx || J.state->ctx != b->ctx) return fail(WD_E_STATE, "batch: job %u is of another device", i);
     if (J.cloud->n > J.state->capacity)
       return fail(WD_E_CAPACITY, "batch: job %u: %llu splats are more than its state holds, %llu", i,
@@ -102,6 +102,8 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
 int wd_batch_run(wd_context *ctx, wd_batch *b) {
   if (!ctx || !b || b->magic != 0x42415443 || b->ctx != ctx) return fail(WD_E_STATE, "batch: destroyed or of another device");
   if (b->n == 0) return fail(WD_E_STATE, "batch: no jobs set");
+  for (const void *p : b->shape)
+    if (!ctx->live.count(p)) return fail(WD_E_STATE, "batch: a job's splats or sort state was destroyed; set the jobs again");
   WD_HIP(hipSetDevice(ctx->device), "batch: set device");
   if (ctx->profiling) return wd::launch_batch(ctx, b);  // timed launches go straight to the stream
   if (!b->exec) {

@@ -28,17 +28,8 @@
 namespace wd {
 namespace {
 
-// Lanes of this wave whose 9-bit digit equals this lane's, among `valid`.
-__device__ __forceinline__ uint64_t match_bin(uint32_t d, uint64_t valid) {
-  uint64_t m = valid;
-#pragma unroll
-  for (int b = 0; b < 9; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bal = __ballot(bit);
-    m &= bit ? bal : ~bal;
-  }
-  return m;
-}
+// Lanes of this wave whose 9-bit bin equals this lane's, among `valid`.
+__device__ __forceinline__ uint64_t match_bin(uint32_t d, uint64_t valid) { return match_bits<9>(d, valid); }
 
 // Exclusive scan of one value per thread over a workgroup of NW waves.
 template <int NW>
@@ -182,7 +173,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict_
   __shared__ BinScatterSmem sm;
   constexpr int KPT = kKeysPerThread;
   const BatchJob &J = jobs[blockIdx.y];
-  const uint32_t n = J.n, t = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t n = J.n, t = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t d0 = tid, d1 = tid + 256u, tile0 = t * (uint32_t)kTile;
   const uint32_t nwords = (n + 63u) >> 6, wbase = tile0 + w * (64u * KPT);
   uint32_t raw[KPT];

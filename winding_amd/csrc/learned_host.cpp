@@ -91,7 +91,7 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
   if (l->ctx != ctx) return fail(WD_E_STATE, "learned: of another device");
   if (n_views == 0 || n_views > l->max_views)
     return fail(WD_E_ARG, "learned: %u views; this one takes 1 to %u", n_views, l->max_views);
-  if (!cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "learned: the splats were unloaded");
+  if (!cloud || !ctx->live.count(cloud)) return fail(WD_E_STATE, "learned: the splats were unloaded");
   if (cloud->ctx != ctx) return fail(WD_E_STATE, "learned: the splats are of another device");
   const uint32_t n = (uint32_t)cloud->n;
   if (cloud->n > wd::kLearnedMaxSplats)
@@ -99,7 +99,7 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
                 (unsigned long long)cloud->n, wd::kLearnedMaxSplats);
   for (uint32_t v = 0; v < n_views; ++v) {
     wd_state *s = states[v];
-    if (!s || s->magic != 0x53544154) return fail(WD_E_STATE, "learned: view %u: its sort state was destroyed", v);
+    if (!s || !ctx->live.count(s)) return fail(WD_E_STATE, "learned: view %u: its sort state was destroyed", v);
     if (s->ctx != ctx) return fail(WD_E_STATE, "learned: view %u: its sort state is of another device", v);
     if (cloud->n > s->capacity)
       return fail(WD_E_CAPACITY, "learned: view %u: %u splats are more than its state holds, %llu", v, n,

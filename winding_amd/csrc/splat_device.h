@@ -57,17 +57,38 @@ __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t *s_w, u
   return off + inc - v;
 }
 
-// Lanes of this wave whose 8-bit digit equals this lane's, among `valid`.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
-  uint64_t m = valid;
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bal = __ballot(bit);
-    m &= bit ? bal : ~bal;
-  }
-  return m;
+// XCD-aware item order. Workgroups go to the 8 XCDs in turn (workgroup b to
+// XCD b mod 8), and each XCD has its own L2. A scatter item's output runs
+// end where the next item's runs of the same digits begin, so items that
+// are neighbours in the output should share an L2: there the two partial
+// lines at each seam merge into one whole line before they are written
+// back. This maps workgroup b to item k-th-of-its-XCD + the XCD's first item,
+// each XCD taking a contiguous range (a bijection on [0, G)).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t G) {
+  const uint32_t x = b % kXcds, k = b / kXcds, q = G / kXcds, r = G % kXcds;
+  return x * q + min(x, r) + k;
 }
+
+// Lanes of this wave whose BITS-bit digit equals this lane's, among `valid`:
+// for each bit, the ballot of the wave XOR this lane's bit spread over 32
+// bits (0 or ~0) marks the lanes that differ, and those are cleared. Kept in
+// two 32-bit halves so each bit costs a handful of VALU operations (the
+// compiler merges the clears into 3-input ORs), not a 64-bit select.
+template <int BITS>
+__device__ __forceinline__ uint64_t match_bits(uint32_t d, uint64_t valid) {
+  uint32_t dlo = 0, dhi = 0;  // lanes differing in some bit
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) {
+    const uint32_t s = (uint32_t)(((int)(d << (31 - b))) >> 31);
+    const uint64_t bal = __ballot(s != 0u);
+    dlo |= (uint32_t)bal ^ s;
+    dhi |= (uint32_t)(bal >> 32) ^ s;
+  }
+  return ((uint64_t)(~dhi & (uint32_t)(valid >> 32)) << 32) | (~dlo & (uint32_t)valid);
+}
+
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) { return match_bits<8>(d, valid); }
 
 // ---------------------------------------------------------------------------
 // The FP64 definition, for the rare splat the f32 bound cannot decide.
@@ -370,10 +391,10 @@ __device__ __forceinline__ uint32_t digit_bases(uint32_t total_d, uint32_t *s_w,
 
 // ---------------------------------------------------------------------------
 // The high-byte histogram of tile t of the low-byte pass's output (positions
-// t * kTile ...), the upsweep of the high-byte pass. Each wave keeps its own
-// row of counts and adds to it once per distinct digit of a 64-lane round (a
-// ballot match), so there are no LDS atomics: a slab seen edge on, whose
-// survivors share a few high bytes, does not serialise on a few counters.
+// t * kTile ...), the upsweep of the high-byte pass: counts only, so plain
+// LDS adds, each wave into its own row (a slab seen edge on, whose survivors
+// share a few high bytes, then contends within a wave only). A round whose
+// 64 digits are all one digit adds once.
 struct HistSmem {
   uint32_t h[4][256];
 };
@@ -397,10 +418,14 @@ __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-    const uint64_t valid = __ballot(i < limit);
     const uint32_t d = raw[r] >> 24;
-    const uint64_t peers = match_digit(d, valid);
-    if (i < limit && count_below(peers) == 0) sm.h[w][d] += (uint32_t)__popcll(peers);
+    const uint64_t valid = __ballot(i < limit);
+    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+    if (__ballot(d != d0) == 0) {
+      if (lane == 0) sm.h[w][d0] += (uint32_t)__popcll(valid);
+    } else if (i < limit) {
+      atomicAdd(&sm.h[w][d], 1u);
+    }
   }
   __syncthreads();
   tilehist3[(size_t)t * 256 + tid] = (uint16_t)(sm.h[0][tid] + sm.h[1][tid] + sm.h[2][tid] + sm.h[3][tid]);

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_scatter(const uint16_t *__restrict__ ke
                                                  const uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl,
                                                  uint32_t *__restrict__ out) {
   __shared__ ScatterSmem<kSuper> sm;
-  scatter_super<HI, kSuper>(sm, blockIdx.x, keys, mask, in_packed, n, hist, hstride, prefix,
+  scatter_super<HI, kSuper>(sm, xcd_item(blockIdx.x, gridDim.x), keys, mask, in_packed, n, hist, hstride, prefix,
                             HI ? ctl->total + 256 : ctl->total, ctl, out);
 }
 

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "winding_amd.h"
@@ -140,6 +141,9 @@ struct wd_context {
   uint64_t mem_total = 0;
   uint64_t mem_free_at_open = 0;
   bool closed = false;
+  // Clouds and states alive on this context: a batch checks its jobs against
+  // it before a run, so a destroyed job is an error, never a dangling pointer.
+  std::unordered_set<const void *> live;
   // Per-kernel event timing.
   bool profiling = false;
   struct Pending { int kernel; hipEvent_t a, b; };

@@ -17,11 +17,11 @@
 
 #include "winding_amd.h"
 
-enum { K_CTX = 1, K_CLOUD = 2, K_STATE = 3 };
+enum { K_CTX = 1, K_CLOUD = 2, K_STATE = 3, K_BATCH = 4, K_MLP = 5, K_LEARNED = 6 };
 
 typedef struct Box {
   int kind;
-  void *ptr;         /* wd_context*, wd_cloud*, wd_state*; NULL once destroyed */
+  void *ptr;         /* wd_context*, wd_cloud*, wd_state*, wd_batch*, wd_mlp*, wd_learned*; NULL once destroyed */
   napi_ref parent;   /* the context's external, for clouds and states */
   struct Box *ctx;   /* the context's box */
   int children;      /* contexts: live clouds and states */
@@ -46,6 +46,9 @@ static void release_box(napi_env env, Box *b) {
   if (b->ptr) {
     if (b->kind == K_CLOUD) wd_cloud_destroy((wd_cloud *)b->ptr);
     if (b->kind == K_STATE) wd_state_destroy((wd_state *)b->ptr);
+    if (b->kind == K_BATCH) wd_batch_destroy((wd_batch *)b->ptr);
+    if (b->kind == K_MLP) wd_mlp_destroy((wd_mlp *)b->ptr);
+    if (b->kind == K_LEARNED) wd_learned_destroy((wd_learned *)b->ptr);
     if (b->kind == K_CTX) wd_close((wd_context *)b->ptr);
     b->ptr = NULL;
     if (b->ctx) b->ctx->children--;
@@ -358,6 +361,300 @@ static napi_value js_profile_read(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* ---- batches, the predictor and the learned sort --------------------- */
+
+static napi_value handle_of(napi_env env, int kind, void *ptr, napi_value ctx_value, Box *ctx_box) {
+  return make_handle(env, kind, ptr, ctx_value, ctx_box);
+}
+
+static int u32_arg(napi_env env, napi_value v, uint32_t *out) {
+  double d;
+  if (napi_get_value_double(env, v, &d) != napi_ok || !(d >= 0) || d > 4294967295.0) {
+    napi_throw_type_error(env, NULL, "winding_amd: expected a count");
+    return 0;
+  }
+  *out = (uint32_t)d;
+  return 1;
+}
+
+/* Reads a JS array of n handles of `kind` into ptrs. */
+static int handle_list(napi_env env, napi_value arr, int kind, void **ptrs, uint32_t cap, uint32_t *n) {
+  bool is = false;
+  if (napi_is_array(env, arr, &is) != napi_ok || !is) {
+    napi_throw_type_error(env, NULL, "winding_amd: expected an array");
+    return 0;
+  }
+  napi_get_array_length(env, arr, n);
+  if (*n > cap) {
+    napi_throw_range_error(env, NULL, "winding_amd: too many items");
+    return 0;
+  }
+  for (uint32_t i = 0; i < *n; ++i) {
+    napi_value e;
+    napi_get_element(env, arr, i, &e);
+    Box *b = get_box(env, e, kind);
+    if (!b) return 0;
+    ptrs[i] = b->ptr;
+  }
+  return 1;
+}
+
+/* Reads a JS array of n Float32Array(37) uniforms. */
+static int view_list(napi_env env, napi_value arr, wd_view *views, uint32_t cap, uint32_t *n) {
+  bool is = false;
+  if (napi_is_array(env, arr, &is) != napi_ok || !is) {
+    napi_throw_type_error(env, NULL, "winding_amd: expected an array of uniforms");
+    return 0;
+  }
+  napi_get_array_length(env, arr, n);
+  if (*n > cap) {
+    napi_throw_range_error(env, NULL, "winding_amd: too many views");
+    return 0;
+  }
+  for (uint32_t i = 0; i < *n; ++i) {
+    napi_value e;
+    void *u;
+    size_t l;
+    napi_get_element(env, arr, i, &e);
+    if (!typed(env, e, napi_float32_array, &u, &l) || l < sizeof(wd_view) / 4) return 0;
+    memcpy(&views[i], u, sizeof(wd_view));
+  }
+  return 1;
+}
+
+#define MAX_LIST 4096
+
+/* batchCreate(context, maxJobs) -> batch */
+static napi_value js_batch_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  uint32_t m;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c || !u32_arg(env, argv[1], &m)) return NULL;
+  wd_batch *b = NULL;
+  int rc = wd_batch_create((wd_context *)c->ptr, m, &b);
+  return rc ? throw_wd(env, rc) : handle_of(env, K_BATCH, b, argv[0], c);
+}
+
+/* batchSet(batch, [clouds], [uniforms], [states]) */
+static napi_value js_batch_set(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  Box *b = get_box(env, argv[0], K_BATCH);
+  if (!b) return NULL;
+  uint32_t nc = 0, nv = 0, ns = 0;
+  void **clouds = (void **)calloc(MAX_LIST, sizeof(void *));
+  void **states = (void **)calloc(MAX_LIST, sizeof(void *));
+  wd_view *views = (wd_view *)calloc(MAX_LIST, sizeof(wd_view));
+  wd_job *jobs = (wd_job *)calloc(MAX_LIST, sizeof(wd_job));
+  napi_value ret = NULL;
+  if (clouds && states && views && jobs && handle_list(env, argv[1], K_CLOUD, clouds, MAX_LIST, &nc) &&
+      view_list(env, argv[2], views, MAX_LIST, &nv) && handle_list(env, argv[3], K_STATE, states, MAX_LIST, &ns)) {
+    if (nc != nv || nc != ns) {
+      napi_throw_range_error(env, NULL, "batch: clouds, uniforms and states differ in number");
+    } else {
+      for (uint32_t i = 0; i < nc; ++i) {
+        jobs[i].cloud = (const wd_cloud *)clouds[i];
+        jobs[i].view = views[i];
+        jobs[i].state = (wd_state *)states[i];
+      }
+      int rc = wd_batch_set((wd_batch *)b->ptr, jobs, nc);
+      if (rc) throw_wd(env, rc);
+    }
+  }
+  free(clouds);
+  free(states);
+  free(views);
+  free(jobs);
+  return ret;
+}
+
+/* batchRun(context, batch) */
+static napi_value js_batch_run(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *b = get_box(env, argv[1], K_BATCH);
+  if (!b) return NULL;
+  int rc = wd_batch_run((wd_context *)c->ptr, (wd_batch *)b->ptr);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
+/* batchTune(context, batch, reps) -> {schedule: 'phased'|'persistent', msPhased, msPersistent} */
+static napi_value js_batch_tune(napi_env env, napi_callback_info info) {
+  napi_value argv[3], out, sv;
+  uint32_t reps;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *b = get_box(env, argv[1], K_BATCH);
+  if (!b || !u32_arg(env, argv[2], &reps)) return NULL;
+  int sched = 0;
+  double a = 0, p = 0;
+  int rc = wd_batch_tune((wd_context *)c->ptr, (wd_batch *)b->ptr, reps, &sched, &a, &p);
+  if (rc) return throw_wd(env, rc);
+  CHECK_NAPI(napi_create_object(env, &out));
+  napi_create_string_utf8(env, sched == WD_SCHEDULE_PERSISTENT ? "persistent" : "phased", NAPI_AUTO_LENGTH, &sv);
+  napi_set_named_property(env, out, "schedule", sv);
+  napi_set_named_property(env, out, "msPhased", num(env, a));
+  napi_set_named_property(env, out, "msPersistent", num(env, p));
+  return out;
+}
+
+/* mlpCreate(context, seed) -> predictor */
+static napi_value js_mlp_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  uint32_t seed;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c || !u32_arg(env, argv[1], &seed)) return NULL;
+  wd_mlp *m = NULL;
+  int rc = wd_mlp_create((wd_context *)c->ptr, seed, &m);
+  return rc ? throw_wd(env, rc) : handle_of(env, K_MLP, m, argv[0], c);
+}
+
+/* mlpSave(predictor) -> Uint8Array;  mlpLoad(predictor, Uint8Array) */
+static napi_value js_mlp_save(napi_env env, napi_callback_info info) {
+  napi_value argv[1], ab, arr;
+  if (!get_args(env, info, 1, argv)) return NULL;
+  Box *m = get_box(env, argv[0], K_MLP);
+  if (!m) return NULL;
+  uint64_t size = 0;
+  int rc = wd_mlp_save((wd_mlp *)m->ptr, NULL, 0, &size);
+  if (rc) return throw_wd(env, rc);
+  void *data;
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)size, &data, &ab));
+  rc = wd_mlp_save((wd_mlp *)m->ptr, data, size, &size);
+  if (rc) return throw_wd(env, rc);
+  napi_create_typedarray(env, napi_uint8_array, (size_t)size, ab, 0, &arr);
+  return arr;
+}
+
+static napi_value js_mlp_load(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  void *data;
+  size_t l;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *m = get_box(env, argv[0], K_MLP);
+  if (!m || !typed(env, argv[1], napi_uint8_array, &data, &l)) return NULL;
+  int rc = wd_mlp_load((wd_mlp *)m->ptr, data, l);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
+/* mlpTrain(context, predictor, poses Float32Array(n*12), targets Float32Array(n*256), epochs, batch, lr, seed) -> loss */
+static napi_value js_mlp_train(napi_env env, napi_callback_info info) {
+  napi_value argv[8];
+  void *x, *t;
+  size_t lx, lt;
+  uint32_t epochs, batch, seed;
+  double lr;
+  if (!get_args(env, info, 8, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *m = get_box(env, argv[1], K_MLP);
+  if (!m || !typed(env, argv[2], napi_float32_array, &x, &lx) || !typed(env, argv[3], napi_float32_array, &t, &lt))
+    return NULL;
+  if (!u32_arg(env, argv[4], &epochs) || !u32_arg(env, argv[5], &batch) || !u32_arg(env, argv[7], &seed)) return NULL;
+  CHECK_NAPI(napi_get_value_double(env, argv[6], &lr));
+  const uint32_t n = (uint32_t)(lx / WD_POSE_FLOATS);
+  if (lx != (size_t)n * WD_POSE_FLOATS || lt != (size_t)n * WD_COARSE_BINS) {
+    napi_throw_range_error(env, NULL, "mlp: poses are n x 12 floats and targets n x 256");
+    return NULL;
+  }
+  float loss = 0.f;
+  int rc = wd_mlp_train((wd_context *)c->ptr, (wd_mlp *)m->ptr, (const float *)x, (const float *)t, n, epochs, batch,
+                        (float)lr, seed, &loss);
+  return rc ? throw_wd(env, rc) : num(env, loss);
+}
+
+/* mlpPredict(context, predictor, poses Float32Array(n*12)) -> Float32Array(n*256) */
+static napi_value js_mlp_predict(napi_env env, napi_callback_info info) {
+  napi_value argv[3], ab, arr;
+  void *x, *out;
+  size_t lx;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *m = get_box(env, argv[1], K_MLP);
+  if (!m || !typed(env, argv[2], napi_float32_array, &x, &lx)) return NULL;
+  const uint32_t n = (uint32_t)(lx / WD_POSE_FLOATS);
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)n * WD_COARSE_BINS * 4, &out, &ab));
+  int rc = wd_mlp_predict((wd_context *)c->ptr, (wd_mlp *)m->ptr, (const float *)x, n, (float *)out);
+  if (rc) return throw_wd(env, rc);
+  napi_create_typedarray(env, napi_float32_array, (size_t)n * WD_COARSE_BINS, ab, 0, &arr);
+  return arr;
+}
+
+/* poseFeatures(uniform Float32Array(37)) -> Float32Array(12) */
+static napi_value js_pose_features(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  void *u;
+  size_t l;
+  if (!get_args(env, info, 1, argv)) return NULL;
+  if (!typed(env, argv[0], napi_float32_array, &u, &l) || l < sizeof(wd_view) / 4) return NULL;
+  wd_view v;
+  float f[WD_POSE_FLOATS];
+  memcpy(&v, u, sizeof v);
+  int rc = wd_pose_features(&v, f);
+  return rc ? throw_wd(env, rc) : f32_array(env, f, WD_POSE_FLOATS);
+}
+
+/* stateQuantiles(context, state) -> Uint32Array(257) */
+static napi_value js_state_quantiles(napi_env env, napi_callback_info info) {
+  napi_value argv[2], ab, arr;
+  void *data;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *s = get_box(env, argv[1], K_STATE);
+  if (!s) return NULL;
+  CHECK_NAPI(napi_create_arraybuffer(env, 257 * 4, &data, &ab));
+  int rc = wd_state_quantiles((wd_context *)c->ptr, (wd_state *)s->ptr, (uint32_t *)data);
+  if (rc) return throw_wd(env, rc);
+  napi_create_typedarray(env, napi_uint32_array, 257, ab, 0, &arr);
+  return arr;
+}
+
+/* learnedCreate(context, maxViews) -> learned */
+static napi_value js_learned_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  uint32_t m;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c || !u32_arg(env, argv[1], &m)) return NULL;
+  wd_learned *l = NULL;
+  int rc = wd_learned_create((wd_context *)c->ptr, m, &l);
+  return rc ? throw_wd(env, rc) : handle_of(env, K_LEARNED, l, argv[0], c);
+}
+
+/* learnedRun(context, learned, predictor, cloud, [uniforms], [states], cap) */
+static napi_value js_learned_run(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  uint32_t nv = 0, ns = 0, cap;
+  wd_view views[8];
+  void *states[8];
+  if (!get_args(env, info, 7, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *l = get_box(env, argv[1], K_LEARNED);
+  if (!l) return NULL;
+  Box *m = get_box(env, argv[2], K_MLP);
+  if (!m) return NULL;
+  Box *cl = get_box(env, argv[3], K_CLOUD);
+  if (!cl) return NULL;
+  if (!view_list(env, argv[4], views, 8, &nv) || !handle_list(env, argv[5], K_STATE, states, 8, &ns) ||
+      !u32_arg(env, argv[6], &cap))
+    return NULL;
+  if (nv != ns) {
+    napi_throw_range_error(env, NULL, "learned: uniforms and states differ in number");
+    return NULL;
+  }
+  int rc = wd_learned_run((wd_context *)c->ptr, (wd_learned *)l->ptr, (wd_mlp *)m->ptr, (const wd_cloud *)cl->ptr,
+                          views, (wd_state *const *)states, nv, cap);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   const struct {
     const char *name;
@@ -367,6 +664,11 @@ static napi_value init(napi_env env, napi_value exports) {
       {"cloudCreate", js_cloud_create}, {"destroy", js_destroy},  {"viewPrepare", js_view_prepare},
       {"stateCreate", js_state_create}, {"sort", js_sort},        {"stateRead", js_state_read},
       {"synth", js_synth},         {"profile", js_profile},       {"profileRead", js_profile_read},
+      {"batchCreate", js_batch_create}, {"batchSet", js_batch_set}, {"batchRun", js_batch_run},
+      {"batchTune", js_batch_tune},     {"mlpCreate", js_mlp_create}, {"mlpSave", js_mlp_save},
+      {"mlpLoad", js_mlp_load},         {"mlpTrain", js_mlp_train}, {"mlpPredict", js_mlp_predict},
+      {"poseFeatures", js_pose_features}, {"stateQuantiles", js_state_quantiles},
+      {"learnedCreate", js_learned_create}, {"learnedRun", js_learned_run},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; ++i) {
     napi_value f;

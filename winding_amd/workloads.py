@@ -93,3 +93,39 @@ def orbit_sequence(count=1024, radius=10.0):
         a = 2.0 * math.pi * k / count
         eyes.append((radius * math.sin(a), 3.0 * math.sin(3.0 * a), radius * math.cos(a)))
     return eyes
+
+
+LEARNED = dict(n=4194304, kind=0, seed=0, width=1280, height=720, train_poses=4096, train_seed=4242)
+
+
+def view_at(cloud, eye, width, height, cull=WD_SPLAT_CULL, target=(0.0, 0.0, 0.0)):
+    """The per-view uniform for a cloud already on the device (its box from the upload check)."""
+    v, p = camera(eye, width, height, target)
+    return prepare_view(v, p, float(width), float(height), cloud.min, cloud.max, cull)
+
+
+def training_poses(count, seed, radius=(8.0, 12.0), height=3.0):
+    """Random eyes of the orbit family: any angle about y, radius and height drawn uniformly."""
+    st = Stream(seed)
+    eyes = []
+    for _ in range(count):
+        a = 2.0 * math.pi * st.uniform()
+        r = radius[0] + (radius[1] - radius[0]) * st.uniform()
+        y = height * (2.0 * st.uniform() - 1.0)
+        eyes.append((r * math.sin(a), y, r * math.cos(a)))
+    return eyes
+
+
+def training_set(ctx, cloud, state, eyes, width, height, cull=WD_SPLAT_CULL):
+    """(poses, targets) for the predictor: each eye's 12 pose floats, and the 256 equal-mass bin widths of
+    the keys its exact sort produces (the sort generates its own training data)."""
+    from . import pose_features, widths_from_edges
+    X, Y = [], []
+    for eye in eyes:
+        view = view_at(cloud, eye, width, height, cull)
+        if view is None:
+            continue
+        state.sort(cloud, view)
+        X.append(pose_features(view))
+        Y.append(widths_from_edges(state.quantiles()))
+    return np.array(X, np.float32), np.array(Y, np.float32)
