@@ -97,5 +97,63 @@ test('a NaN position is refused at upload, by index', () => {
   assert.throws(() => new wd.Splats(gpu, wd.parseSplats(bytes)), /splat 1 has a position that is not a number/);
 });
 
+test('three clouds sorted in one batch match each cloud sorted alone', () => {
+  const cam = new wd.Camera({ fovY: Math.PI / 3, near: 0.1 });
+  cam.position.set([2, 1, 9]);
+  cam.update(1280 / 720);
+  const sizes = [5000, 70001, 4096];
+  const clouds = sizes.map((n, k) => new wd.Splats(gpu, wd.parseSplats(gpu.native.synth(k % 2, 50 + k, n))));
+  const scene = new wd.SplatScene();
+  const entities = clouds.map((splats) => scene.addSplats({ splats }));
+  const pass = new wd.SplatPass(gpu);
+  pass.prepare(scene, cam, null, 1280, 720);
+  assert.strictEqual(pass.sort(), 3);   // one batch launch
+  const batched = entities.map((e) => pass.read(e));
+  clouds.forEach((splats, k) => {
+    const one = sortOnce(gpu.native.synth(k % 2, 50 + k, sizes[k]), cam, 1280, 720, 0.5).got;
+    assert.strictEqual(batched[k].count, one.count);
+    assert.deepStrictEqual(Array.from(batched[k].order), Array.from(one.order));
+  });
+  pass.destroy();
+  clouds.forEach((c) => c.destroy());
+});
+
+test('learned sort: a predictor trained on the exact sort, then 4 views, each equal to the exact order', () => {
+  const n = 200000;
+  const splats = new wd.Splats(gpu, wd.parseSplats(gpu.native.synth(0, 3, n)));
+  const world = new Float32Array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]);
+  const uniformAt = (a, r, y) => {
+    const cam = new wd.Camera({ fovY: Math.PI / 3, near: 0.1 });
+    cam.position.set([r * Math.sin(a), y, r * Math.cos(a)]);
+    cam.update(1280 / 720);
+    return wd.cloudUniform(new Float32Array(37), cam.view, world, cam.projection, 1280, 720, splats.min, splats.max, 0.5).slice();
+  };
+  const train = [];
+  for (let k = 0; k < 256; k++) train.push(uniformAt(2 * Math.PI * k / 256, 9 + 3 * ((k * 37) % 11) / 11, 3 * Math.sin(k)));
+  const data = wd.trainingSet(gpu, splats, train);
+  const predictor = new wd.Predictor(gpu, 1);
+  predictor.train(data.poses, data.targets, { epochs: 150, batch: 64, lr: 2e-3, seed: 2 });
+  const bytes = predictor.save();
+  const copy = new wd.Predictor(gpu, 7);
+  copy.load(bytes);
+  const sorter = new wd.LearnedSort(gpu, copy, 8);
+  const views = [0.3, 1.9, 3.4, 5.0].map((a) => uniformAt(a, 10, 1));
+  sorter.sort(splats, views);
+  const exact = gpu.native.stateCreate(gpu.handle, n);
+  views.forEach((u, k) => {
+    const got = sorter.read(k);
+    gpu.native.sort(gpu.handle, splats.handle, u, exact);
+    const want = gpu.native.stateRead(gpu.handle, exact, 0);
+    assert.strictEqual(got.count, want.count);
+    assert.deepStrictEqual(Array.from(got.order), Array.from(want.order));
+    assert.strictEqual(got.fallback, 0);
+  });
+  gpu.native.destroy(exact);
+  sorter.destroy();
+  copy.destroy();
+  predictor.destroy();
+  splats.destroy();
+});
+
 gpu.destroy();
 console.log(passed + ' checks passed');

@@ -113,6 +113,7 @@ __global__ __launch_bounds__(256) void k_project_views(const BatchJob *__restric
   auto work = [&](uint32_t tile, const uint4 &a, const uint4 &b) {
     const uint32_t i = (tile << 8) + tid;
     const bool in = i < n;
+    const SplatLocal L = splat_local(a, b);  // the view-independent part, once for every view
 #pragma unroll 1
     for (uint32_t v = 0; v < nv; ++v) {
       const BatchJob &J = jobs[v];
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void k_project_views(const BatchJob *__restric
       const float bw0 = fabsf(vk.mv[0]) + fabsf(vk.mv[4]) + fabsf(vk.mv[8]);
       const float bw1 = fabsf(vk.mv[1]) + fabsf(vk.mv[5]) + fabsf(vk.mv[9]);
       const float bw2 = fabsf(vk.mv[2]) + fabsf(vk.mv[6]) + fabsf(vk.mv[10]);
-      Projected p = project_one<STD>(vk, a, b, bw0, bw1, bw2);
+      Projected p = project_with<STD, true>(vk, a, b, &L, bw0, bw1, bw2);
       p.keep = p.keep && in;
       const uint64_t bal = __ballot(p.keep);
       if ((tid & 63u) == 0 && in) J.mask[i >> 6] = bal;
@@ -284,14 +285,18 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const BatchJob *__restrict__ 
   if (c == 0) return;
   const uint32_t chunk = ((c + 1023u) >> 10) << 6;  // a multiple of 64 a wave
   const uint32_t w0 = min(w * chunk, c), w1 = min(w0 + chunk, c);
+  // counts first: LDS adds (a round of one offset adds once), no ranks needed yet
   for (uint32_t p0 = w0; p0 < w1; p0 += 64) {
     const uint32_t pos = p0 + lane;
     const bool ok = pos < w1;
     const uint64_t valid = __ballot(ok);
-    const uint32_t v = ok ? J.packed[base + pos] : 0u;
-    const uint32_t k = v >> 23;
-    const uint64_t peers = match_bin(k, valid);
-    if (ok && count_below(peers) == 0) cnt[w][k] += (uint32_t)__popcll(peers);
+    const uint32_t k = ok ? J.packed[base + pos] >> 23 : 0u;
+    const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
+    if (__ballot(ok && k != k0) == 0) {
+      if (lane == 0) cnt[w][k0] += (uint32_t)__popcll(valid);
+    } else if (ok) {
+      atomicAdd(&cnt[w][k], 1u);
+    }
   }
   __syncthreads();
   {

@@ -161,9 +161,42 @@ struct Projected {
   float xx, xy, yy;
 };
 
-template <bool STD>
-__device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 b, float bn0w0, float bn1w1,
-                                                 float bw2) {
+// What the footprint needs of a splat whatever the view: its rotation from
+// the byte quaternion (unnormalised: the integer products are exact in f32)
+// and its scales. A kernel projecting one record for several views makes it
+// once.
+struct SplatLocal {
+  float r00, r01, r02, r10, r11, r12, r20, r21, r22;
+  float sx, sy, sz;
+};
+
+__device__ __forceinline__ SplatLocal splat_local(uint4 a, uint4 b) {
+#pragma clang fp contract(fast)
+  SplatLocal L;
+  const uint32_t qb = b.w;
+  const float qw = (float)(qb & 255u) - 128.f, qx = (float)((qb >> 8) & 255u) - 128.f;
+  const float qy = (float)((qb >> 16) & 255u) - 128.f, qz = (float)(qb >> 24) - 128.f;
+  const float xx2 = qx * qx, yy2 = qy * qy, zz2 = qz * qz;
+  const float n2 = qw * qw + xx2 + yy2 + zz2;
+  const float s = n2 > 0.f ? 2.f * __builtin_amdgcn_rcpf(n2) : 0.f;
+  L.r00 = 1.f - s * (yy2 + zz2);
+  L.r01 = s * (qx * qy - qw * qz);
+  L.r02 = s * (qx * qz + qw * qy);
+  L.r10 = s * (qx * qy + qw * qz);
+  L.r11 = 1.f - s * (xx2 + zz2);
+  L.r12 = s * (qy * qz - qw * qx);
+  L.r20 = s * (qx * qz - qw * qy);
+  L.r21 = s * (qy * qz + qw * qx);
+  L.r22 = 1.f - s * (xx2 + yy2);
+  L.sx = f32_of(a.w);
+  L.sy = f32_of(b.x);
+  L.sz = f32_of(b.y);
+  return L;
+}
+
+template <bool STD, bool HAVE_LOCAL>
+__device__ __forceinline__ Projected project_with(const ViewK &v, uint4 a, uint4 b, const SplatLocal *Lp,
+                                                  float bn0w0, float bn1w1, float bw2) {
   const double x = f32_of(a.x), y = f32_of(a.y), z = f32_of(a.z);
   // view = modelView * (p, 1); each product is exact, so fma is the sum as written.
   const double vx = fma(v.dmv[8], z, fma(v.dmv[4], y, v.dmv[0] * x)) + v.dmv[12];
@@ -216,17 +249,9 @@ __device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 
     // perspective's third column folded in: T = diag(j0, j1) E.
     const float e00 = xd * v.mv[2] + v.mv[0], e01 = xd * v.mv[6] + v.mv[4], e02 = xd * v.mv[10] + v.mv[8];
     const float e10 = yd * v.mv[2] + v.mv[1], e11 = yd * v.mv[6] + v.mv[5], e12 = yd * v.mv[10] + v.mv[9];
-    // R from the byte quaternion, unnormalised: integer products are exact in f32.
-    const uint32_t qb = b.w;
-    const float qw = (float)(qb & 255u) - 128.f, qx = (float)((qb >> 8) & 255u) - 128.f;
-    const float qy = (float)((qb >> 16) & 255u) - 128.f, qz = (float)(qb >> 24) - 128.f;
-    const float xx2 = qx * qx, yy2 = qy * qy, zz2 = qz * qz;
-    const float n2 = qw * qw + xx2 + yy2 + zz2;
-    const float s = n2 > 0.f ? 2.f * __builtin_amdgcn_rcpf(n2) : 0.f;
-    const float r00 = 1.f - s * (yy2 + zz2), r01 = s * (qx * qy - qw * qz), r02 = s * (qx * qz + qw * qy);
-    const float r10 = s * (qx * qy + qw * qz), r11 = 1.f - s * (xx2 + zz2), r12 = s * (qy * qz - qw * qx);
-    const float r20 = s * (qx * qz - qw * qy), r21 = s * (qy * qz + qw * qx), r22 = 1.f - s * (xx2 + yy2);
-    const float sx = f32_of(a.w), sy = f32_of(b.x), sz = f32_of(b.y);
+    const SplatLocal L = HAVE_LOCAL ? *Lp : splat_local(a, b);
+    const float r00 = L.r00, r01 = L.r01, r02 = L.r02, r10 = L.r10, r11 = L.r11, r12 = L.r12;
+    const float r20 = L.r20, r21 = L.r21, r22 = L.r22, sx = L.sx, sy = L.sy, sz = L.sz;
     const float g00 = (e00 * r00 + e01 * r10 + e02 * r20) * sx;
     const float g01 = (e00 * r01 + e01 * r11 + e02 * r21) * sy;
     const float g02 = (e00 * r02 + e01 * r12 + e02 * r22) * sz;
@@ -265,6 +290,12 @@ __device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 
     }
   }
   return o;
+}
+
+template <bool STD>
+__device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 b, float bn0w0, float bn1w1,
+                                                 float bw2) {
+  return project_with<STD, false>(v, a, b, nullptr, bn0w0, bn1w1, bw2);
 }
 
 // ---------------------------------------------------------------------------

@@ -8,5 +8,6 @@ export const {
   SPLAT_BYTES, BUCKETS, SPLAT_CULL, CLOUD_FLOATS, KEEP_FRAMES,
   native, Gpu, parseSplats, Splats, SplatScene, SplatPass, cloudUniform,
   mat4Create, mat4Multiply, mat4LookAt, mat4PerspectiveReverseZInfinite, mat4OrthographicReverseZ, Camera,
+  POSE_FLOATS, BINS, Predictor, LearnedSort, poseFeatures, trainingSet,
 } = api;
 export default api;

@@ -233,18 +233,29 @@ class SplatPass {
     return items.length;
   }
 
-  /** Sorts on the GPU every cloud prepare marked; returns how many. Enqueued, not waited for. */
+  /**
+   * Sorts on the GPU every cloud prepare marked; returns how many. Enqueued, not waited for.
+   * One cloud is one sort; more are one batch, every marked cloud in one graph launch (what
+   * _encodeSort's loop of per-cloud dispatches becomes, src/render/splats.js:676-697).
+   */
   sort() {
-    let sorted = 0;
-    for (const item of this._items) {
-      if (!item.sort) continue;
-      const state = item.state;
-      state.last.set(state.next);
+    const marked = this._items.filter((i) => i.sort);
+    for (const item of marked) item.state.last.set(item.state.next);
+    if (marked.length === 1) {
+      const state = marked[0].state;
       this.native.sort(this.gpu.handle, state.splats.handle, state.uniform, state.handle);
-      state.sortedOnce = true;
-      sorted++;
+    } else if (marked.length > 1) {
+      if (this._batch === undefined || this._batchJobs < marked.length) {
+        if (this._batch !== undefined) this.native.destroy(this._batch);
+        this._batchJobs = Math.max(8, marked.length);
+        this._batch = this.native.batchCreate(this.gpu.handle, this._batchJobs);
+      }
+      this.native.batchSet(this._batch, marked.map((i) => i.state.splats.handle),
+        marked.map((i) => i.state.uniform), marked.map((i) => i.state.handle));
+      this.native.batchRun(this.gpu.handle, this._batch);
     }
-    return sorted;
+    for (const item of marked) item.state.sortedOnce = true;
+    return marked.length;
   }
 
   /** The clouds of the last prepare, far first: [{ entity, count }]. */
@@ -284,6 +295,8 @@ class SplatPass {
   }
 
   destroy() {
+    if (this._batch !== undefined) this.native.destroy(this._batch);
+    this._batch = undefined;
     for (const state of this._states.values()) this._destroyState(state);
     this._states.clear();
     this._items.length = 0;

@@ -303,6 +303,8 @@ static napi_value js_state_read(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, out, "order", arr);
   napi_set_named_property(env, out, "count", num(env, st.count));
   napi_set_named_property(env, out, "recomputed", num(env, st.recomputed));
+  napi_set_named_property(env, out, "fallback", num(env, st.fallback));
+  napi_set_named_property(env, out, "overflowBins", num(env, st.overflow_bins));
   if (cov_ab) {
     napi_create_typedarray(env, napi_float32_array, (size_t)ncov * 4, cov_ab, 0, &arr);
     napi_set_named_property(env, out, "cov2d", arr);
