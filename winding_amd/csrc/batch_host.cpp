@@ -1,5 +1,9 @@
-// Host side of batches (wd_batch_*): the job table, the ticket queue's item
-// counts, and the hipGraph the persistent batch kernel is replayed from.
+// Host side of batches (wd_batch_*): the job table, the item starts, the
+// hipGraph a schedule is replayed from, and the autotuner.
+//
+// set() uploads the tables from pinned staging (two buffers used in turn; one
+// is refilled only once the copy out of it has finished), in stream order
+// behind the runs already enqueued, so a new frame's views cost no host wait.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -23,6 +27,11 @@ int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out) {
   if (e == hipSuccess) e = hipMalloc(&b->starts, sizeof(uint32_t) * 2 * (max_jobs + 1));
   if (e == hipSuccess) e = hipMalloc(&b->ctl, ctl_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(b->ctl, 0, ctl_bytes, ctx->stream);
+  b->stage_bytes = sizeof(wd::BatchJob) * max_jobs + sizeof(uint32_t) * 2 * (max_jobs + 1);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+    e = hipHostMalloc(&b->stage[i], b->stage_bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     wd_batch_destroy(b);
@@ -77,14 +86,22 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   std::copy(u.begin(), u.end(), starts.begin() + (n + 1));
   wd_context *ctx = b->ctx;
   WD_HIP(hipSetDevice(ctx->device), "batch: set device");
-  // The table is read by the kernel at run time: a later set() with new views
+  // The table is read by the kernels at run time: a later set() with new views
   // and the same clouds and states replays the same graph.
-  WD_HIP(hipMemcpyAsync(b->jobs, hj.data(), sizeof(wd::BatchJob) * n, hipMemcpyHostToDevice, ctx->stream),
+  const int f = b->flip;
+  if (b->staged_used[f]) WD_HIP(hipEventSynchronize(b->staged[f]), "batch: staging");
+  uint8_t *stage = (uint8_t *)b->stage[f];
+  std::memcpy(stage, hj.data(), sizeof(wd::BatchJob) * n);
+  std::memcpy(stage + sizeof(wd::BatchJob) * b->max_jobs, starts.data(), sizeof(uint32_t) * starts.size());
+  WD_HIP(hipMemcpyAsync(b->jobs, stage, sizeof(wd::BatchJob) * n, hipMemcpyHostToDevice, ctx->stream),
          "batch: job table");
-  WD_HIP(hipMemcpyAsync(b->starts, starts.data(), sizeof(uint32_t) * starts.size(), hipMemcpyHostToDevice,
-                        ctx->stream), "batch: item table");
-  WD_HIP(hipStreamSynchronize(ctx->stream), "batch: upload");
+  WD_HIP(hipMemcpyAsync(b->starts, stage + sizeof(wd::BatchJob) * b->max_jobs, sizeof(uint32_t) * starts.size(),
+                        hipMemcpyHostToDevice, ctx->stream), "batch: item table");
+  WD_HIP(hipEventRecord(b->staged[f], ctx->stream), "batch: staging");
+  b->staged_used[f] = true;
+  b->flip ^= 1;
   const uint32_t total = 2 * t[n] + 2 * u[n] + (512 / wd::kScanCols + 256 / wd::kScanCols) * n;
+  const bool same_total = total == b->total;
   if (shape != b->shape || n != b->n || total != b->total) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
@@ -96,7 +113,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   b->shape.swap(shape);
   b->n = n;
   b->total = total;
-  return wd::batch_grid(ctx, total, &b->grid);
+  return same_total && b->grid ? WD_OK : wd::batch_grid(ctx, total, &b->grid);
 }
 
 int wd_batch_run(wd_context *ctx, wd_batch *b) {
@@ -187,6 +204,10 @@ int wd_batch_destroy(wd_batch *b) {
   if (b->jobs) hipFree(b->jobs);
   if (b->starts) hipFree(b->starts);
   if (b->ctl) hipFree(b->ctl);
+  for (int i = 0; i < 2; ++i) {
+    if (b->stage[i]) hipHostFree(b->stage[i]);
+    if (b->staged[i]) hipEventDestroy(b->staged[i]);
+  }
   b->magic = 0;
   delete b;
   return WD_OK;

@@ -7,9 +7,9 @@
 //   k_bin_table      the widths as bin edges on the 16-bit key line; a bin
 //                    wider than kBinWidth keys is split, so a key's offset in
 //                    its bin fits 9 bits. Writes key -> bin (65536 u16).
-//   k_project_views  pass 1 for every view from one read of each record: the
-//                    same keys, survivor bits and cov2d as k_project, and the
-//                    chunk's histogram over bins instead of over key bytes.
+//   k_project_bins   pass 1 of every view: the same keys, survivor bits and
+//                    cov2d as k_project, and the chunk's histogram over bins
+//                    instead of over key bytes.
 //   (k_scan_bins)    column prefixes and totals of the bin histograms.
 //   k_bin_scatter    the check: every bin's total is at most the cap, or this
 //                    view falls back. Then one stable pass by bin: the
@@ -86,65 +86,17 @@ __global__ __launch_bounds__(256) void k_bin_table(const BatchJob *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Pass 1 for nv views of one cloud, a workgroup a chunk of kTile splats: each
-// record is loaded once and projected for every view.
-struct ViewsSmem {
-  uint32_t h[kMaxViews][512];  // survivors per bin, per view
-  uint32_t recomp[kMaxViews];
-};
-
+// Pass 1 of each view, its histogram over the view's bins. Grid (tiles,
+// views): a view's chunks are separate work items, like the batch's. (One
+// workgroup projecting a chunk for every view reads the records once, but at
+// the sizes this serves the records stay in the MALL between views, and the
+// separate items hide latency better: it measured slower.)
 template <bool STD>
-__global__ __launch_bounds__(256) void k_project_views(const BatchJob *__restrict__ jobs, uint32_t nv) {
-  __shared__ ViewsSmem sm;
-  const BatchJob &J0 = jobs[0];
-  const uint4 *__restrict__ rec = (const uint4 *)J0.rec;
-  const uint32_t n = J0.n, chunk = blockIdx.x, tid = threadIdx.x;
-  for (uint32_t k = tid; k < nv * 512u; k += 256) sm.h[k >> 9][k & 511u] = 0;
-  if (tid < kMaxViews) sm.recomp[tid] = 0;
-  __syncthreads();
-  const uint32_t first = chunk * (kTile / 256), end = min(first + kTile / 256, (n + 255u) >> 8);
-  auto load = [&](uint32_t tile, uint4 &a, uint4 &b) {
-    const uint32_t i = (tile << 8) + tid;
-    if (tile < end && i < n) {
-      a = rec[2 * (size_t)i];
-      b = rec[2 * (size_t)i + 1];
-    }
-  };
-  auto work = [&](uint32_t tile, const uint4 &a, const uint4 &b) {
-    const uint32_t i = (tile << 8) + tid;
-    const bool in = i < n;
-    const SplatLocal L = splat_local(a, b);  // the view-independent part, once for every view
-#pragma unroll 1
-    for (uint32_t v = 0; v < nv; ++v) {
-      const BatchJob &J = jobs[v];
-      const ViewK &vk = J.view;
-      const float bw0 = fabsf(vk.mv[0]) + fabsf(vk.mv[4]) + fabsf(vk.mv[8]);
-      const float bw1 = fabsf(vk.mv[1]) + fabsf(vk.mv[5]) + fabsf(vk.mv[9]);
-      const float bw2 = fabsf(vk.mv[2]) + fabsf(vk.mv[6]) + fabsf(vk.mv[10]);
-      Projected p = project_with<STD, true>(vk, a, b, &L, bw0, bw1, bw2);
-      p.keep = p.keep && in;
-      const uint64_t bal = __ballot(p.keep);
-      if ((tid & 63u) == 0 && in) J.mask[i >> 6] = bal;
-      if (in) {
-        J.keys[i] = (uint16_t)p.key;
-        ((float4 *)J.cov)[i] =
-            p.keep ? make_float4(p.xx, p.xy, p.yy, __uint_as_float(b.z)) : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-      if (p.recomputed && in) atomicAdd(&sm.recomp[v], 1u);
-      if (p.keep) atomicAdd(&sm.h[v][J.lut[p.key]], 1u);
-    }
-  };
-  uint4 a0 = make_uint4(0, 0, 0, 0), b0 = a0, a1 = a0, b1 = a0;
-  load(first, a0, b0);
-  for (uint32_t tile = first; tile < end; tile += 2) {
-    load(tile + 1, a1, b1);
-    work(tile, a0, b0);
-    load(tile + 2, a0, b0);
-    if (tile + 1 < end) work(tile + 1, a1, b1);
-  }
-  __syncthreads();
-  for (uint32_t k = tid; k < nv * 512u; k += 256) jobs[k >> 9].tilehist[(size_t)chunk * 512 + (k & 511u)] = (uint16_t)sm.h[k >> 9][k & 511u];
-  if (tid < nv && sm.recomp[tid]) atomicAdd(&jobs[tid].ctl->recomputed_acc, sm.recomp[tid]);
+__global__ __launch_bounds__(256) void k_project_bins(const BatchJob *__restrict__ jobs) {
+  __shared__ ProjectSmem sm;
+  const BatchJob &J = jobs[blockIdx.y];
+  project_chunk<STD, true>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
+                           &J.ctl->recomputed_acc, blockIdx.x, J.lut);
 }
 
 // Column prefixes and totals of the bin histograms. Grid (512 / kScanCols, views).
@@ -226,8 +178,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict_
     const uint32_t dd = bin[r] & 511u;
     const uint64_t peers = match_bin(dd, valid);
     const uint32_t below = count_below(peers);
-    uint32_t before = 0;
-    if (ok) before = sm.wh[w][dd];
+    const uint32_t before = sm.wh[w][dd];  // any bin is a valid index: read unconditionally
     if (ok && below == 0) sm.wh[w][dd] = before + (uint32_t)__popcll(peers);
     rank[r] = before + below;
     okbits |= (ok ? 1u : 0u) << r;
@@ -244,22 +195,54 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict_
     sm.wh[3][d] = base + c0 + c1 + c2;
   }
   __syncthreads();
+  uint32_t pos[KPT], lo[KPT];
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
-    if ((okbits >> r) & 1u) {
-      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-      const uint32_t dd = bin[r] & 511u;
-      const uint32_t off = (raw[r] - (uint32_t)sm.binlo[dd]) & 511u;  // below kBinWidth by the table's construction
-      const uint32_t pos = sm.wh[w][dd] + rank[r];
-      if (pos < (uint32_t)kTile) sm.ent[pos] = (dd << 23) | (off << 14) | (i - tile0);
-    }
+    pos[r] = sm.wh[w][bin[r] & 511u] + rank[r];
+    lo[r] = sm.binlo[bin[r] & 511u];
+  }
+#pragma unroll
+  for (int r = 0; r < KPT; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+    const uint32_t dd = bin[r] & 511u;
+    const uint32_t off = (raw[r] - lo[r]) & 511u;  // below kBinWidth by the table's construction
+    if (((okbits >> r) & 1u) && pos[r] < (uint32_t)kTile) sm.ent[pos[r]] = (dd << 23) | (off << 14) | (i - tile0);
   }
   __syncthreads();
-  for (uint32_t p = tid; p < nvalid; p += 256) {
-    const uint32_t e = sm.ent[p];
-    const uint32_t g = sm.shift[e >> 23] + p;
-    if (g < n) J.packed[g] = (((e >> 14) & 511u) << 23) | (tile0 + (e & 16383u));
+  for (uint32_t p0 = 0; p0 < nvalid; p0 += 8 * 256) {
+    uint32_t e[8], g[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t p = p0 + (uint32_t)k * 256u + tid;
+      e[k] = p < nvalid ? sm.ent[p] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) g[k] = sm.shift[e[k] >> 23] + p0 + (uint32_t)k * 256u + tid;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t p = p0 + (uint32_t)k * 256u + tid;
+      if (p < nvalid && g[k] < n) J.packed[g[k]] = (((e[k] >> 14) & 511u) << 23) | (tile0 + (e[k] & 16383u));
+    }
   }
+}
+
+// The counts of k_bin_sort's 16 waves per offset -> where each wave's run of
+// each offset starts in the bin (offsets ascending, waves in order).
+__device__ __forceinline__ void bin_offsets(uint32_t (*cnt)[512], uint32_t *s_w) {
+  __syncthreads();
+  const uint32_t tid = threadIdx.x;
+  uint32_t s = 0;
+  if (tid < 512u)
+    for (int k = 0; k < 16; ++k) s += cnt[k][tid];
+  uint32_t sum;
+  uint32_t run = block_scan_excl_n<16>(s, s_w, &sum);
+  if (tid < 512u)
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t x = cnt[k][tid];
+      cnt[k][tid] = run;
+      run += x;
+    }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -285,7 +268,45 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const BatchJob *__restrict__ 
   if (c == 0) return;
   const uint32_t chunk = ((c + 1023u) >> 10) << 6;  // a multiple of 64 a wave
   const uint32_t w0 = min(w * chunk, c), w1 = min(w0 + chunk, c);
-  // counts first: LDS adds (a round of one offset adds once), no ranks needed yet
+  constexpr uint32_t kHeld = 24;  // rounds a wave holds in registers: bins up to 24,576 survivors in one go
+  if (chunk <= 64u * kHeld) {
+    // every element of the wave's run loaded at once, then both passes from registers
+    uint32_t val[kHeld];
+#pragma unroll
+    for (uint32_t r = 0; r < kHeld; ++r) {
+      const uint32_t pos = w0 + r * 64u + lane;
+      val[r] = pos < w1 ? J.packed[base + pos] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kHeld; ++r) {
+      if (w0 + r * 64u >= w1) break;
+      const bool ok = w0 + r * 64u + lane < w1;
+      const uint64_t valid = __ballot(ok);
+      const uint32_t k = val[r] >> 23;
+      const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
+      if (__ballot(ok && k != k0) == 0) {
+        if (lane == 0) cnt[w][k0] += (uint32_t)__popcll(valid);
+      } else if (ok) {
+        atomicAdd(&cnt[w][k], 1u);
+      }
+    }
+    bin_offsets(cnt, s_w);
+#pragma unroll
+    for (uint32_t r = 0; r < kHeld; ++r) {
+      if (w0 + r * 64u >= w1) break;
+      const bool ok = w0 + r * 64u + lane < w1;
+      const uint64_t valid = __ballot(ok);
+      const uint32_t k = val[r] >> 23;
+      const uint64_t peers = match_bin(k, valid);
+      const uint32_t below = count_below(peers);
+      const uint32_t before = cnt[w][k];
+      if (ok && below == 0) cnt[w][k] = before + (uint32_t)__popcll(peers);
+      const uint32_t rk = before + below;
+      if (ok && rk < c) J.order[base + rk] = val[r] & 0x7FFFFFu;
+    }
+    return;
+  }
+  // a bin too big to hold: the same two passes, reading the run twice
   for (uint32_t p0 = w0; p0 < w1; p0 += 64) {
     const uint32_t pos = p0 + lane;
     const bool ok = pos < w1;
@@ -298,22 +319,7 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const BatchJob *__restrict__ 
       atomicAdd(&cnt[w][k], 1u);
     }
   }
-  __syncthreads();
-  {
-    // digit starts, then each wave's start within its digit
-    uint32_t s = 0;
-    if (tid < 512u)
-      for (int k = 0; k < 16; ++k) s += cnt[k][tid];
-    uint32_t sum;
-    uint32_t run = block_scan_excl_n<16>(s, s_w, &sum);
-    if (tid < 512u)
-      for (int k = 0; k < 16; ++k) {
-        const uint32_t x = cnt[k][tid];
-        cnt[k][tid] = run;
-        run += x;
-      }
-  }
-  __syncthreads();
+  bin_offsets(cnt, s_w);
   for (uint32_t p0 = w0; p0 < w1; p0 += 64) {
     const uint32_t pos = p0 + lane;
     const bool ok = pos < w1;
@@ -361,11 +367,11 @@ int launch_learned(wd_context *ctx, const BatchJob *jobs, uint32_t nv, uint32_t 
   } while (0)
   WD_LAUNCH(kKBinTable, "learned: bin table launch", k_bin_table, dim3(256, nv), dim3(256), 0, ctx->stream, jobs, gate);
   if (std_form)
-    WD_LAUNCH(kKProject, "learned: project launch", k_project_views<true>, dim3(tiles), dim3(256), 0, ctx->stream,
-              jobs, nv);
+    WD_LAUNCH(kKProject, "learned: project launch", k_project_bins<true>, dim3(tiles, nv), dim3(256), 0, ctx->stream,
+              jobs);
   else
-    WD_LAUNCH(kKProject, "learned: project launch", k_project_views<false>, dim3(tiles), dim3(256), 0, ctx->stream,
-              jobs, nv);
+    WD_LAUNCH(kKProject, "learned: project launch", k_project_bins<false>, dim3(tiles, nv), dim3(256), 0, ctx->stream,
+              jobs);
   WD_LAUNCH(kKScan, "learned: scan launch", k_scan_bins, dim3(512 / kScanCols, nv), dim3(256), 0, ctx->stream, jobs);
   WD_LAUNCH(kKBinScatter, "learned: bin scatter launch", k_bin_scatter, dim3(tiles, nv), dim3(256), 0, ctx->stream,
             jobs, gate);

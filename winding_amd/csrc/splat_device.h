@@ -15,6 +15,7 @@ constexpr double kC13 = (double)1.3f;
 constexpr double kC03 = (double)0.3f;
 constexpr double kC2PI = (double)6.2831853f;
 constexpr float kU = 5.9604644775390625e-8f;  // 2^-24, unit roundoff of f32
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float f32_of(uint32_t bits) { return __uint_as_float(bits); }
 
@@ -309,12 +310,16 @@ struct ProjectSmem {
   uint32_t recomp;
 };
 
-template <bool STD>
+// BINS (the learned sort): the histogram is over the 512 bins of `lut`
+// (key -> bin) instead of over the two key bytes. A tile's bin lookups are
+// counted one tile later, so their latency hides behind the next tile's
+// arithmetic.
+template <bool STD, bool BINS = false>
 __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
                                               const ViewK &v, uint16_t *__restrict__ keys,
                                               uint64_t *__restrict__ mask, float4 *__restrict__ cov,
                                               uint16_t *__restrict__ tilehist, uint32_t *__restrict__ recomp_acc,
-                                              uint32_t chunk) {
+                                              uint32_t chunk, const uint16_t *__restrict__ lut = nullptr) {
   const uint32_t tid = threadIdx.x;
   __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 512; k += 256) sm.h[k] = 0;
@@ -336,6 +341,8 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       b = rec[2 * (size_t)i + 1];
     }
   };
+  bool pend = false;     // BINS: a survivor of the previous tile whose bin is in flight
+  uint32_t pend_bin = 0;
   auto work = [&](uint32_t tile, const uint4 &a, const uint4 &b) {
     const uint32_t i = (tile << 8) + tid;
     const bool in = i < n;
@@ -345,10 +352,17 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
     if ((tid & 63u) == 0 && in) mask[i >> 6] = bal;
     if (in) {
       keys[i] = (uint16_t)p.key;
-      cov[i] = p.keep ? make_float4(p.xx, p.xy, p.yy, __uint_as_float(b.z)) : make_float4(0.f, 0.f, 0.f, 0.f);
+      // cov2d is the sort's output, never read back by it: a streaming store keeps it
+      // from displacing the keys and scatter data the next passes read from the caches
+      const f32x4 c = p.keep ? f32x4{p.xx, p.xy, p.yy, __uint_as_float(b.z)} : f32x4{0.f, 0.f, 0.f, 0.f};
+      __builtin_nontemporal_store(c, (f32x4 *)&cov[i]);
     }
     recomp += (p.recomputed && in) ? 1u : 0u;
-    if (p.keep) {
+    if (BINS) {
+      if (pend) atomicAdd(&sm.h[pend_bin & 511u], 1u);
+      pend = p.keep;
+      pend_bin = p.keep ? (uint32_t)lut[p.key] : 0u;
+    } else if (p.keep) {
       atomicAdd(&sm.h[p.key & 255u], 1u);
       atomicAdd(&sm.h[256u + (p.key >> 8)], 1u);
     }
@@ -361,6 +375,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
     load(tile + 2, a0, b0);
     if (tile + 1 < end) work(tile + 1, a1, b1);
   }
+  if (BINS && pend) atomicAdd(&sm.h[pend_bin & 511u], 1u);
   if (recomp) atomicAdd(&sm.recomp, recomp);
   __syncthreads();
   for (uint32_t k = tid; k < 512; k += 256) tilehist[(size_t)chunk * 512 + k] = (uint16_t)sm.h[k];
@@ -566,8 +581,7 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
       const bool ok = (valid >> lane) & 1ull;
       const uint64_t peers = match_digit(dd, valid);
       const uint32_t below = count_below(peers);
-      uint32_t before = 0;
-      if (ok) before = sm.wh[w][dd];
+      const uint32_t before = sm.wh[w][dd];  // any digit is a valid index: read unconditionally
       if (ok && below == 0) sm.wh[w][dd] = before + (uint32_t)__popcll(peers);
       rank[r] = before + below;
       okbits |= (ok ? 1u : 0u) << r;
@@ -583,25 +597,38 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
       sm.wh[3][d] = base + c0 + c1 + c2;
     }
     __syncthreads();
+    // every slot read first, then every write: no read waits behind a write
+    uint32_t pos[KPT];
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) pos[r] = sm.wh[w][HI ? raw[s][r] >> 24 : raw[s][r] & 255u] + rank[r];
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
-      if ((okbits >> r) & 1u) {
-        const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-        const uint32_t dd = HI ? raw[s][r] >> 24 : raw[s][r] & 255u;
-        const uint32_t e = HI ? raw[s][r] : (dd << 24) | ((raw[s][r] >> 8) << kLocalBits) | (i - sbase);
-        const uint32_t pos = sm.wh[w][dd] + rank[r];
-        if (pos < (uint32_t)(S * kTile)) sm.ent[pos] = e;  // always, when the counts agree; a guard
-      }
+      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+      const uint32_t dd = HI ? raw[s][r] >> 24 : raw[s][r] & 255u;
+      const uint32_t e = HI ? raw[s][r] : (dd << 24) | ((raw[s][r] >> 8) << kLocalBits) | (i - sbase);
+      // pos is always in range when the counts agree; the test is a guard
+      if (((okbits >> r) & 1u) && pos[r] < (uint32_t)(S * kTile)) sm.ent[pos[r]] = e;
     }
     __syncthreads();
   }
   const uint32_t nv = min(sm.nvalid, (uint32_t)(S * kTile));
-  for (uint32_t p = tid; p < nv; p += 256) {
-    const uint32_t e = sm.ent[p];
-    const uint32_t g = sm.shift[e >> 24] + p;
-    const uint32_t value =
-        HI ? (e & 0xFFFFFFu) : (((e >> kLocalBits) & 255u) << 24) | (sbase + (e & ((1u << kLocalBits) - 1u)));
-    if (g < n) out[g] = value;  // n bounds both outputs; a guard against a miscount
+  // eight slots a thread at a time, so the two LDS reads of each overlap the others'
+  for (uint32_t p0 = 0; p0 < nv; p0 += 8 * 256) {
+    uint32_t e[8], g[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t p = p0 + (uint32_t)k * 256u + tid;
+      e[k] = p < nv ? sm.ent[p] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) g[k] = sm.shift[e[k] >> 24] + p0 + (uint32_t)k * 256u + tid;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t p = p0 + (uint32_t)k * 256u + tid;
+      const uint32_t value = HI ? (e[k] & 0xFFFFFFu)
+                                : (((e[k] >> kLocalBits) & 255u) << 24) | (sbase + (e[k] & ((1u << kLocalBits) - 1u)));
+      if (p < nv && g[k] < n) out[g[k]] = value;  // n bounds both outputs; a guard against a miscount
+    }
   }
 }
 

@@ -177,6 +177,11 @@ struct wd_batch {
   hipGraphExec_t exec = nullptr;
   uint32_t grid = 0, total = 0;
   int schedule = WD_SCHEDULE_PHASED;
+  void *stage[2] = {nullptr, nullptr};          // pinned staging for the tables
+  hipEvent_t staged[2] = {nullptr, nullptr};    // the copy out of stage[i] has finished
+  bool staged_used[2] = {false, false};
+  int flip = 0;
+  size_t stage_bytes = 0;
   uint32_t magic = 0x42415443;
 };
 
