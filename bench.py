@@ -92,7 +92,7 @@ def bench_batch(args):
     total = sum(sizes)
     batch = wa.Batch(ctx, len(jobs))
     batch.set(jobs)
-    sched, ms_phased, ms_persistent = batch.tune(reps=5)
+    tuning = batch.tune(reps=5)
     for _ in range(args.warmup):
         batch.run()
     ctx.sync()
@@ -125,8 +125,9 @@ def bench_batch(args):
         "config": {"workload": "batch-64", "scenes": len(sizes), "splats": total, "survivors": v,
                    "width": 1280, "height": 720, "cull": args.cull, "size_seed": seed,
                    "parallelism": "single GPU"},
-        "batch": {"schedule": ["phased", "persistent"][sched], "tuned_ms_phased": ms_phased,
-                  "tuned_ms_persistent": ms_persistent, "plain_ms": ms_plain,
+        "batch": {"schedule": ["phased", "persistent"][tuning.schedule], "super_tiles": tuning.super_tiles,
+                  "tuned_ms": {"phased_1": tuning.ms_phased_1, "phased_2": tuning.ms_phased_2,
+                               "persistent": tuning.ms_persistent}, "plain_ms": ms_plain,
                   "plain_value": total / (ms_plain * 1e-3) / 1e6, "speedup_vs_plain": ms_plain / ms},
         "path": {"bytes_per_step": path_bytes, "achieved_gbs": path_bytes / (ms * 1e-3) / 1e9,
                  "frac_of_peak": path_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
@@ -170,7 +171,8 @@ def bench_learned(args):
     groups = [(views[i:i + K], states[:len(views[i:i + K])]) for i in range(0, len(views), K)]
     learned = wa.Learned(ctx, K)
     batch = wa.Batch(ctx, K)
-    batch.schedule(wa.WD_SCHEDULE_PHASED)
+    batch.set([(cloud, v, s) for v, s in zip(views[:K], states)])
+    batch.tune(reps=5)
 
     def orbit_plain():
         for v in views:

@@ -203,19 +203,28 @@ typedef struct wd_job {
  *   WD_SCHEDULE_PERSISTENT  ONE persistent kernel taking every job's work
  *                           tile by tile from a ticketed queue;
  *   WD_SCHEDULE_PHASED      six kernels, each over every job's tiles.
- * A new batch runs phased; wd_batch_tune times both and keeps the faster.
+ * A new batch runs phased with super-tiles of 2; wd_batch_tune times the
+ * configurations and keeps the fastest.
  */
 #define WD_SCHEDULE_PHASED 0
 #define WD_SCHEDULE_PERSISTENT 1
 int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out);
-/* Picks the schedule (re-captures the graph on the next run). */
-int wd_batch_schedule(wd_batch *batch, int schedule);
 /*
- * The autotuner: runs the set jobs `reps` times on each schedule, keeps the
- * faster, and reports it and both mean times (milliseconds a run).
+ * Picks the schedule and the scatter passes' work item: super_tiles tiles of
+ * 4096 a workgroup, 1 or 2 (the persistent schedule always takes 2). The
+ * graph is captured again on the next run.
  */
-int wd_batch_tune(wd_context *ctx, wd_batch *batch, uint32_t reps,
-                  int *schedule, double *ms_phased, double *ms_persistent);
+int wd_batch_schedule(wd_batch *batch, int schedule, uint32_t super_tiles);
+/* The autotuner's report: the configuration it kept, and each one's mean ms a run. */
+typedef struct wd_tuning {
+  int schedule;
+  uint32_t super_tiles;
+  double ms_phased_1;     /* phased, super-tiles of 1 */
+  double ms_phased_2;     /* phased, super-tiles of 2 */
+  double ms_persistent;   /* persistent (super-tiles of 2) */
+} wd_tuning;
+/* Runs the set jobs `reps` times on each configuration and keeps the fastest. */
+int wd_batch_tune(wd_context *ctx, wd_batch *batch, uint32_t reps, wd_tuning *out);
 /*
  * Sets the jobs (views may change between runs at no cost; other clouds or
  * states re-capture the graph on the next run).

@@ -21,8 +21,9 @@ def check_job(state, recs, view, label):
         assert err <= 1e-5, "%s: cov2d error %g" % (label, err)
 
 
-@pytest.mark.parametrize("schedule", [wa.WD_SCHEDULE_PHASED, wa.WD_SCHEDULE_PERSISTENT])
-def test_batch_uneven_jobs_and_replays(ctx, schedule):
+@pytest.mark.parametrize("schedule,super_tiles", [(wa.WD_SCHEDULE_PHASED, 2), (wa.WD_SCHEDULE_PHASED, 1),
+                                                  (wa.WD_SCHEDULE_PERSISTENT, 2)])
+def test_batch_uneven_jobs_and_replays(ctx, schedule, super_tiles):
     sizes = [1, 100, 4096, 5000, 70001, 300000, 12289, 65536]
     st = wl.Stream(11)
     jobs, data = [], []
@@ -37,7 +38,7 @@ def test_batch_uneven_jobs_and_replays(ctx, schedule):
         data.append(recs)
     batch = wa.Batch(ctx, len(jobs))
     try:
-        batch.schedule(schedule)
+        batch.schedule(schedule, super_tiles)
         batch.set(jobs)
         for replay in range(3):
             if replay == 2:  # new views, same clouds and states: the captured graph is replayed
@@ -74,18 +75,18 @@ def test_batch_64_scenes_32m(ctx):
             data.append(recs)
         batch = wa.Batch(ctx, 64)
         batch.set(jobs)
-        sched, ms_phased, ms_persistent = batch.tune(reps=3)
-        print("64 scenes, %d splats: phased %.3f ms, persistent %.3f ms, kept %d"
-              % (sum(sizes), ms_phased, ms_persistent, sched), flush=True)
-        for which in (wa.WD_SCHEDULE_PERSISTENT, wa.WD_SCHEDULE_PHASED):
-            batch.schedule(which)
+        t = batch.tune(reps=3)
+        print("64 scenes, %d splats: phased/1 %.3f ms, phased/2 %.3f ms, persistent %.3f ms, kept %d/%d"
+              % (sum(sizes), t.ms_phased_1, t.ms_phased_2, t.ms_persistent, t.schedule, t.super_tiles), flush=True)
+        for which, sup in ((wa.WD_SCHEDULE_PERSISTENT, 2), (wa.WD_SCHEDULE_PHASED, 1), (wa.WD_SCHEDULE_PHASED, 2)):
+            batch.schedule(which, sup)
             for cloud, view, state in jobs:  # stale results must not pass for new ones
                 state.sort(cloud, wl.view_for(data[0], (0.0, 0.0, 50.0), 64, 64) or view)
             batch.run()
             ctx.sync()
             assert batch.stuck() == 0
             for (cloud, view, state), recs in zip(jobs, data):
-                check_job(state, recs, view, "schedule %d scene n=%d" % (which, len(recs)))
+                check_job(state, recs, view, "schedule %d/%d scene n=%d" % (which, sup, len(recs)))
         batch.destroy()
     finally:
         for cloud, _, state in jobs:

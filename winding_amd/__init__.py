@@ -60,6 +60,12 @@ class Stats(C.Structure):
                 ("overflow_bins", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    """wd_tuning: the autotuner's choice and the mean ms a run of each configuration."""
+    _fields_ = [("schedule", C.c_int), ("super_tiles", C.c_uint32), ("ms_phased_1", C.c_double),
+                ("ms_phased_2", C.c_double), ("ms_persistent", C.c_double)]
+
+
 class WindingError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(message)
@@ -102,8 +108,8 @@ def lib():
             "wd_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(U64), I,
                                     C.POINTER(I)]),
             "wd_batch_create": (I, [P, U32, C.POINTER(P)]),
-            "wd_batch_schedule": (I, [P, I]),
-            "wd_batch_tune": (I, [P, P, U32, C.POINTER(I), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "wd_batch_schedule": (I, [P, I, U32]),
+            "wd_batch_tune": (I, [P, P, U32, C.POINTER(Tuning)]),
             "wd_batch_set": (I, [P, C.POINTER(Job), U32]),
             "wd_batch_run": (I, [P, P]),
             "wd_batch_stuck": (I, [P, P, C.POINTER(U32)]),
@@ -359,14 +365,14 @@ class Batch:
     def run(self):
         check(lib().wd_batch_run(self.ctx.ptr, self.ptr))
 
-    def schedule(self, which):
-        check(lib().wd_batch_schedule(self.ptr, which))
+    def schedule(self, which, super_tiles=2):
+        check(lib().wd_batch_schedule(self.ptr, which, super_tiles))
 
     def tune(self, reps=5):
-        """Times both schedules on the set jobs, keeps the faster: (schedule, ms phased, ms persistent)."""
-        sched, a, b = C.c_int(), C.c_double(), C.c_double()
-        check(lib().wd_batch_tune(self.ctx.ptr, self.ptr, reps, C.byref(sched), C.byref(a), C.byref(b)))
-        return sched.value, a.value, b.value
+        """Times the configurations on the set jobs and keeps the fastest: a Tuning."""
+        t = Tuning()
+        check(lib().wd_batch_tune(self.ctx.ptr, self.ptr, reps, C.byref(t)))
+        return t
 
     def stuck(self):
         s = C.c_uint32()

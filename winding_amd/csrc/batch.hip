@@ -87,6 +87,7 @@ __device__ __forceinline__ void item_scan2(const BatchJob &J, uint32_t cg, uint3
 }
 
 constexpr uint32_t kScanItems = 512 / kScanCols, kScan2Items = 256 / kScanCols;
+static_assert(kScanItems % kXcds == 0 && kScan2Items % kXcds == 0, "scan items a multiple of the XCDs");
 
 // gate: when given, the whole launch is skipped unless *gate is nonzero (the
 // learned sort's fallback: it runs, for every job, when any view failed its
@@ -178,24 +179,26 @@ __global__ __launch_bounds__(256) void kb_project(const BatchJob *__restrict__ j
   item_project(sm, jobs[j], blockIdx.x - starts[j]);
 }
 
+// (kScanItems and kScan2Items are multiples of 8, so workgroup b's XCD is
+// b mod 8 within every job too: the column groups are made XCD-contiguous.)
 __global__ __launch_bounds__(256) void kb_scan(const BatchJob *__restrict__ jobs) {
   __shared__ uint32_t part[kScanCols][64];
-  item_scan(jobs[blockIdx.x / kScanItems], blockIdx.x % kScanItems, part);
+  item_scan(jobs[blockIdx.x / kScanItems], xcd_item(blockIdx.x % kScanItems, kScanItems), part);
 }
 
-template <bool HI>
+template <bool HI, int S>
 __global__ __launch_bounds__(256) void kb_scatter(const BatchJob *__restrict__ jobs, const uint32_t *__restrict__ starts,
                                                   uint32_t nj) {
-  __shared__ ScatterSmem<kSuper> sm;
+  __shared__ ScatterSmem<S> sm;
   const uint32_t *u_start = starts + (nj + 1);
   const uint32_t item = xcd_item(blockIdx.x, gridDim.x), j = find_job(u_start, nj, item);
   const BatchJob &J = jobs[j];
   if (HI)
-    scatter_super<true, kSuper>(sm, item - u_start[j], nullptr, nullptr, J.packed, J.n, J.tilehist3, 256,
-                                J.prefix_hi, J.ctl->total + 256, J.ctl, J.order);
+    scatter_super<true, S>(sm, item - u_start[j], nullptr, nullptr, J.packed, J.n, J.tilehist3, 256, J.prefix_hi,
+                           J.ctl->total + 256, J.ctl, J.order);
   else
-    scatter_super<false, kSuper>(sm, item - u_start[j], J.keys, J.mask, nullptr, J.n, J.tilehist, 512,
-                                 J.prefix_lo, J.ctl->total, J.ctl, J.packed);
+    scatter_super<false, S>(sm, item - u_start[j], J.keys, J.mask, nullptr, J.n, J.tilehist, 512, J.prefix_lo,
+                            J.ctl->total, J.ctl, J.packed);
 }
 
 __global__ __launch_bounds__(256) void kb_hist(const BatchJob *__restrict__ jobs, const uint32_t *__restrict__ starts,
@@ -208,7 +211,7 @@ __global__ __launch_bounds__(256) void kb_hist(const BatchJob *__restrict__ jobs
 
 __global__ __launch_bounds__(256) void kb_scan2(const BatchJob *__restrict__ jobs) {
   __shared__ uint32_t part[kScanCols][64];
-  item_scan2(jobs[blockIdx.x / kScan2Items], blockIdx.x % kScan2Items, part);
+  item_scan2(jobs[blockIdx.x / kScan2Items], xcd_item(blockIdx.x % kScan2Items, kScan2Items), part);
 }
 
 }  // namespace
@@ -238,10 +241,16 @@ int launch_batch(wd_context *ctx, wd_batch *b) {
   if (ctx->prof_begin(kKBatch, &ev)) return WD_E_HIP;
   hipLaunchKernelGGL(kb_project, dim3(nt), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   hipLaunchKernelGGL(kb_scan, dim3(kScanItems * nj), dim3(256), 0, ctx->stream, b->jobs);
-  hipLaunchKernelGGL(kb_scatter<false>, dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+  if (b->super_tiles == 1)
+    hipLaunchKernelGGL((kb_scatter<false, 1>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+  else
+    hipLaunchKernelGGL((kb_scatter<false, kSuper>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   hipLaunchKernelGGL(kb_hist, dim3(nt), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   hipLaunchKernelGGL(kb_scan2, dim3(kScan2Items * nj), dim3(256), 0, ctx->stream, b->jobs);
-  hipLaunchKernelGGL(kb_scatter<true>, dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+  if (b->super_tiles == 1)
+    hipLaunchKernelGGL((kb_scatter<true, 1>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+  else
+    hipLaunchKernelGGL((kb_scatter<true, kSuper>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   WD_HIP(hipGetLastError(), "batch: launch");
   return ctx->prof_end(kKBatch, ev);
 }

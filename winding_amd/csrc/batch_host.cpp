@@ -76,7 +76,8 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     B.prefix_hi = s->prefix_hi;
     B.tiles = (B.n + wd::kTile - 1) / wd::kTile;
     t[i + 1] = t[i] + B.tiles;
-    u[i + 1] = u[i] + (B.tiles + wd::kSuper - 1) / wd::kSuper;
+    const uint32_t S = b->schedule == WD_SCHEDULE_PERSISTENT ? (uint32_t)wd::kSuper : b->super_tiles;
+    u[i + 1] = u[i] + (B.tiles + S - 1) / S;
     s->last_n = B.n;
     shape.push_back(J.cloud);
     shape.push_back(J.state);
@@ -111,6 +112,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   b->host_jobs.swap(hj);
   b->host_starts.swap(starts);
   b->shape.swap(shape);
+  if (b->api_jobs.data() != jobs) b->api_jobs.assign(jobs, jobs + n);
   b->n = n;
   b->total = total;
   return same_total && b->grid ? WD_OK : wd::batch_grid(ctx, total, &b->grid);
@@ -140,33 +142,38 @@ int wd_batch_run(wd_context *ctx, wd_batch *b) {
   return WD_OK;
 }
 
-int wd_batch_schedule(wd_batch *b, int schedule) {
+int wd_batch_schedule(wd_batch *b, int schedule, uint32_t super_tiles) {
   if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
   if (schedule != WD_SCHEDULE_PHASED && schedule != WD_SCHEDULE_PERSISTENT)
     return fail(WD_E_ARG, "batch: schedule %d is neither phased (0) nor persistent (1)", schedule);
-  if (schedule != b->schedule) {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
-    b->schedule = schedule;
-  }
-  return WD_OK;
+  if (super_tiles != 1 && super_tiles != 2) return fail(WD_E_ARG, "batch: super-tiles of %u; 1 or 2", super_tiles);
+  if (schedule == b->schedule && super_tiles == b->super_tiles) return WD_OK;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->schedule = schedule;
+  b->super_tiles = super_tiles;
+  if (b->n == 0) return WD_OK;
+  // the item starts depend on the super-tile size: set the same jobs again
+  const std::vector<wd_job> jobs = b->api_jobs;
+  return wd_batch_set(b, jobs.data(), (uint32_t)jobs.size());
 }
 
-int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, int *schedule, double *ms_phased,
-                  double *ms_persistent) {
+int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, wd_tuning *out) {
   if (!ctx || !b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
   if (b->n == 0) return fail(WD_E_STATE, "batch: no jobs set");
   if (ctx->profiling) return fail(WD_E_STATE, "batch: tuning times whole runs; switch kernel profiling off first");
   reps = std::max(reps, 1u);
-  double ms[2] = {0, 0};
+  const int sched[3] = {WD_SCHEDULE_PHASED, WD_SCHEDULE_PHASED, WD_SCHEDULE_PERSISTENT};
+  const uint32_t sup[3] = {1, 2, 2};
+  double ms[3] = {0, 0, 0};
   hipEvent_t a = nullptr, z = nullptr;
   WD_HIP(hipEventCreate(&a), "batch: tune events");
   hipError_t e = hipEventCreate(&z);
   int rc = WD_OK;
-  for (int sched = 0; sched < 2 && e == hipSuccess && rc == WD_OK; ++sched) {
-    rc = wd_batch_schedule(b, sched);
+  for (int c = 0; c < 3 && e == hipSuccess && rc == WD_OK; ++c) {
+    rc = wd_batch_schedule(b, sched[c], sup[c]);
     if (!rc) rc = wd_batch_run(ctx, b);  // capture and warm
     if (!rc) e = hipEventRecord(a, ctx->stream);
     for (uint32_t r = 0; r < reps && !rc && e == hipSuccess; ++r) rc = wd_batch_run(ctx, b);
@@ -174,17 +181,23 @@ int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, int *schedule, do
     if (!rc && e == hipSuccess) e = hipEventSynchronize(z);
     float t = 0.f;
     if (!rc && e == hipSuccess) e = hipEventElapsedTime(&t, a, z);
-    ms[sched] = (double)t / reps;
+    ms[c] = (double)t / reps;
   }
   hipEventDestroy(a);
   if (z) hipEventDestroy(z);
   if (rc) return rc;
   if (e != hipSuccess) return wd::hip_fail(e, "batch: tune");
-  const int best = ms[1] < ms[0] ? WD_SCHEDULE_PERSISTENT : WD_SCHEDULE_PHASED;
-  if ((rc = wd_batch_schedule(b, best))) return rc;
-  if (schedule) *schedule = best;
-  if (ms_phased) *ms_phased = ms[0];
-  if (ms_persistent) *ms_persistent = ms[1];
+  int best = 0;
+  for (int c = 1; c < 3; ++c)
+    if (ms[c] < ms[best]) best = c;
+  if ((rc = wd_batch_schedule(b, sched[best], sup[best]))) return rc;
+  if (out) {
+    out->schedule = sched[best];
+    out->super_tiles = sup[best];
+    out->ms_phased_1 = ms[0];
+    out->ms_phased_2 = ms[1];
+    out->ms_persistent = ms[2];
+  }
   return WD_OK;
 }
 

@@ -103,7 +103,8 @@ __global__ __launch_bounds__(256) void k_project_bins(const BatchJob *__restrict
 __global__ __launch_bounds__(256) void k_scan_bins(const BatchJob *__restrict__ jobs) {
   __shared__ uint32_t s_part[kScanCols][64];
   const BatchJob &J = jobs[blockIdx.y];
-  scan_cols(blockIdx.x * kScanCols, J.tiles, J.tilehist, 512, J.prefix_bin, 512, 512, J.ctl->total, s_part);
+  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, J.tiles, J.tilehist, 512, J.prefix_bin, 512, 512,
+            J.ctl->total, s_part);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     J.ctl->recomputed = J.ctl->recomputed_acc;
     J.ctl->recomputed_acc = 0;

@@ -91,6 +91,26 @@ __device__ __forceinline__ uint64_t match_bits(uint32_t d, uint64_t valid) {
 
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) { return match_bits<8>(d, valid); }
 
+// Adds a wave's digits (lanes in `valid`) to a row of LDS counts (shared or
+// wave-private). Skewed depths put most of a wave on a few digits, and
+// same-address LDS atomics serialise: the lowest remaining lane's digit is
+// peeled off with one ballot and one add, up to three times, before the rest
+// go as single atomics.
+__device__ __forceinline__ void count_digit(uint32_t *row, uint32_t d, uint64_t valid) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t rem = valid;
+#pragma unroll
+  for (int it = 0; it < 3; ++it) {
+    if (rem == 0) break;
+    const uint32_t first = (uint32_t)__builtin_ctzll(rem);
+    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)first);
+    const uint64_t eq = __ballot(d == d0) & rem;
+    if (lane == first) atomicAdd(&row[d0], (uint32_t)__popcll(eq));
+    rem &= ~eq;
+  }
+  if ((rem >> lane) & 1ull) atomicAdd(&row[d], 1u);
+}
+
 // ---------------------------------------------------------------------------
 // The FP64 definition, for the rare splat the f32 bound cannot decide.
 // put() (src/scene/splats.js:350-368), footprint() (src/render/splats.js:
@@ -439,8 +459,8 @@ __device__ __forceinline__ uint32_t digit_bases(uint32_t total_d, uint32_t *s_w,
 // The high-byte histogram of tile t of the low-byte pass's output (positions
 // t * kTile ...), the upsweep of the high-byte pass: counts only, so plain
 // LDS adds, each wave into its own row (a slab seen edge on, whose survivors
-// share a few high bytes, then contends within a wave only). A round whose
-// 64 digits are all one digit adds once.
+// share a few high bytes, then contends within a wave only), hot digits
+// peeled off first (count_digit).
 struct HistSmem {
   uint32_t h[4][256];
 };
@@ -451,28 +471,28 @@ __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t limit = ctl->count, tile0 = t * (uint32_t)kTile;
   if (tile0 >= limit) return;
+  // Counting needs no order: thread tid takes 16 consecutive elements, in four 16-byte loads.
   uint32_t raw[KPT];
-  const uint32_t wbase = tile0 + w * (64u * KPT);
+  const uint32_t first = tile0 + tid * (uint32_t)KPT;
+  if (first + KPT <= limit) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(packed + first);
 #pragma unroll
-  for (int r = 0; r < KPT; ++r) {
-    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-    raw[r] = i < limit ? packed[i] : 0u;
+    for (int k = 0; k < KPT / 4; ++k) {
+      const uint4 v = q[k];
+      raw[4 * k] = v.x;
+      raw[4 * k + 1] = v.y;
+      raw[4 * k + 2] = v.z;
+      raw[4 * k + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) raw[r] = first + r < limit ? packed[first + r] : 0u;
   }
   __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 1024; k += 256) (&sm.h[0][0])[k] = 0;
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < KPT; ++r) {
-    const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-    const uint32_t d = raw[r] >> 24;
-    const uint64_t valid = __ballot(i < limit);
-    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
-    if (__ballot(d != d0) == 0) {
-      if (lane == 0) sm.h[w][d0] += (uint32_t)__popcll(valid);
-    } else if (i < limit) {
-      atomicAdd(&sm.h[w][d], 1u);
-    }
-  }
+  for (int r = 0; r < KPT; ++r) count_digit(sm.h[w], raw[r] >> 24, __ballot(first + r < limit));
   __syncthreads();
   tilehist3[(size_t)t * 256 + tid] = (uint16_t)(sm.h[0][tid] + sm.h[1][tid] + sm.h[2][tid] + sm.h[3][tid]);
 }

@@ -52,7 +52,9 @@ __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, 
 __global__ __launch_bounds__(256) void k_scan(uint32_t T, const uint16_t *__restrict__ tilehist,
                                               uint32_t *__restrict__ prefix_lo, Ctl *__restrict__ ctl) {
   __shared__ uint32_t s_part[kScanCols][64];
-  scan_cols(blockIdx.x * kScanCols, T, tilehist, 512, prefix_lo, 256, 256, ctl->total, s_part);
+  // XCD-contiguous column groups: the 4-column pieces of one prefix row's cache
+  // lines are written from one L2
+  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T, tilehist, 512, prefix_lo, 256, 256, ctl->total, s_part);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->recomputed = ctl->recomputed_acc;
     ctl->recomputed_acc = 0;
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(256) void k_scan2(const uint16_t *__restrict__ tile
                                                Ctl *__restrict__ ctl) {
   __shared__ uint32_t s_part[kScanCols][64];
   const uint32_t T3 = (ctl->count + kTile - 1) / kTile;
-  scan_cols(blockIdx.x * kScanCols, T3, tilehist3, 256, prefix_hi, 256, 256, ctl->total3, s_part);
+  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T3, tilehist3, 256, prefix_hi, 256, 256, ctl->total3, s_part);
 }
 
 // ---------------------------------------------------------------------------

@@ -177,6 +177,8 @@ struct wd_batch {
   hipGraphExec_t exec = nullptr;
   uint32_t grid = 0, total = 0;
   int schedule = WD_SCHEDULE_PHASED;
+  uint32_t super_tiles = 2;                     // the scatter work item: tiles a workgroup (1 or 2)
+  std::vector<wd_job> api_jobs;                 // the jobs as set, to rebuild the tables on a change
   void *stage[2] = {nullptr, nullptr};          // pinned staging for the tables
   hipEvent_t staged[2] = {nullptr, nullptr};    // the copy out of stage[i] has finished
   bool staged_used[2] = {false, false};

@@ -483,7 +483,7 @@ static napi_value js_batch_run(napi_env env, napi_callback_info info) {
   return rc ? throw_wd(env, rc) : NULL;
 }
 
-/* batchTune(context, batch, reps) -> {schedule: 'phased'|'persistent', msPhased, msPersistent} */
+/* batchTune(context, batch, reps) -> {schedule: 'phased'|'persistent', superTiles, msPhased1, msPhased2, msPersistent} */
 static napi_value js_batch_tune(napi_env env, napi_callback_info info) {
   napi_value argv[3], out, sv;
   uint32_t reps;
@@ -492,15 +492,16 @@ static napi_value js_batch_tune(napi_env env, napi_callback_info info) {
   if (!c) return NULL;
   Box *b = get_box(env, argv[1], K_BATCH);
   if (!b || !u32_arg(env, argv[2], &reps)) return NULL;
-  int sched = 0;
-  double a = 0, p = 0;
-  int rc = wd_batch_tune((wd_context *)c->ptr, (wd_batch *)b->ptr, reps, &sched, &a, &p);
+  wd_tuning t;
+  int rc = wd_batch_tune((wd_context *)c->ptr, (wd_batch *)b->ptr, reps, &t);
   if (rc) return throw_wd(env, rc);
   CHECK_NAPI(napi_create_object(env, &out));
-  napi_create_string_utf8(env, sched == WD_SCHEDULE_PERSISTENT ? "persistent" : "phased", NAPI_AUTO_LENGTH, &sv);
+  napi_create_string_utf8(env, t.schedule == WD_SCHEDULE_PERSISTENT ? "persistent" : "phased", NAPI_AUTO_LENGTH, &sv);
   napi_set_named_property(env, out, "schedule", sv);
-  napi_set_named_property(env, out, "msPhased", num(env, a));
-  napi_set_named_property(env, out, "msPersistent", num(env, p));
+  napi_set_named_property(env, out, "superTiles", num(env, t.super_tiles));
+  napi_set_named_property(env, out, "msPhased1", num(env, t.ms_phased_1));
+  napi_set_named_property(env, out, "msPhased2", num(env, t.ms_phased_2));
+  napi_set_named_property(env, out, "msPersistent", num(env, t.ms_persistent));
   return out;
 }
 
