@@ -172,6 +172,7 @@ int wd_state_stats(wd_context *ctx, wd_state *state, wd_stats *stats);
 /*
  * Copies the 16-bit keys of every splat and the survivor bitmask (bit i of
  * word i/64) of the last wd_sort: the parity tests compare them per splat.
+ * A learned run keeps no keys: after one, asking for keys is WD_E_STATE.
  */
 int wd_state_read_keys(wd_context *ctx, wd_state *state, uint16_t *keys,
                        uint64_t *mask, uint64_t n);
@@ -203,7 +204,7 @@ typedef struct wd_job {
  *   WD_SCHEDULE_PERSISTENT  ONE persistent kernel taking every job's work
  *                           tile by tile from a ticketed queue;
  *   WD_SCHEDULE_PHASED      six kernels, each over every job's tiles.
- * A new batch runs phased with super-tiles of 2; wd_batch_tune times the
+ * A new batch runs phased with super-tiles of 1; wd_batch_tune times the
  * configurations and keeps the fastest.
  */
 #define WD_SCHEDULE_PHASED 0
@@ -211,7 +212,7 @@ typedef struct wd_job {
 int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out);
 /*
  * Picks the schedule and the scatter passes' work item: super_tiles tiles of
- * 4096 a workgroup, 1 or 2 (the persistent schedule always takes 2). The
+ * 4096 a workgroup, 1 or 2 (the persistent schedule always takes 1). The
  * graph is captured again on the next run.
  */
 int wd_batch_schedule(wd_batch *batch, int schedule, uint32_t super_tiles);
@@ -221,7 +222,7 @@ typedef struct wd_tuning {
   uint32_t super_tiles;
   double ms_phased_1;     /* phased, super-tiles of 1 */
   double ms_phased_2;     /* phased, super-tiles of 2 */
-  double ms_persistent;   /* persistent (super-tiles of 2) */
+  double ms_persistent;   /* persistent (super-tiles of 1) */
 } wd_tuning;
 /* Runs the set jobs `reps` times on each configuration and keeps the fastest. */
 int wd_batch_tune(wd_context *ctx, wd_batch *batch, uint32_t reps, wd_tuning *out);
@@ -292,7 +293,8 @@ int wd_learned_destroy(wd_learned *learned);
 /*
  * Training targets from an exact sort in `state`: edges[k] = the key at rank
  * floor(k V / 256) (k = 1..255), edges[0] = 0, edges[256] = 65536. The
- * predictor's target widths are (edges[k+1] - edges[k]) / 65536.
+ * predictor's target widths are (edges[k+1] - edges[k]) / 65536. A state
+ * whose last sort was a learned run keeps no keys: WD_E_STATE.
  */
 int wd_state_quantiles(wd_context *ctx, wd_state *state, uint32_t edges[257]);
 

@@ -309,7 +309,7 @@ int wd_state_destroy(wd_state *s) {
   hipStreamSynchronize(s->ctx->stream);
   for (void *p : {(void *)s->ctl, (void *)s->keys, (void *)s->mask, (void *)s->packed, (void *)s->order,
                   (void *)s->cov, (void *)s->tilehist, (void *)s->tilehist3, (void *)s->prefix_lo,
-                  (void *)s->prefix_hi, (void *)s->lut, (void *)s->binlo, (void *)s->prefix_bin})
+                  (void *)s->prefix_hi, (void *)s->lut, (void *)s->prefix_bin})
     if (p) hipFree(p);
   s->ctx->live.erase(s);
   s->magic = 0;
@@ -386,6 +386,7 @@ int wd_state_read_keys(wd_context *ctx, wd_state *s, uint16_t *keys, uint64_t *m
   if (!ctx || !s || s->magic != 0x53544154) return fail(WD_E_ARG, "read keys: null argument or destroyed state");
   if (n > s->last_n) return fail(WD_E_ARG, "read keys: %llu asked, last sort had %llu", (unsigned long long)n,
                                  (unsigned long long)s->last_n);
+  if (keys && !s->keys_kept) return fail(WD_E_STATE, "read keys: the last sort of this state was a learned run, which keeps no keys");
   WD_HIP(hipStreamSynchronize(ctx->stream), "read keys: sync");
   if (keys) WD_HIP(hipMemcpy(keys, s->keys, (size_t)n * 2, hipMemcpyDeviceToHost), "read keys: keys");
   if (mask) WD_HIP(hipMemcpy(mask, s->mask, (size_t)((n + 63) / 64) * 8, hipMemcpyDeviceToHost), "read keys: mask");

@@ -73,17 +73,17 @@ __device__ __forceinline__ void item_project(ProjectSmem &sm, const BatchJob &J,
                          &J.ctl->recomputed_acc, t);
 }
 
-__device__ __forceinline__ void item_scan(const BatchJob &J, uint32_t cg, uint32_t (*s_part)[64]) {
-  scan_cols(cg * kScanCols, J.tiles, J.tilehist, 512, J.prefix_lo, 256, 256, J.ctl->total, s_part);
+__device__ __forceinline__ void item_scan(const BatchJob &J, uint32_t cg, uint32_t (*s_w)[kScanCols]) {
+  scan_cols(cg * kScanCols, J.tiles, J.tilehist, 512, J.prefix_lo, 256, 256, J.ctl->total, s_w);
   if (cg == 0 && threadIdx.x == 0) {
     J.ctl->recomputed = J.ctl->recomputed_acc;
     J.ctl->recomputed_acc = 0;
   }
 }
 
-__device__ __forceinline__ void item_scan2(const BatchJob &J, uint32_t cg, uint32_t (*s_part)[64]) {
+__device__ __forceinline__ void item_scan2(const BatchJob &J, uint32_t cg, uint32_t (*s_w)[kScanCols]) {
   const uint32_t T3 = (J.ctl->count + kTile - 1) / kTile;
-  scan_cols(cg * kScanCols, T3, J.tilehist3, 256, J.prefix_hi, 256, 256, J.ctl->total3, s_part);
+  scan_cols(cg * kScanCols, T3, J.tilehist3, 256, J.prefix_hi, 256, 256, J.ctl->total3, s_w);
 }
 
 constexpr uint32_t kScanItems = 512 / kScanCols, kScan2Items = 256 / kScanCols;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
     ProjectSmem p;
     HistSmem h;
     ScatterSmem<kSuper> s;
-    uint32_t part[kScanCols][64];
+    uint32_t part[4][kScanCols];
   } sm;
   __shared__ uint32_t s_item;
   uint32_t *proj_done = reinterpret_cast<uint32_t *>(bc + 1);
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void kb_project(const BatchJob *__restrict__ j
 // (kScanItems and kScan2Items are multiples of 8, so workgroup b's XCD is
 // b mod 8 within every job too: the column groups are made XCD-contiguous.)
 __global__ __launch_bounds__(256) void kb_scan(const BatchJob *__restrict__ jobs) {
-  __shared__ uint32_t part[kScanCols][64];
+  __shared__ uint32_t part[4][kScanCols];
   item_scan(jobs[blockIdx.x / kScanItems], xcd_item(blockIdx.x % kScanItems, kScanItems), part);
 }
 
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void kb_hist(const BatchJob *__restrict__ jobs
 }
 
 __global__ __launch_bounds__(256) void kb_scan2(const BatchJob *__restrict__ jobs) {
-  __shared__ uint32_t part[kScanCols][64];
+  __shared__ uint32_t part[4][kScanCols];
   item_scan2(jobs[blockIdx.x / kScan2Items], xcd_item(blockIdx.x % kScan2Items, kScan2Items), part);
 }
 
@@ -244,13 +244,13 @@ int launch_batch(wd_context *ctx, wd_batch *b) {
   if (b->super_tiles == 1)
     hipLaunchKernelGGL((kb_scatter<false, 1>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   else
-    hipLaunchKernelGGL((kb_scatter<false, kSuper>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+    hipLaunchKernelGGL((kb_scatter<false, 2>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   hipLaunchKernelGGL(kb_hist, dim3(nt), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   hipLaunchKernelGGL(kb_scan2, dim3(kScan2Items * nj), dim3(256), 0, ctx->stream, b->jobs);
   if (b->super_tiles == 1)
     hipLaunchKernelGGL((kb_scatter<true, 1>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   else
-    hipLaunchKernelGGL((kb_scatter<true, kSuper>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
+    hipLaunchKernelGGL((kb_scatter<true, 2>), dim3(nu), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);
   WD_HIP(hipGetLastError(), "batch: launch");
   return ctx->prof_end(kKBatch, ev);
 }

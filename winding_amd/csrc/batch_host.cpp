@@ -79,6 +79,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     const uint32_t S = b->schedule == WD_SCHEDULE_PERSISTENT ? (uint32_t)wd::kSuper : b->super_tiles;
     u[i + 1] = u[i] + (B.tiles + S - 1) / S;
     s->last_n = B.n;
+    s->keys_kept = true;
     shape.push_back(J.cloud);
     shape.push_back(J.state);
   }
@@ -166,7 +167,7 @@ int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, wd_tuning *out) {
   if (ctx->profiling) return fail(WD_E_STATE, "batch: tuning times whole runs; switch kernel profiling off first");
   reps = std::max(reps, 1u);
   const int sched[3] = {WD_SCHEDULE_PHASED, WD_SCHEDULE_PHASED, WD_SCHEDULE_PERSISTENT};
-  const uint32_t sup[3] = {1, 2, 2};
+  const uint32_t sup[3] = {1, 2, (uint32_t)wd::kSuper};
   double ms[3] = {0, 0, 0};
   hipEvent_t a = nullptr, z = nullptr;
   WD_HIP(hipEventCreate(&a), "batch: tune events");

@@ -6,10 +6,11 @@
 // views of one cloud:
 //   k_bin_table      the widths as bin edges on the 16-bit key line; a bin
 //                    wider than kBinWidth keys is split, so a key's offset in
-//                    its bin fits 9 bits. Writes key -> bin (65536 u16).
-//   k_project_bins   pass 1 of every view: the same keys, survivor bits and
-//                    cov2d as k_project, and the chunk's histogram over bins
-//                    instead of over key bytes.
+//                    its bin fits 9 bits. Writes key -> bin << 9 | offset.
+//   k_project_bins   pass 1 of every view: the same survivor bits and cov2d
+//                    as k_project; in place of the key, the key's table entry
+//                    (parked in the view's order buffer until the last pass
+//                    writes it); the chunk's histogram over bins.
 //   (k_scan_bins)    column prefixes and totals of the bin histograms.
 //   k_bin_scatter    the check: every bin's total is at most the cap, or this
 //                    view falls back. Then one stable pass by bin: the
@@ -79,9 +80,8 @@ __global__ __launch_bounds__(256) void k_bin_table(const BatchJob *__restrict__ 
     const uint32_t mid = (lo + hi) >> 1;
     if (B[mid] <= key) lo = mid; else hi = mid;
   }
-  J.lut[key] = (uint16_t)(F[lo] + (key - B[lo]) / kBinWidth);
-  if (blockIdx.x == 0)
-    for (uint32_t j = 0; j < parts; ++j) J.binlo[first + j] = (uint16_t)(B[t] + j * kBinWidth);
+  const uint32_t rel = key - B[lo];
+  J.lut[key] = ((F[lo] + rel / kBinWidth) << 9) | (rel % kBinWidth);
   if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) *gate = 0;  // no view has fallen back yet this run
 }
 
@@ -96,15 +96,15 @@ __global__ __launch_bounds__(256) void k_project_bins(const BatchJob *__restrict
   __shared__ ProjectSmem sm;
   const BatchJob &J = jobs[blockIdx.y];
   project_chunk<STD, true>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
-                           &J.ctl->recomputed_acc, blockIdx.x, J.lut);
+                           &J.ctl->recomputed_acc, blockIdx.x, J.lut, J.order);
 }
 
 // Column prefixes and totals of the bin histograms. Grid (512 / kScanCols, views).
 __global__ __launch_bounds__(256) void k_scan_bins(const BatchJob *__restrict__ jobs) {
-  __shared__ uint32_t s_part[kScanCols][64];
+  __shared__ uint32_t s_w[4][kScanCols];
   const BatchJob &J = jobs[blockIdx.y];
   scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, J.tiles, J.tilehist, 512, J.prefix_bin, 512, 512,
-            J.ctl->total, s_part);
+            J.ctl->total, s_w);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     J.ctl->recomputed = J.ctl->recomputed_acc;
     J.ctl->recomputed_acc = 0;
@@ -119,7 +119,6 @@ struct BinScatterSmem {
   uint32_t wh[4][512];   // per-wave running bin counts, then wave offsets
   uint32_t shift[512];   // global start - LDS start of each bin
   uint32_t lstart[512];
-  uint16_t binlo[512];
   uint32_t w[4];
 };
 
@@ -134,15 +133,13 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict_
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-    raw[r] = i < n ? (uint32_t)J.keys[i] : 0u;
+    raw[r] = i < n ? J.order[i] : 0u;  // bin << 9 | offset, left there by the projection
   }
   const uint32_t wi = (wbase >> 6) + lane;
   const uint64_t vm = (lane < (uint32_t)KPT && wi < nwords) ? J.mask[wi] : 0ull;
   const uint32_t tot0 = J.ctl->total[d0], tot1 = J.ctl->total[d1];
   const uint32_t cnt0 = J.tilehist[(size_t)t * 512 + d0], cnt1 = J.tilehist[(size_t)t * 512 + d1];
   const uint32_t pre0 = J.prefix_bin[(size_t)t * 512 + d0], pre1 = J.prefix_bin[(size_t)t * 512 + d1];
-  sm.binlo[d0] = J.binlo[d0];
-  sm.binlo[d1] = J.binlo[d1];
   // The check, made by every workgroup from the same totals.
   const int over = __syncthreads_count(tot0 > J.cap) + __syncthreads_count(tot1 > J.cap);
   uint32_t T0, T1;
@@ -165,7 +162,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict_
   sm.shift[d1] = b1 + pre1 - l1;
   uint32_t bin[KPT];
 #pragma unroll
-  for (int r = 0; r < KPT; ++r) bin[r] = J.lut[raw[r]];
+  for (int r = 0; r < KPT; ++r) bin[r] = raw[r] >> 9;
   for (uint32_t k = tid; k < 2048; k += 256) (&sm.wh[0][0])[k] = 0;
   __syncthreads();
   uint32_t rank[KPT];
@@ -196,17 +193,14 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict_
     sm.wh[3][d] = base + c0 + c1 + c2;
   }
   __syncthreads();
-  uint32_t pos[KPT], lo[KPT];
+  uint32_t pos[KPT];
 #pragma unroll
-  for (int r = 0; r < KPT; ++r) {
-    pos[r] = sm.wh[w][bin[r] & 511u] + rank[r];
-    lo[r] = sm.binlo[bin[r] & 511u];
-  }
+  for (int r = 0; r < KPT; ++r) pos[r] = sm.wh[w][bin[r] & 511u] + rank[r];
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t i = wbase + (uint32_t)r * 64u + lane;
     const uint32_t dd = bin[r] & 511u;
-    const uint32_t off = (raw[r] - lo[r]) & 511u;  // below kBinWidth by the table's construction
+    const uint32_t off = raw[r] & 511u;
     if (((okbits >> r) & 1u) && pos[r] < (uint32_t)kTile) sm.ent[pos[r]] = (dd << 23) | (off << 14) | (i - tile0);
   }
   __syncthreads();

@@ -74,42 +74,29 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t G) {
 // Lanes of this wave whose BITS-bit digit equals this lane's, among `valid`:
 // for each bit, the ballot of the wave XOR this lane's bit spread over 32
 // bits (0 or ~0) marks the lanes that differ, and those are cleared. Kept in
-// two 32-bit halves so each bit costs a handful of VALU operations (the
-// compiler merges the clears into 3-input ORs), not a 64-bit select.
+// two 32-bit halves; each half's "differs |= ballot ^ spread" is one 3-input
+// bitwise operation (v_bitop3, truth table 0xF6 over (acc, ballot, spread)),
+// and the ballot is taken of the spread bit itself, so a bit costs four VALU
+// operations.
 template <int BITS>
 __device__ __forceinline__ uint64_t match_bits(uint32_t d, uint64_t valid) {
   uint32_t dlo = 0, dhi = 0;  // lanes differing in some bit
 #pragma unroll
   for (int b = 0; b < BITS; ++b) {
-    const uint32_t s = (uint32_t)(((int)(d << (31 - b))) >> 31);
+    const uint32_t s = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
     const uint64_t bal = __ballot(s != 0u);
-    dlo |= (uint32_t)bal ^ s;
-    dhi |= (uint32_t)(bal >> 32) ^ s;
+    if (b == 0) {
+      dlo = (uint32_t)bal ^ s;
+      dhi = (uint32_t)(bal >> 32) ^ s;
+    } else {
+      dlo = __builtin_amdgcn_bitop3_b32(dlo, (uint32_t)bal, s, 0xF6);
+      dhi = __builtin_amdgcn_bitop3_b32(dhi, (uint32_t)(bal >> 32), s, 0xF6);
+    }
   }
   return ((uint64_t)(~dhi & (uint32_t)(valid >> 32)) << 32) | (~dlo & (uint32_t)valid);
 }
 
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) { return match_bits<8>(d, valid); }
-
-// Adds a wave's digits (lanes in `valid`) to a row of LDS counts (shared or
-// wave-private). Skewed depths put most of a wave on a few digits, and
-// same-address LDS atomics serialise: the lowest remaining lane's digit is
-// peeled off with one ballot and one add, up to three times, before the rest
-// go as single atomics.
-__device__ __forceinline__ void count_digit(uint32_t *row, uint32_t d, uint64_t valid) {
-  const uint32_t lane = threadIdx.x & 63u;
-  uint64_t rem = valid;
-#pragma unroll
-  for (int it = 0; it < 3; ++it) {
-    if (rem == 0) break;
-    const uint32_t first = (uint32_t)__builtin_ctzll(rem);
-    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)first);
-    const uint64_t eq = __ballot(d == d0) & rem;
-    if (lane == first) atomicAdd(&row[d0], (uint32_t)__popcll(eq));
-    rem &= ~eq;
-  }
-  if ((rem >> lane) & 1ull) atomicAdd(&row[d], 1u);
-}
 
 // ---------------------------------------------------------------------------
 // The FP64 definition, for the rare splat the f32 bound cannot decide.
@@ -330,16 +317,18 @@ struct ProjectSmem {
   uint32_t recomp;
 };
 
-// BINS (the learned sort): the histogram is over the 512 bins of `lut`
-// (key -> bin) instead of over the two key bytes. A tile's bin lookups are
-// counted one tile later, so their latency hides behind the next tile's
+// BINS (the learned sort): each splat's `lut` entry (key -> bin << 9 |
+// offset in the bin) goes to binkey[] in place of the key, and the histogram
+// is over the 512 bins instead of over the two key bytes. A tile's lookups
+// are used one tile later, so their latency hides behind the next tile's
 // arithmetic.
 template <bool STD, bool BINS = false>
 __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
                                               const ViewK &v, uint16_t *__restrict__ keys,
                                               uint64_t *__restrict__ mask, float4 *__restrict__ cov,
                                               uint16_t *__restrict__ tilehist, uint32_t *__restrict__ recomp_acc,
-                                              uint32_t chunk, const uint16_t *__restrict__ lut = nullptr) {
+                                              uint32_t chunk, const uint32_t *__restrict__ lut = nullptr,
+                                              uint32_t *__restrict__ binkey = nullptr) {
   const uint32_t tid = threadIdx.x;
   __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 512; k += 256) sm.h[k] = 0;
@@ -351,9 +340,9 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
   const float bw2 = fabsf(v.mv[2]) + fabsf(v.mv[6]) + fabsf(v.mv[10]);
   uint32_t recomp = 0;
   const uint32_t first = chunk * (kTile / 256), end = min(first + kTile / 256, (n + 255u) >> 8);
-  // Two register buffers: a tile's records are loaded while the previous
-  // tile is worked on, so every wave keeps a tile of loads in flight behind
-  // its arithmetic.
+  // Three register buffers: two tiles' records are in flight while a tile is
+  // worked on, which hides the load latency at four waves a SIMD better than
+  // one tile in flight does at five.
   auto load = [&](uint32_t tile, uint4 &a, uint4 &b) {
     const uint32_t i = (tile << 8) + tid;
     if (tile < end && i < n) {
@@ -361,8 +350,8 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       b = rec[2 * (size_t)i + 1];
     }
   };
-  bool pend = false;     // BINS: a survivor of the previous tile whose bin is in flight
-  uint32_t pend_bin = 0;
+  bool pend_keep = false;  // BINS: the previous tile's splat, whose table entry is in flight
+  uint32_t pend_i = 0xFFFFFFFFu, pend_val = 0;
   auto work = [&](uint32_t tile, const uint4 &a, const uint4 &b) {
     const uint32_t i = (tile << 8) + tid;
     const bool in = i < n;
@@ -371,31 +360,40 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
     const uint64_t bal = __ballot(p.keep);
     if ((tid & 63u) == 0 && in) mask[i >> 6] = bal;
     if (in) {
-      keys[i] = (uint16_t)p.key;
-      // cov2d is the sort's output, never read back by it: a streaming store keeps it
-      // from displacing the keys and scatter data the next passes read from the caches
+      if (!BINS) __builtin_nontemporal_store((uint16_t)p.key, &keys[i]);
+      // Streaming stores for the keys and cov2d (the sort never reads cov2d back; the
+      // keys are read once, by the next pass): lines that sit dirty in L2 at the end
+      // of the launch are written back before the next launch starts.
       const f32x4 c = p.keep ? f32x4{p.xx, p.xy, p.yy, __uint_as_float(b.z)} : f32x4{0.f, 0.f, 0.f, 0.f};
       __builtin_nontemporal_store(c, (f32x4 *)&cov[i]);
     }
     recomp += (p.recomputed && in) ? 1u : 0u;
     if (BINS) {
-      if (pend) atomicAdd(&sm.h[pend_bin & 511u], 1u);
-      pend = p.keep;
-      pend_bin = p.keep ? (uint32_t)lut[p.key] : 0u;
+      if (pend_i < n) binkey[pend_i] = pend_val;
+      if (pend_keep) atomicAdd(&sm.h[(pend_val >> 9) & 511u], 1u);
+      pend_keep = p.keep;
+      pend_i = in ? i : 0xFFFFFFFFu;
+      pend_val = lut[p.key];
     } else if (p.keep) {
       atomicAdd(&sm.h[p.key & 255u], 1u);
       atomicAdd(&sm.h[256u + (p.key >> 8)], 1u);
     }
   };
-  uint4 a0 = make_uint4(0, 0, 0, 0), b0 = a0, a1 = a0, b1 = a0;
+  uint4 a0 = make_uint4(0, 0, 0, 0), b0 = a0, a1 = a0, b1 = a0, a2 = a0, b2 = a0;
   load(first, a0, b0);
-  for (uint32_t tile = first; tile < end; tile += 2) {
-    load(tile + 1, a1, b1);
+  load(first + 1, a1, b1);
+  for (uint32_t tile = first; tile < end; tile += 3) {
+    load(tile + 2, a2, b2);
     work(tile, a0, b0);
-    load(tile + 2, a0, b0);
+    load(tile + 3, a0, b0);
     if (tile + 1 < end) work(tile + 1, a1, b1);
+    load(tile + 4, a1, b1);
+    if (tile + 2 < end) work(tile + 2, a2, b2);
   }
-  if (BINS && pend) atomicAdd(&sm.h[pend_bin & 511u], 1u);
+  if (BINS) {
+    if (pend_i < n) binkey[pend_i] = pend_val;
+    if (pend_keep) atomicAdd(&sm.h[(pend_val >> 9) & 511u], 1u);
+  }
   if (recomp) atomicAdd(&sm.recomp, recomp);
   __syncthreads();
   for (uint32_t k = tid; k < 512; k += 256) tilehist[(size_t)chunk * 512 + k] = (uint16_t)sm.h[k];
@@ -403,48 +401,82 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
 }
 
 // ---------------------------------------------------------------------------
-// Scans between the passes. A workgroup takes 4 columns (digits) of a
-// [T][stride] u16 histogram with 64 row lanes: lane r sums rows
-// [r R, (r + 1) R), R = ceil(T / 64), the 64 partial sums of a column are
-// scanned by one wave, and the lane walks its rows again writing the
-// exclusive prefix down the column, prefix[t * pstride + c], for columns
-// below pcols; every column's sum goes to total[c]. No workgroup waits on
-// another: a scatter workgroup turns the totals into digit bases itself
-// (digit_bases), so no fence, look-back or last-block step is needed.
+// Scans between the passes. A workgroup takes kScanCols adjacent columns
+// (digits) of a [T][stride] u16 histogram, one row block a thread: thread t
+// sums rows [t R, (t + 1) R), R = ceil(T / 256), with one 8-byte load a row,
+// the workgroup scans the four column sums (a wave scan and one barrier), and
+// the thread writes its rows' exclusive prefixes, prefix[r * pstride + c],
+// one 16-byte store a row, from the rows it still holds (up to kHeld rows a
+// thread, 1M tiles of 4096; beyond that it reads them again). Only columns
+// below pcols get prefixes; every column's sum goes to total[c]. No
+// workgroup waits on another: a scatter workgroup turns the totals into digit
+// bases itself (digit_bases), so no fence, look-back or last-block step is
+// needed.
 __device__ __forceinline__ void scan_cols(uint32_t c0, uint32_t T, const uint16_t *__restrict__ hist, uint32_t stride,
                                           uint32_t *__restrict__ prefix, uint32_t pstride, uint32_t pcols,
-                                          uint32_t *__restrict__ total, uint32_t (*s_part)[64]) {
-  const uint32_t tid = threadIdx.x, cl = tid & 3u, rl = tid >> 2, c = c0 + cl;
-  const uint32_t R = (T + 63u) >> 6, r0 = min(rl * R, T), r1 = min(r0 + R, T);
-  uint32_t sum = 0;
-  for (uint32_t t0 = r0; t0 < r1; t0 += 32) {
-    uint32_t v[32];
+                                          uint32_t *__restrict__ total, uint32_t (*s_w)[kScanCols]) {
+  static_assert(kScanCols == 4, "one 8-byte load and one 16-byte store a row");
+  constexpr uint32_t kHeld = 16;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t R = (T + 255u) >> 8;
+  const uint32_t r0 = min(tid * R, T), r1 = min(r0 + R, T);
+  auto row = [&](uint32_t r) { return *reinterpret_cast<const uint2 *>(hist + (size_t)r * stride + c0); };
+  auto add = [](uint32_t (&acc)[4], uint2 x) {
+    acc[0] += x.x & 0xFFFFu;
+    acc[1] += x.x >> 16;
+    acc[2] += x.y & 0xFFFFu;
+    acc[3] += x.y >> 16;
+  };
+  uint32_t sum[4] = {0, 0, 0, 0};
+  uint2 held[kHeld];
+  const bool hold = R <= kHeld;
+  if (hold && T > 0) {
+    // every row is loaded before any is added: a thread's loads are in flight together
 #pragma unroll
-    for (uint32_t k = 0; k < 32; ++k) v[k] = (t0 + k < r1) ? hist[(size_t)(t0 + k) * stride + c] : 0u;
+    for (uint32_t k = 0; k < kHeld; ++k) {
+      const bool ok = r0 + k < r1;
+      const uint2 x = row(ok ? r0 + k : 0u);  // row 0 exists when T > 0
+      held[k] = ok ? x : make_uint2(0u, 0u);
+    }
 #pragma unroll
-    for (uint32_t k = 0; k < 32; ++k) sum += v[k];
+    for (uint32_t k = 0; k < kHeld; ++k) add(sum, held[k]);
+  } else {
+    for (uint32_t r = r0; r < r1; ++r) add(sum, row(r));
   }
-  s_part[cl][rl] = sum;
+  uint32_t inc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) inc[c] = wave_scan_incl(sum[c]);
+  if (lane == 63u)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_w[w][c] = inc[c];
   __syncthreads();
-  {
-    // wave w scans column w's 64 partial sums
-    const uint32_t w = tid >> 6, lane = tid & 63u;
-    const uint32_t x = s_part[w][lane];
-    const uint32_t inc = wave_scan_incl(x);
-    s_part[w][lane] = inc - x;
-    if (lane == 63) total[c0 + w] = inc;
+  uint32_t run[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint32_t t = s_w[k][c];
+      off += k < w ? t : 0u;
+      tot += t;
+    }
+    run[c] = off + inc[c] - sum[c];
+    if (tid == (uint32_t)c) total[c0 + c] = tot;
   }
-  __syncthreads();
-  if (!prefix || c >= pcols) return;
-  uint32_t run = s_part[cl][rl];
-  for (uint32_t t0 = r0; t0 < r1; t0 += 32) {
-    uint32_t v[32];
+  if (!prefix || c0 >= pcols || T == 0) return;
+  auto put = [&](uint32_t r) {
+    *reinterpret_cast<uint4 *>(prefix + (size_t)r * pstride + c0) = make_uint4(run[0], run[1], run[2], run[3]);
+  };
+  if (hold) {
 #pragma unroll
-    for (uint32_t k = 0; k < 32; ++k) v[k] = (t0 + k < r1) ? hist[(size_t)(t0 + k) * stride + c] : 0u;
-#pragma unroll
-    for (uint32_t k = 0; k < 32; ++k) {
-      if (t0 + k < r1) prefix[(size_t)(t0 + k) * pstride + c] = run;
-      run += v[k];
+    for (uint32_t k = 0; k < kHeld; ++k) {
+      if (r0 + k < r1) put(r0 + k);
+      add(run, held[k]);
+    }
+  } else {
+    for (uint32_t r = r0; r < r1; ++r) {
+      put(r);
+      add(run, row(r));
     }
   }
 }
@@ -459,8 +491,9 @@ __device__ __forceinline__ uint32_t digit_bases(uint32_t total_d, uint32_t *s_w,
 // The high-byte histogram of tile t of the low-byte pass's output (positions
 // t * kTile ...), the upsweep of the high-byte pass: counts only, so plain
 // LDS adds, each wave into its own row (a slab seen edge on, whose survivors
-// share a few high bytes, then contends within a wave only), hot digits
-// peeled off first (count_digit).
+// share a few high bytes, then contends within a wave only). Same-address
+// adds of one instruction serialise in the LDS, and that still costs less
+// than peeling the hot digits off with ballots first (measured on a slab).
 struct HistSmem {
   uint32_t h[4][256];
 };
@@ -492,7 +525,8 @@ __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32
   for (uint32_t k = tid; k < 1024; k += 256) (&sm.h[0][0])[k] = 0;
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < KPT; ++r) count_digit(sm.h[w], raw[r] >> 24, __ballot(first + r < limit));
+  for (int r = 0; r < KPT; ++r)
+    if (first + r < limit) atomicAdd(&sm.h[w][raw[r] >> 24], 1u);
   __syncthreads();
   tilehist3[(size_t)t * 256 + tid] = (uint16_t)(sm.h[0][tid] + sm.h[1][tid] + sm.h[2][tid] + sm.h[3][tid]);
 }
@@ -647,7 +681,15 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
       const uint32_t p = p0 + (uint32_t)k * 256u + tid;
       const uint32_t value = HI ? (e[k] & 0xFFFFFFu)
                                 : (((e[k] >> kLocalBits) & 255u) << 24) | (sbase + (e[k] & ((1u << kLocalBits) - 1u)));
-      if (p < nv && g[k] < n) out[g[k]] = value;  // n bounds both outputs; a guard against a miscount
+      // n bounds both outputs; a guard against a miscount. The draw order is the
+      // path's output: streamed out, so that the next launch does not start behind
+      // its write-back.
+      if (p < nv && g[k] < n) {
+        if (HI)
+          __builtin_nontemporal_store(value, &out[g[k]]);
+        else
+          out[g[k]] = value;
+      }
     }
   }
 }

@@ -51,10 +51,10 @@ __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, 
 // the FP64 recompute count of pass 1 to where the host reads it.
 __global__ __launch_bounds__(256) void k_scan(uint32_t T, const uint16_t *__restrict__ tilehist,
                                               uint32_t *__restrict__ prefix_lo, Ctl *__restrict__ ctl) {
-  __shared__ uint32_t s_part[kScanCols][64];
+  __shared__ uint32_t s_w[4][kScanCols];
   // XCD-contiguous column groups: the 4-column pieces of one prefix row's cache
   // lines are written from one L2
-  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T, tilehist, 512, prefix_lo, 256, 256, ctl->total, s_part);
+  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T, tilehist, 512, prefix_lo, 256, 256, ctl->total, s_w);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->recomputed = ctl->recomputed_acc;
     ctl->recomputed_acc = 0;
@@ -83,9 +83,9 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t *__restrict__ packe
 // Scan of pass 3: high-byte tile prefixes over the tiles of V.
 __global__ __launch_bounds__(256) void k_scan2(const uint16_t *__restrict__ tilehist3, uint32_t *__restrict__ prefix_hi,
                                                Ctl *__restrict__ ctl) {
-  __shared__ uint32_t s_part[kScanCols][64];
+  __shared__ uint32_t s_w[4][kScanCols];
   const uint32_t T3 = (ctl->count + kTile - 1) / kTile;
-  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T3, tilehist3, 256, prefix_hi, 256, 256, ctl->total3, s_part);
+  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T3, tilehist3, 256, prefix_hi, 256, 256, ctl->total3, s_w);
 }
 
 // ---------------------------------------------------------------------------
@@ -243,6 +243,7 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
   const ViewK vk = make_viewk(view);
   const bool std_form = is_std(view);
   st->last_n = n;
+  st->keys_kept = true;
   const uint32_t tiles = (n + kTile - 1) / kTile;
   const uint32_t supers = (tiles + kSuper - 1) / kSuper;
   const uint4 *rec = (const uint4 *)cloud->records;

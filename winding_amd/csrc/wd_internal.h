@@ -82,16 +82,17 @@ struct BatchJob {
   uint16_t *tilehist, *tilehist3;
   uint32_t *prefix_lo, *prefix_hi;
   // the learned sort's (unused by the exact path)
-  uint16_t *lut;          // [65536] key -> bin
-  uint16_t *binlo;        // [kBins] lowest key of each bin
+  uint32_t *lut;          // [65536] key -> bin << 9 | the key's offset in its bin
   uint32_t *prefix_bin;   // [tiles][512] where a tile's run of each bin starts
   const float *probs;     // [256] predicted bin widths (the predictor's output row)
   uint32_t cap;           // most survivors a bin may hold; more and the view runs the exact path
   uint32_t n, std_form;
   uint32_t tiles;    // projection chunks = scatter tiles, of kTile splats
 };
-// Tiles a scatter work item takes (a super-tile).
-constexpr int kSuper = 2;
+// Tiles a scatter work item takes (a super-tile) on the single-sort and
+// persistent paths: 1 measured faster than 2 at 1M and 8M splats (more
+// workgroups in flight); the phased batch lets the autotuner pick 1 or 2.
+constexpr int kSuper = 1;
 // The learned sort: bins a view's key line is cut into (256 predicted, the
 // wide ones split), keys a bin may span (its offsets fit 9 bits), views one
 // projection launch takes, and the largest cloud (indices packed in 23 bits).
@@ -177,7 +178,7 @@ struct wd_batch {
   hipGraphExec_t exec = nullptr;
   uint32_t grid = 0, total = 0;
   int schedule = WD_SCHEDULE_PHASED;
-  uint32_t super_tiles = 2;                     // the scatter work item: tiles a workgroup (1 or 2)
+  uint32_t super_tiles = 1;                     // the scatter work item: tiles a workgroup (1 or 2)
   std::vector<wd_job> api_jobs;                 // the jobs as set, to rebuild the tables on a change
   void *stage[2] = {nullptr, nullptr};          // pinned staging for the tables
   hipEvent_t staged[2] = {nullptr, nullptr};    // the copy out of stage[i] has finished
@@ -191,6 +192,7 @@ struct wd_state {
   wd_context *ctx = nullptr;
   uint64_t capacity = 0;
   uint64_t last_n = 0;
+  bool keys_kept = true;       // false after a learned run, which writes no keys
   wd::Ctl *ctl = nullptr;      // device
   uint16_t *keys = nullptr;    // n keys, every splat
   uint64_t *mask = nullptr;    // survivor bits, ceil(n/64) words
@@ -200,7 +202,7 @@ struct wd_state {
   uint16_t *tilehist = nullptr;   // per 4096-splat chunk: 256 low-byte then 256 high-byte counts
   uint16_t *tilehist3 = nullptr;  // per tile of the low-byte pass's output: 256 high-byte counts
   uint32_t *prefix_lo = nullptr, *prefix_hi = nullptr;  // [tiles][256]: where a tile's digit run starts
-  uint16_t *lut = nullptr, *binlo = nullptr;          // learned sort: key -> bin, bin -> lowest key
+  uint32_t *lut = nullptr;                             // learned sort: key -> bin << 9 | offset
   uint32_t *prefix_bin = nullptr;                      // learned sort: [tiles][512]
   uint64_t tiles = 0;
   uint32_t magic = 0x53544154;
