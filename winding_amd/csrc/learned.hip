@@ -243,9 +243,18 @@ __device__ __forceinline__ void bin_offsets(uint32_t (*cnt)[512], uint32_t *s_w)
 // ---------------------------------------------------------------------------
 // A bin's survivors, in index order, into the view's order by their 9-bit
 // offset in the bin: one stable counting pass. Grid (512, views), 16 waves;
-// wave w ranks a contiguous run of the bin, so the ranks stay stable.
+// wave w ranks a contiguous run of the bin, so the ranks stay stable. A bin
+// of up to kHeld rounds a wave is held in registers: ranked within each wave
+// (which also counts it), placed into LDS once the waves' starts are known,
+// and written out in order (whole lines, streamed: the draw order is the
+// path's output). A bigger one counts, then ranks straight to memory,
+// reading its run twice.
+constexpr uint32_t kHeld = 24;                     // rounds a wave holds: bins up to 24,576 survivors in one go
+constexpr uint32_t kStage = kHeld * 64u * 16u;     // LDS slots of a held bin
+
 __global__ __launch_bounds__(1024) void k_bin_sort(const BatchJob *__restrict__ jobs) {
   __shared__ uint32_t cnt[16][512];
+  __shared__ uint32_t stage[kStage];
   __shared__ uint32_t s_w[16], s_base, s_count;
   const BatchJob &J = jobs[blockIdx.y];
   const uint32_t b = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -263,7 +272,6 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const BatchJob *__restrict__ 
   if (c == 0) return;
   const uint32_t chunk = ((c + 1023u) >> 10) << 6;  // a multiple of 64 a wave
   const uint32_t w0 = min(w * chunk, c), w1 = min(w0 + chunk, c);
-  constexpr uint32_t kHeld = 24;  // rounds a wave holds in registers: bins up to 24,576 survivors in one go
   if (chunk <= 64u * kHeld) {
     // every element of the wave's run loaded at once, then both passes from registers
     uint32_t val[kHeld];
@@ -272,23 +280,13 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const BatchJob *__restrict__ 
       const uint32_t pos = w0 + r * 64u + lane;
       val[r] = pos < w1 ? J.packed[base + pos] : 0u;
     }
+    // Ranks within the wave's run first (the wave's counts per offset come out of
+    // the same pass), then the waves' starts per offset, then the placement.
+    uint32_t rank[kHeld];
 #pragma unroll
     for (uint32_t r = 0; r < kHeld; ++r) {
-      if (w0 + r * 64u >= w1) break;
-      const bool ok = w0 + r * 64u + lane < w1;
-      const uint64_t valid = __ballot(ok);
-      const uint32_t k = val[r] >> 23;
-      const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
-      if (__ballot(ok && k != k0) == 0) {
-        if (lane == 0) cnt[w][k0] += (uint32_t)__popcll(valid);
-      } else if (ok) {
-        atomicAdd(&cnt[w][k], 1u);
-      }
-    }
-    bin_offsets(cnt, s_w);
-#pragma unroll
-    for (uint32_t r = 0; r < kHeld; ++r) {
-      if (w0 + r * 64u >= w1) break;
+      rank[r] = 0;
+      if (w0 + r * 64u >= w1) continue;
       const bool ok = w0 + r * 64u + lane < w1;
       const uint64_t valid = __ballot(ok);
       const uint32_t k = val[r] >> 23;
@@ -296,9 +294,18 @@ __global__ __launch_bounds__(1024) void k_bin_sort(const BatchJob *__restrict__ 
       const uint32_t below = count_below(peers);
       const uint32_t before = cnt[w][k];
       if (ok && below == 0) cnt[w][k] = before + (uint32_t)__popcll(peers);
-      const uint32_t rk = before + below;
-      if (ok && rk < c) J.order[base + rk] = val[r] & 0x7FFFFFu;
+      rank[r] = before + below;
     }
+    bin_offsets(cnt, s_w);
+#pragma unroll
+    for (uint32_t r = 0; r < kHeld; ++r) {
+      if (w0 + r * 64u >= w1) break;
+      const bool ok = w0 + r * 64u + lane < w1;
+      const uint32_t rk = cnt[w][val[r] >> 23] + rank[r];
+      if (ok && rk < kStage) stage[rk] = val[r] & 0x7FFFFFu;  // rk < c <= kStage by the counts
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < c; p += 1024) __builtin_nontemporal_store(stage[p], &J.order[base + p]);
     return;
   }
   // a bin too big to hold: the same two passes, reading the run twice

@@ -369,7 +369,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
     }
     recomp += (p.recomputed && in) ? 1u : 0u;
     if (BINS) {
-      if (pend_i < n) binkey[pend_i] = pend_val;
+      if (pend_i < n) __builtin_nontemporal_store(pend_val, &binkey[pend_i]);
       if (pend_keep) atomicAdd(&sm.h[(pend_val >> 9) & 511u], 1u);
       pend_keep = p.keep;
       pend_i = in ? i : 0xFFFFFFFFu;
@@ -391,7 +391,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
     if (tile + 2 < end) work(tile + 2, a2, b2);
   }
   if (BINS) {
-    if (pend_i < n) binkey[pend_i] = pend_val;
+    if (pend_i < n) __builtin_nontemporal_store(pend_val, &binkey[pend_i]);
     if (pend_keep) atomicAdd(&sm.h[(pend_val >> 9) & 511u], 1u);
   }
   if (recomp) atomicAdd(&sm.recomp, recomp);
