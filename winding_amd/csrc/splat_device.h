@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "wd_internal.h"
 
 namespace wd {
@@ -618,28 +620,37 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
     __syncthreads();
     uint32_t rank[KPT];
     uint32_t okbits = 0;
+    // FULL: every element of the tile is valid (the high-byte pass away from the
+    // end of V), so the round needs no ballot of the bounds and no mask.
+    auto rank_rounds = [&](auto full_c) {
+      constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
-    for (int r = 0; r < KPT; ++r) {
-      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-      uint64_t valid;
-      uint32_t dd;
-      if (HI) {
-        valid = __ballot(i < limit);
-        dd = raw[s][r] >> 24;
-      } else {
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)vm[s], r);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(vm[s] >> 32), r);
-        valid = ((uint64_t)hi << 32) | lo;
-        dd = raw[s][r] & 255u;
+      for (int r = 0; r < KPT; ++r) {
+        const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+        uint64_t valid;
+        uint32_t dd;
+        if (HI) {
+          valid = FULL ? ~0ull : __ballot(i < limit);
+          dd = raw[s][r] >> 24;
+        } else {
+          const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)vm[s], r);
+          const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(vm[s] >> 32), r);
+          valid = ((uint64_t)hi << 32) | lo;
+          dd = raw[s][r] & 255u;
+        }
+        const bool ok = FULL ? true : ((valid >> lane) & 1ull) != 0;
+        const uint64_t peers = match_digit(dd, valid);
+        const uint32_t below = count_below(peers);
+        const uint32_t before = sm.wh[w][dd];  // any digit is a valid index: read unconditionally
+        if (ok && below == 0) sm.wh[w][dd] = before + (uint32_t)__popcll(peers);
+        rank[r] = before + below;
+        okbits |= (ok ? 1u : 0u) << r;
       }
-      const bool ok = (valid >> lane) & 1ull;
-      const uint64_t peers = match_digit(dd, valid);
-      const uint32_t below = count_below(peers);
-      const uint32_t before = sm.wh[w][dd];  // any digit is a valid index: read unconditionally
-      if (ok && below == 0) sm.wh[w][dd] = before + (uint32_t)__popcll(peers);
-      rank[r] = before + below;
-      okbits |= (ok ? 1u : 0u) << r;
-    }
+    };
+    if (HI && (t0 + s + 1) * (uint32_t)kTile <= limit)
+      rank_rounds(std::integral_constant<bool, true>{});
+    else
+      rank_rounds(std::integral_constant<bool, false>{});
     __syncthreads();
     {
       // wave offsets of digit d within this tile, plus the tile's start
