@@ -48,13 +48,13 @@ def orbit_views(cloud, first, count, step=37):
     return [wl.view_at(cloud, eyes[(first + k * step) % 1024], W, H) for k in range(count)]
 
 
-def check_views(ctx, cloud, exact, states, views, label):
+def check_views(ctx, cloud, exact, states, views, label, n=N):
     """Each view's learned result against wd_sort's; returns how many views fell back."""
     fell = 0
     for k, (view, st) in enumerate(zip(views, states)):
-        order, cov, stats = st.read(n_cov=N)
+        order, cov, stats = st.read(n_cov=n)
         exact.sort(cloud, view)
-        e_order, e_cov, e_stats = exact.read(n_cov=N)
+        e_order, e_cov, e_stats = exact.read(n_cov=n)
         assert stats.count == e_stats.count, "%s view %d: V %d vs %d" % (label, k, stats.count, e_stats.count)
         assert np.array_equal(order, e_order), "%s view %d: order differs" % (label, k)
         # cov2d: the same survivors and colours; the footprint to the FP32 bar (both kernels compile the f32
@@ -126,3 +126,32 @@ def test_trained_predictor_beats_key_width_bins(ctx, scene, trained):
     finally:
         learned.destroy()
         flat.destroy()
+
+
+@pytest.mark.parametrize("n", [1, 1000, 70001])
+def test_ragged_sizes_and_a_view_with_no_survivors(ctx, n):
+    """Clouds that end inside a wave, a round and a tile, and a view that culls every splat (no
+    survivors, so every bin is empty): still the exact order, with no fallback."""
+    recs = wa.synth_records(0, 11, n)
+    cloud = wa.Cloud(ctx, recs)
+    exact = wa.State(ctx, n)
+    states = [wa.State(ctx, n) for _ in range(3)]
+    flat = wa.Mlp(ctx, seed=0)
+    flat.set_params(np.zeros(wa.WD_MLP_PARAMS, np.float32))
+    learned = wa.Learned(ctx, 3)
+    try:
+        eyes = wl.orbit_sequence(1024)
+        views = [wl.view_at(cloud, eyes[k * 300], W, H) for k in range(2)]
+        away = wl.view_at(cloud, eyes[0], W, H)
+        away.model_view[14] += 1000.0  # everything behind the camera
+        views.append(away)
+        learned.run(flat, cloud, views, states)
+        assert check_views(ctx, cloud, exact, states, views, "n=%d" % n, n=n) == 0
+        assert states[2].stats().count == 0
+    finally:
+        learned.destroy()
+        flat.destroy()
+        for st in states:
+            st.destroy()
+        exact.destroy()
+        cloud.destroy()
