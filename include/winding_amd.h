@@ -172,7 +172,6 @@ int wd_state_stats(wd_context *ctx, wd_state *state, wd_stats *stats);
 /*
  * Copies the 16-bit keys of every splat and the survivor bitmask (bit i of
  * word i/64) of the last wd_sort: the parity tests compare them per splat.
- * A learned run keeps no keys: after one, asking for keys is WD_E_STATE.
  */
 int wd_state_read_keys(wd_context *ctx, wd_state *state, uint16_t *keys,
                        uint64_t *mask, uint64_t n);
@@ -293,8 +292,7 @@ int wd_learned_destroy(wd_learned *learned);
 /*
  * Training targets from an exact sort in `state`: edges[k] = the key at rank
  * floor(k V / 256) (k = 1..255), edges[0] = 0, edges[256] = 65536. The
- * predictor's target widths are (edges[k+1] - edges[k]) / 65536. A state
- * whose last sort was a learned run keeps no keys: WD_E_STATE.
+ * predictor's target widths are (edges[k+1] - edges[k]) / 65536.
  */
 int wd_state_quantiles(wd_context *ctx, wd_state *state, uint32_t edges[257]);
 

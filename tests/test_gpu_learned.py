@@ -78,9 +78,9 @@ def test_learned_is_exact_and_stays_on_the_fast_path(ctx, scene, trained):
             fell = check_views(ctx, cloud, exact, states, views, "trained")
             print("orbit from %d: %d of 8 views fell back" % (first, fell), flush=True)
             assert fell == 0
-        # a learned run writes no keys: reading them from its state is refused, not stale
-        with pytest.raises(wa.WindingError, match="keeps no keys"):
-            states[0].quantiles()
+        # a learned run keeps the keys as well: its state's training targets are the exact sort's
+        exact.sort(cloud, views[0])
+        assert np.array_equal(states[0].quantiles(), exact.quantiles())
         # fewer views than the run holds, and a single view
         for k in (3, 1):
             views = orbit_views(cloud, 500, k)

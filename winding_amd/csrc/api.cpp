@@ -386,7 +386,6 @@ int wd_state_read_keys(wd_context *ctx, wd_state *s, uint16_t *keys, uint64_t *m
   if (!ctx || !s || s->magic != 0x53544154) return fail(WD_E_ARG, "read keys: null argument or destroyed state");
   if (n > s->last_n) return fail(WD_E_ARG, "read keys: %llu asked, last sort had %llu", (unsigned long long)n,
                                  (unsigned long long)s->last_n);
-  if (keys && !s->keys_kept) return fail(WD_E_STATE, "read keys: the last sort of this state was a learned run, which keeps no keys");
   WD_HIP(hipStreamSynchronize(ctx->stream), "read keys: sync");
   if (keys) WD_HIP(hipMemcpy(keys, s->keys, (size_t)n * 2, hipMemcpyDeviceToHost), "read keys: keys");
   if (mask) WD_HIP(hipMemcpy(mask, s->mask, (size_t)((n + 63) / 64) * 8, hipMemcpyDeviceToHost), "read keys: mask");

@@ -79,7 +79,6 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     const uint32_t S = b->schedule == WD_SCHEDULE_PERSISTENT ? (uint32_t)wd::kSuper : b->super_tiles;
     u[i + 1] = u[i] + (B.tiles + S - 1) / S;
     s->last_n = B.n;
-    s->keys_kept = true;
     shape.push_back(J.cloud);
     shape.push_back(J.state);
   }

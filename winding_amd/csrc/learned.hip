@@ -4,13 +4,13 @@
 // The predictor (mlp.hip) gives, per view, 256 bin widths in key space whose
 // bins should each hold 1/256 of the survivors. Then, for up to kMaxViews
 // views of one cloud:
-//   k_bin_table      the widths as bin edges on the 16-bit key line; a bin
-//                    wider than kBinWidth keys is split, so a key's offset in
-//                    its bin fits 9 bits. Writes key -> bin << 9 | offset.
-//   k_project_bins   pass 1 of every view: the same survivor bits and cov2d
-//                    as k_project; in place of the key, the key's table entry
-//                    (parked in the view's order buffer until the last pass
-//                    writes it); the chunk's histogram over bins.
+//   k_bin_table      the widths as bin edges on the 16-bit key line, each on a
+//                    multiple of 16 keys; a bin wider than kBinWidth keys is
+//                    split, so a key's offset in its bin fits 9 bits. Writes
+//                    one u16 a 16-key block (bin_key), 8 KB a view, which the
+//                    next two kernels hold in LDS.
+//   k_project_bins   pass 1 of every view: the same keys, survivor bits and
+//                    cov2d as k_project, and the chunk's histogram over bins.
 //   (k_scan_bins)    column prefixes and totals of the bin histograms.
 //   k_bin_scatter    the check: every bin's total is at most the cap, or this
 //                    view falls back. Then one stable pass by bin: the
@@ -20,8 +20,7 @@
 //                    9-bit offset in LDS, so key ascending then index
 //                    ascending, written to the view's order.
 // Against the exact path (two passes over the key's bytes) this leaves out
-// the high-byte histogram pass, its scan and the low-byte scatter's second
-// read. A view whose check fails is sorted again by the exact path in the
+// the high-byte histogram pass and its scan. A view whose check fails is sorted again by the exact path in the
 // same enqueue (the gated persistent batch kernel), so the output is exact
 // either way: the prediction only ever decides the speed.
 #include "splat_device.h"
@@ -52,9 +51,9 @@ __device__ __forceinline__ uint32_t block_scan_excl_n(uint32_t v, uint32_t *s_w,
 }
 
 // ---------------------------------------------------------------------------
-// Bin edges from the predicted widths, and the key -> bin table. Grid
-// (256, views): workgroup x fills keys [256 x, 256 x + 256); each recomputes
-// the edges (256 values) rather than wait on another.
+// Bin edges from the predicted widths, and the block table. Grid (16, views):
+// workgroup x fills blocks [256 x, 256 x + 256); each recomputes the edges
+// (256 values) rather than wait on another.
 __global__ __launch_bounds__(256) void k_bin_table(const BatchJob *__restrict__ jobs, uint32_t *__restrict__ gate) {
   const BatchJob &J = jobs[blockIdx.y];
   __shared__ uint32_t B[257], F[257], s_w[4];
@@ -64,25 +63,34 @@ __global__ __launch_bounds__(256) void k_bin_table(const BatchJob *__restrict__ 
   const uint32_t q = p > 0.f ? (uint32_t)fminf(p * 1048576.f + 0.5f, 1048576.f) : 0u;
   uint32_t Q;
   const uint32_t cum = block_scan_excl(q, s_w, &Q);
-  B[t] = Q ? (uint32_t)(((uint64_t)cum * 65536u) / Q) : t * 256u;
+  const uint32_t e = Q ? (uint32_t)(((uint64_t)cum * 65536u) / Q) : t * 256u;
+  B[t] = (e + kBinBlock / 2) / kBinBlock * kBinBlock;  // nearest block edge: still non-decreasing
   if (t == 0) B[256] = 65536u;
   __syncthreads();
   const uint32_t width = B[t + 1] - B[t];
-  const uint32_t parts = (width + kBinWidth - 1) / kBinWidth;
+  const uint32_t parts = (width + kBinWidth - 1) / kBinWidth;  // 0 for a bin that rounded away
   uint32_t nb;
   const uint32_t first = block_scan_excl(parts, s_w, &nb);
   F[t] = first;
   if (t == 0) F[256] = nb;
   __syncthreads();
-  const uint32_t key = blockIdx.x * 256u + t;
-  uint32_t lo = 0, hi = 256;  // B[lo] <= key < B[hi]
+  const uint32_t key = (blockIdx.x * 256u + t) * kBinBlock;
+  uint32_t lo = 0, hi = 256;  // B[lo] <= key < B[hi]: lo's bin is not empty
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
     if (B[mid] <= key) lo = mid; else hi = mid;
   }
   const uint32_t rel = key - B[lo];
-  J.lut[key] = ((F[lo] + rel / kBinWidth) << 9) | (rel % kBinWidth);
+  J.lut[key / kBinBlock] = (uint16_t)((((F[lo] + rel / kBinWidth) << 9) | (rel % kBinWidth)) >> 4);
   if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) *gate = 0;  // no view has fallen back yet this run
+}
+
+// The view's block table into LDS: 8 KB, two 16-byte loads a thread. The
+// caller's next barrier makes it visible.
+__device__ __forceinline__ void load_blocks(uint16_t *dst, const uint16_t *__restrict__ src) {
+  static_assert(kBinBlocks * 2 == 512 * 16, "two 16-byte loads a thread");
+  for (uint32_t k = threadIdx.x; k < 512u; k += blockDim.x)
+    reinterpret_cast<uint4 *>(dst)[k] = reinterpret_cast<const uint4 *>(src)[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -94,9 +102,11 @@ __global__ __launch_bounds__(256) void k_bin_table(const BatchJob *__restrict__ 
 template <bool STD>
 __global__ __launch_bounds__(256) void k_project_bins(const BatchJob *__restrict__ jobs) {
   __shared__ ProjectSmem sm;
+  __shared__ __align__(16) uint16_t blk[kBinBlocks];
   const BatchJob &J = jobs[blockIdx.y];
+  load_blocks(blk, J.lut);  // project_chunk starts with a barrier
   project_chunk<STD, true>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
-                           &J.ctl->recomputed_acc, blockIdx.x, J.lut, J.order);
+                           &J.ctl->recomputed_acc, blockIdx.x, blk);
 }
 
 // Column prefixes and totals of the bin histograms. Grid (512 / kScanCols, views).
@@ -120,6 +130,7 @@ struct BinScatterSmem {
   uint32_t shift[512];   // global start - LDS start of each bin
   uint32_t lstart[512];
   uint32_t w[4];
+  __align__(16) uint16_t blk[kBinBlocks];  // the view's block table
 };
 
 __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict__ jobs, uint32_t *__restrict__ gate) {
@@ -133,10 +144,11 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict_
 #pragma unroll
   for (int r = 0; r < KPT; ++r) {
     const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-    raw[r] = i < n ? J.order[i] : 0u;  // bin << 9 | offset, left there by the projection
+    raw[r] = i < n ? (uint32_t)J.keys[i] : 0u;
   }
   const uint32_t wi = (wbase >> 6) + lane;
   const uint64_t vm = (lane < (uint32_t)KPT && wi < nwords) ? J.mask[wi] : 0ull;
+  load_blocks(sm.blk, J.lut);  // visible after the check's barriers
   const uint32_t tot0 = J.ctl->total[d0], tot1 = J.ctl->total[d1];
   const uint32_t cnt0 = J.tilehist[(size_t)t * 512 + d0], cnt1 = J.tilehist[(size_t)t * 512 + d1];
   const uint32_t pre0 = J.prefix_bin[(size_t)t * 512 + d0], pre1 = J.prefix_bin[(size_t)t * 512 + d1];
@@ -162,7 +174,10 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const BatchJob *__restrict_
   sm.shift[d1] = b1 + pre1 - l1;
   uint32_t bin[KPT];
 #pragma unroll
-  for (int r = 0; r < KPT; ++r) bin[r] = raw[r] >> 9;
+  for (int r = 0; r < KPT; ++r) {
+    raw[r] = bin_key(sm.blk, raw[r]);
+    bin[r] = raw[r] >> 9;
+  }
   for (uint32_t k = tid; k < 2048; k += 256) (&sm.wh[0][0])[k] = 0;
   __syncthreads();
   uint32_t rank[KPT];
@@ -367,7 +382,8 @@ int launch_learned(wd_context *ctx, const BatchJob *jobs, uint32_t nv, uint32_t 
     WD_HIP(hipGetLastError(), what);              \
     if (ctx->prof_end(id, ev)) return WD_E_HIP;   \
   } while (0)
-  WD_LAUNCH(kKBinTable, "learned: bin table launch", k_bin_table, dim3(256, nv), dim3(256), 0, ctx->stream, jobs, gate);
+  WD_LAUNCH(kKBinTable, "learned: bin table launch", k_bin_table, dim3(kBinBlocks / 256, nv), dim3(256), 0, ctx->stream,
+            jobs, gate);
   if (std_form)
     WD_LAUNCH(kKProject, "learned: project launch", k_project_bins<true>, dim3(tiles, nv), dim3(256), 0, ctx->stream,
               jobs);

@@ -42,7 +42,7 @@ bool alive(const wd_learned *l) { return l && l->magic == 0x4c524e44; }
 // Learned-sort buffers of a state, made on its first learned run.
 int ensure_learned(wd_context *ctx, wd_state *s) {
   if (s->lut) return WD_OK;
-  hipError_t e = hipMalloc(&s->lut, 65536 * sizeof(uint32_t));
+  hipError_t e = hipMalloc(&s->lut, wd::kBinBlocks * sizeof(uint16_t));
   if (e == hipSuccess) e = hipMalloc(&s->prefix_bin, s->tiles * 512 * sizeof(uint32_t));
   if (e != hipSuccess) return wd::hip_fail(e, "learned: state buffers");
   return WD_OK;
@@ -167,7 +167,6 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
     B.probs = probs + 256 * v;
     B.cap = bin_cap;
     s->last_n = n;
-    s->keys_kept = false;
   }
   WD_HIP(hipMemcpyAsync(l->jobs, l->host_jobs[f], sizeof(wd::BatchJob) * n_views, hipMemcpyHostToDevice, ctx->stream),
          "learned: job table");
@@ -196,7 +195,6 @@ int wd_learned_destroy(wd_learned *l) {
 
 int wd_state_quantiles(wd_context *ctx, wd_state *s, uint32_t edges[257]) {
   if (!ctx || !s || s->magic != 0x53544154 || !edges) return fail(WD_E_ARG, "quantiles: null argument or destroyed");
-  if (!s->keys_kept) return fail(WD_E_STATE, "quantiles: the last sort of this state was a learned run, which keeps no keys");
   WD_HIP(hipSetDevice(ctx->device), "quantiles: set device");
   uint32_t *dev = nullptr;
   WD_HIP(hipMalloc(&dev, 257 * sizeof(uint32_t)), "quantiles: allocation");

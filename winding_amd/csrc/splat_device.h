@@ -314,23 +314,27 @@ __device__ __forceinline__ Projected project_one(const ViewK &v, uint4 a, uint4 
 // key, survivor bit and cov2d slot (culled splats get zeros, so every cache
 // line is written whole, by one wave), and the chunk's two 256-bin digit
 // histograms: no global atomics.
+// The learned sort's bin key of a 16-bit key: bin << 9 | the key's offset in
+// its bin. Bin edges fall on multiples of 16 keys, so a 4096-entry table of
+// u16 (the block's bin key >> 4) in LDS gives it with one read and an or.
+__device__ __forceinline__ uint32_t bin_key(const uint16_t *blk, uint32_t key) {
+  return ((uint32_t)blk[key >> 4] << 4) | (key & 15u);
+}
+
 struct ProjectSmem {
   uint32_t h[512];  // survivors per low key byte, then per high key byte
   uint32_t recomp;
 };
 
-// BINS (the learned sort): each splat's `lut` entry (key -> bin << 9 |
-// offset in the bin) goes to binkey[] in place of the key, and the histogram
-// is over the 512 bins instead of over the two key bytes. A tile's lookups
-// are used one tile later, so their latency hides behind the next tile's
-// arithmetic.
+// BINS (the learned sort): the chunk's histogram is over the view's 512 bins
+// instead of over the two key bytes; a survivor's bin comes from the view's
+// block table, already in LDS (bin_key).
 template <bool STD, bool BINS = false>
 __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
                                               const ViewK &v, uint16_t *__restrict__ keys,
                                               uint64_t *__restrict__ mask, float4 *__restrict__ cov,
                                               uint16_t *__restrict__ tilehist, uint32_t *__restrict__ recomp_acc,
-                                              uint32_t chunk, const uint32_t *__restrict__ lut = nullptr,
-                                              uint32_t *__restrict__ binkey = nullptr) {
+                                              uint32_t chunk, const uint16_t *blk = nullptr) {
   const uint32_t tid = threadIdx.x;
   __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 512; k += 256) sm.h[k] = 0;
@@ -352,8 +356,6 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       b = rec[2 * (size_t)i + 1];
     }
   };
-  bool pend_keep = false;  // BINS: the previous tile's splat, whose table entry is in flight
-  uint32_t pend_i = 0xFFFFFFFFu, pend_val = 0;
   auto work = [&](uint32_t tile, const uint4 &a, const uint4 &b) {
     const uint32_t i = (tile << 8) + tid;
     const bool in = i < n;
@@ -362,7 +364,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
     const uint64_t bal = __ballot(p.keep);
     if ((tid & 63u) == 0 && in) mask[i >> 6] = bal;
     if (in) {
-      if (!BINS) __builtin_nontemporal_store((uint16_t)p.key, &keys[i]);
+      __builtin_nontemporal_store((uint16_t)p.key, &keys[i]);
       // Streaming stores for the keys and cov2d (the sort never reads cov2d back; the
       // keys are read once, by the next pass): lines that sit dirty in L2 at the end
       // of the launch are written back before the next launch starts.
@@ -371,11 +373,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
     }
     recomp += (p.recomputed && in) ? 1u : 0u;
     if (BINS) {
-      if (pend_i < n) __builtin_nontemporal_store(pend_val, &binkey[pend_i]);
-      if (pend_keep) atomicAdd(&sm.h[(pend_val >> 9) & 511u], 1u);
-      pend_keep = p.keep;
-      pend_i = in ? i : 0xFFFFFFFFu;
-      pend_val = lut[p.key];
+      if (p.keep) atomicAdd(&sm.h[bin_key(blk, p.key) >> 9], 1u);
     } else if (p.keep) {
       atomicAdd(&sm.h[p.key & 255u], 1u);
       atomicAdd(&sm.h[256u + (p.key >> 8)], 1u);
@@ -391,10 +389,6 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
     if (tile + 1 < end) work(tile + 1, a1, b1);
     load(tile + 4, a1, b1);
     if (tile + 2 < end) work(tile + 2, a2, b2);
-  }
-  if (BINS) {
-    if (pend_i < n) __builtin_nontemporal_store(pend_val, &binkey[pend_i]);
-    if (pend_keep) atomicAdd(&sm.h[(pend_val >> 9) & 511u], 1u);
   }
   if (recomp) atomicAdd(&sm.recomp, recomp);
   __syncthreads();

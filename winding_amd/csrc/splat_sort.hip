@@ -243,7 +243,6 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
   const ViewK vk = make_viewk(view);
   const bool std_form = is_std(view);
   st->last_n = n;
-  st->keys_kept = true;
   const uint32_t tiles = (n + kTile - 1) / kTile;
   const uint32_t supers = (tiles + kSuper - 1) / kSuper;
   const uint4 *rec = (const uint4 *)cloud->records;

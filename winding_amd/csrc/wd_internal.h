@@ -82,7 +82,7 @@ struct BatchJob {
   uint16_t *tilehist, *tilehist3;
   uint32_t *prefix_lo, *prefix_hi;
   // the learned sort's (unused by the exact path)
-  uint32_t *lut;          // [65536] key -> bin << 9 | the key's offset in its bin
+  uint16_t *lut;          // [4096] 16-key block -> (bin << 9 | the block's offset in its bin) >> 4
   uint32_t *prefix_bin;   // [tiles][512] where a tile's run of each bin starts
   const float *probs;     // [256] predicted bin widths (the predictor's output row)
   uint32_t cap;           // most survivors a bin may hold; more and the view runs the exact path
@@ -98,6 +98,8 @@ constexpr int kSuper = 1;
 // projection launch takes, and the largest cloud (indices packed in 23 bits).
 constexpr uint32_t kBins = 512;
 constexpr uint32_t kBinWidth = 512;
+constexpr uint32_t kBinBlock = 16;   // bin edges fall on multiples of 16 keys
+constexpr uint32_t kBinBlocks = 65536 / kBinBlock;
 constexpr uint32_t kMaxViews = 8;
 constexpr uint32_t kLearnedMaxSplats = 1u << 23;
 // One GEMM of the predictor on the matrix cores: C = alpha * A B, element (m, k)
@@ -192,7 +194,6 @@ struct wd_state {
   wd_context *ctx = nullptr;
   uint64_t capacity = 0;
   uint64_t last_n = 0;
-  bool keys_kept = true;       // false after a learned run, which writes no keys
   wd::Ctl *ctl = nullptr;      // device
   uint16_t *keys = nullptr;    // n keys, every splat
   uint64_t *mask = nullptr;    // survivor bits, ceil(n/64) words
@@ -202,7 +203,7 @@ struct wd_state {
   uint16_t *tilehist = nullptr;   // per 4096-splat chunk: 256 low-byte then 256 high-byte counts
   uint16_t *tilehist3 = nullptr;  // per tile of the low-byte pass's output: 256 high-byte counts
   uint32_t *prefix_lo = nullptr, *prefix_hi = nullptr;  // [tiles][256]: where a tile's digit run starts
-  uint32_t *lut = nullptr;                             // learned sort: key -> bin << 9 | offset
+  uint16_t *lut = nullptr;                             // learned sort: 16-key block -> bin and offset
   uint32_t *prefix_bin = nullptr;                      // learned sort: [tiles][512]
   uint64_t tiles = 0;
   uint32_t magic = 0x53544154;
