@@ -70,21 +70,32 @@ ViewK make_viewk(const wd_view &v);
 bool is_std(const wd_view &v);
 
 // One (cloud, view, state) of a batch, as the batch kernel reads it.
+// In device code the job table's pointers are global-memory pointers: read
+// out of a table, a plain pointer could point anywhere as far as the compiler
+// knows, and it would access it with flat instructions, which count against
+// the LDS wait counter too (every LDS wait would then also wait for them).
+// The host sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define WD_GLOBAL __attribute__((address_space(1)))
+#else
+#define WD_GLOBAL
+#endif
+
 struct BatchJob {
   ViewK view;
-  const void *rec;  // uint4 records
-  uint16_t *keys;
-  uint64_t *mask;
-  float *cov;
-  uint32_t *packed;
-  uint32_t *order;
-  Ctl *ctl;
-  uint16_t *tilehist, *tilehist3;
-  uint32_t *prefix_lo, *prefix_hi;
+  const WD_GLOBAL void *rec;  // uint4 records
+  WD_GLOBAL uint16_t *keys;
+  WD_GLOBAL uint64_t *mask;
+  WD_GLOBAL float *cov;
+  WD_GLOBAL uint32_t *packed;
+  WD_GLOBAL uint32_t *order;
+  WD_GLOBAL Ctl *ctl;
+  WD_GLOBAL uint16_t *tilehist, *tilehist3;
+  WD_GLOBAL uint32_t *prefix_lo, *prefix_hi;
   // the learned sort's (unused by the exact path)
-  uint16_t *lut;          // [4096] 16-key block -> (bin << 9 | the block's offset in its bin) >> 4
-  uint32_t *prefix_bin;   // [tiles][512] where a tile's run of each bin starts
-  const float *probs;     // [256] predicted bin widths (the predictor's output row)
+  WD_GLOBAL uint16_t *lut;          // [4096] 16-key block -> (bin << 9 | the block's offset in its bin) >> 4
+  WD_GLOBAL uint32_t *prefix_bin;   // [tiles][512] where a tile's run of each bin starts
+  const WD_GLOBAL float *probs;     // [256] predicted bin widths (the predictor's output row)
   uint32_t cap;           // most survivors a bin may hold; more and the view runs the exact path
   uint32_t n, std_form;
   uint32_t tiles;    // projection chunks = scatter tiles, of kTile splats
