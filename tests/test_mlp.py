@@ -89,6 +89,28 @@ def test_gpu_gradients_match_oracle(ctx):
 
 
 @pytest.mark.gpu
+def test_gpu_inference_matches_oracle_forward(ctx):
+    """The one-launch inference kernel (the learned sort's and predict's) against the oracle's forward with the
+    same bf16 rounding points: equal to fp32 summation order (an activation whose fp32 sum lands the other side of
+    a bf16 rounding edge moves by one bf16 step; 1% of a probability covers that)."""
+    import winding_amd as wa
+    rng = np.random.default_rng(8)
+    p, X, T = random_case(rng, n=40)
+    mlp = wa.Mlp(ctx, seed=3)
+    try:
+        mlp.set_params(p.astype(np.float32))
+        p32 = mlp.params().astype(np.float64)
+        P = mlp.predict(X.astype(np.float32)).astype(np.float64)
+        _, _, Po = mo.forward_backward(p32, X.astype(np.float32).astype(np.float64), T)
+        rel = np.abs(P - Po) / np.maximum(Po, 1e-12)
+        print("inference vs oracle: max rel %.3g, median rel %.3g" % (rel.max(), np.median(rel)), flush=True)
+        assert np.allclose(P.sum(axis=1), 1.0, atol=1e-5)
+        assert np.median(rel) < 1e-5 and np.allclose(P, Po, rtol=1e-2, atol=1e-7)
+    finally:
+        mlp.destroy()
+
+
+@pytest.mark.gpu
 def test_gpu_training_reduces_loss_and_saves(ctx):
     import winding_amd as wa
     rng = np.random.default_rng(4)

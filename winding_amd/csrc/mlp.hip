@@ -10,6 +10,10 @@
 //                   target distribution, and dZ = (p - t) / batch.
 //   k_colsum        bias gradients: column sums.
 //   k_adam          Adam on the fp32 master parameters, and their bf16 copy.
+//   k_mlp_infer     inference, one pose a workgroup in one launch (the
+//                   learned sort's, and wd_mlp_predict's): the same bf16
+//                   weights and bf16 activations as the matrix-core forward,
+//                   products exact in fp32, summed in fp32 in another order.
 #include <hip/hip_runtime.h>
 
 #include "wd_internal.h"
@@ -102,6 +106,62 @@ __global__ __launch_bounds__(256) void k_softmax_xent(const float *__restrict__ 
   }
 }
 
+// The whole forward for one pose: layers 1 and 2 by threads 0..127 (one
+// output each), layer 3 by all 256, the softmax across the workgroup.
+// Parameters: W1 [128][12] | b1 | W2 [128][128] | b2 | W3 [256][128] | b3,
+// bf16 weights (wb) and fp32 biases (w).
+constexpr int kMIn = WD_POSE_FLOATS, kMH = 128;
+constexpr int kOW1 = 0, kOB1 = kOW1 + kMH * kMIn, kOW2 = kOB1 + kMH, kOB2 = kOW2 + kMH * kMH, kOW3 = kOB2 + kMH,
+              kOB3 = kOW3 + WD_COARSE_BINS * kMH;
+static_assert(WD_COARSE_BINS == 256, "one output a thread");
+static_assert(kOW2 % 8 == 0 && kOW3 % 8 == 0, "16-byte rows");
+
+__device__ __forceinline__ float dot_row_bf16(const uint16_t *__restrict__ row, const float *h) {
+  // 128 bf16 weights, 16 bytes at a time, against 128 activations in LDS
+  const uint4 *q = reinterpret_cast<const uint4 *>(row);
+  float acc = 0.f;
+#pragma unroll 4
+  for (int j = 0; j < kMH / 8; ++j) {
+    const uint4 v = q[j];
+    const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      acc += __uint_as_float(u[e] << 16) * h[8 * j + 2 * e];
+      acc += __uint_as_float(u[e] & 0xFFFF0000u) * h[8 * j + 2 * e + 1];
+    }
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ X, const uint16_t *__restrict__ wb,
+                                                   const float *__restrict__ w, float *__restrict__ P) {
+  __shared__ float h1[kMH], h2[kMH], red[4];
+  const int row = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t < kMH) {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < kMIn; ++k) acc += from_bf16(wb[kOW1 + t * kMIn + k]) * from_bf16(to_bf16(X[row * kMIn + k]));
+    h1[t] = from_bf16(to_bf16(fmaxf(acc + w[kOB1 + t], 0.f)));
+  }
+  __syncthreads();
+  if (t < kMH) h2[t] = from_bf16(to_bf16(fmaxf(dot_row_bf16(wb + kOW2 + t * kMH, h1) + w[kOB2 + t], 0.f)));
+  __syncthreads();
+  const float z = dot_row_bf16(wb + kOW3 + t * kMH, h2) + w[kOB3 + t];
+  float mx = z;
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  if (lane == 0) red[wv] = mx;
+  __syncthreads();
+  mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  __syncthreads();
+  const float e = expf(z - mx);
+  float s = e;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  s = red[0] + red[1] + red[2] + red[3];
+  P[row * WD_COARSE_BINS + t] = e / s;
+}
+
 __global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ x, int rows, int cols, float *__restrict__ out) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= cols) return;
@@ -135,6 +195,12 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ w, ui
 int launch_gemm(wd_context *ctx, const MlpGemm &g) {
   hipLaunchKernelGGL(k_gemm, dim3((g.N + 15) / 16, (g.M + 15) / 16), dim3(64), 0, ctx->stream, g);
   WD_HIP(hipGetLastError(), "mlp: gemm launch");
+  return WD_OK;
+}
+
+int launch_mlp_infer(wd_context *ctx, const float *X, const uint16_t *wb, const float *w, int rows, float *P) {
+  hipLaunchKernelGGL(k_mlp_infer, dim3(rows), dim3(256), 0, ctx->stream, X, wb, w, P);
+  WD_HIP(hipGetLastError(), "mlp: inference launch");
   return WD_OK;
 }
 

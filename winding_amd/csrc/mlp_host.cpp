@@ -135,7 +135,7 @@ int mlp_infer(wd_context *ctx, wd_mlp *m, const float *poses_dev, uint32_t rows,
   if (rc) return rc;
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKMlp, &ev)) return WD_E_HIP;
-  if ((rc = forward(ctx, m, (int)rows, false, poses_dev))) return rc;
+  if ((rc = launch_mlp_infer(ctx, poses_dev, m->wb, m->w, (int)rows, m->P))) return rc;
   if (ctx->prof_end(kKMlp, ev)) return WD_E_HIP;
   *probs = m->P;
   return WD_OK;
@@ -310,7 +310,7 @@ int wd_mlp_predict(wd_context *ctx, wd_mlp *m, const float *poses, uint32_t n, f
   int rc = ensure_rows(m, n);
   if (rc) return rc;
   WD_HIP(hipMemcpyAsync(m->X, poses, (size_t)n * kIn * 4, hipMemcpyHostToDevice, ctx->stream), "mlp: upload");
-  if ((rc = forward(ctx, m, (int)n, false))) return rc;
+  if ((rc = wd::launch_mlp_infer(ctx, m->X, m->wb, m->w, (int)n, m->P))) return rc;
   WD_HIP(hipStreamSynchronize(ctx->stream), "mlp: predict");
   WD_HIP(hipMemcpy(probs, m->P, (size_t)n * kOut * 4, hipMemcpyDeviceToHost), "mlp: predict");
   return WD_OK;

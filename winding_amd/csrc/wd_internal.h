@@ -246,6 +246,8 @@ int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid);
 int launch_copy(wd_context *ctx, const void *src, void *dst, size_t bytes);
 // Predictor kernels (mlp.hip).
 int launch_gemm(wd_context *ctx, const MlpGemm &g);
+// The predictor's forward for `rows` poses in one launch: P[rows][256].
+int launch_mlp_infer(wd_context *ctx, const float *X, const uint16_t *wb, const float *w, int rows, float *P);
 int launch_softmax_xent(wd_context *ctx, const float *z, const float *target, int rows, float scale, float *p, float *dz,
                         float *loss);
 int launch_colsum(wd_context *ctx, const float *x, int rows, int cols, float *out);
