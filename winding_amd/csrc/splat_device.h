@@ -86,14 +86,10 @@ __device__ __forceinline__ uint64_t match_bits(uint32_t d, uint64_t valid) {
 #pragma unroll
   for (int b = 0; b < BITS; ++b) {
     const uint32_t s = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
-    const uint64_t bal = __ballot(s != 0u);
-    if (b == 0) {
-      dlo = (uint32_t)bal ^ s;
-      dhi = (uint32_t)(bal >> 32) ^ s;
-    } else {
-      dlo = __builtin_amdgcn_bitop3_b32(dlo, (uint32_t)bal, s, 0xF6);
-      dhi = __builtin_amdgcn_bitop3_b32(dhi, (uint32_t)(bal >> 32), s, 0xF6);
-    }
+    // the ballot as a sign test of the spread bit: one compare on s itself
+    const uint64_t bal = __builtin_amdgcn_ballot_w64((int)s < 0);
+    dlo = __builtin_amdgcn_bitop3_b32(dlo, (uint32_t)bal, s, 0xF6);
+    dhi = __builtin_amdgcn_bitop3_b32(dhi, (uint32_t)(bal >> 32), s, 0xF6);
   }
   return ((uint64_t)(~dhi & (uint32_t)(valid >> 32)) << 32) | (~dlo & (uint32_t)valid);
 }
