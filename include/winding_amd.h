@@ -125,6 +125,22 @@ int wd_cloud_info(const wd_cloud *cloud, uint64_t *count, float min_out[3],
 int wd_cloud_records(const wd_cloud *cloud, const void **device_records);
 int wd_cloud_destroy(wd_cloud *cloud);
 
+/*
+ * Spherical harmonics for a cloud, degree 1 to 3 (0 removes them), in the
+ * reference's `sh` layout (src/scene/splats.js:306-332): for each splat,
+ * ceil(3 n / 2) 32-bit words, n = 3, 8 or 15 coefficients past the constant
+ * one, coefficient by coefficient, red, green and blue as IEEE half floats,
+ * two a word, the lower half first. `words` holds at least count of those.
+ * From then on wd_sort writes, as each survivor's colour, what the reference's
+ * shade pass computes (src/render/splats.js:315-381): the record's colour plus
+ * the harmonics along the direction from the eye (the inverse model-view's
+ * translation, as SplatPass.prepare sets it, :580), clamped to [0, 1] and
+ * packed as unorm bytes, alpha kept; in f32, within one unit of the byte of
+ * the FP64 evaluation. Batches and learned runs refuse such a cloud.
+ */
+int wd_cloud_sh(wd_context *ctx, wd_cloud *cloud, const uint32_t *words,
+                uint64_t n_words, uint32_t degree);
+
 /* ---- the per-view uniform: SplatPass.prepare, src/render/splats.js:559-587 */
 
 /*

@@ -49,6 +49,13 @@ def lib():
         L.or_look_at.argtypes = [P, P, P, P]
         L.or_perspective.argtypes = [P, C.c_double, C.c_double, C.c_double]
         L.or_mat4_multiply.argtypes = [P, P, P]
+        L.or_eye.restype = C.c_int
+        L.or_eye.argtypes = [P, P]
+        L.or_sh_words.restype = C.c_int
+        L.or_sh_words.argtypes = [C.c_int]
+        L.or_shade.restype = C.c_uint32
+        L.or_shade.argtypes = [P, P, C.c_int, P]
+        L.or_shade_all.argtypes = [P, U64, P, C.c_int, P, P]
         _lib = L
     return _lib
 
@@ -129,4 +136,28 @@ def mat4_multiply(a, b):
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)
     lib().or_mat4_multiply(out.ctypes.data, a.ctypes.data, b.ctypes.data)
+    return out
+
+
+def eye(model_view):
+    """The eye in the capture's space (SplatPass.prepare, src/render/splats.js:580), f32 x 3."""
+    mv = np.ascontiguousarray(np.asarray(model_view, np.float32).reshape(16))
+    out = np.zeros(3, np.float32)
+    lib().or_eye(mv.ctypes.data, out.ctypes.data)
+    return out
+
+
+def sh_words(degree):
+    return lib().or_sh_words(degree)
+
+
+def shade(records, sh, degree, view):
+    """The shade pass's colour (rgba bytes as u32) of every splat, FP64 (src/render/splats.js:335-381)."""
+    records = np.ascontiguousarray(records, np.uint8)
+    sh = np.ascontiguousarray(sh, np.uint32)
+    n = records.nbytes // 32
+    assert sh.size >= n * sh_words(degree)
+    e = eye(to_oview(view).model_view)
+    out = np.zeros(n, np.uint32)
+    lib().or_shade_all(records.ctypes.data, n, sh.ctypes.data, degree, e.ctypes.data, out.ctypes.data)
     return out

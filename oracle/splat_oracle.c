@@ -31,6 +31,14 @@
  *                 src/core/math/mat4.js:96-126
  *   or_look_at, or_perspective  mat4LookAt / mat4PerspectiveReverseZInfinite
  *                 src/core/math/mat4.js:243-273,455-468
+ *   or_eye        the eye in a capture's space, as SplatPass.prepare sets it
+ *                 (src/render/splats.js:580): the translation of mat4Invert
+ *                 (src/core/math/mat4.js:186-228) of the f32 model-view, f64
+ *                 maths stored as f32
+ *   or_shade      the shade pass's colour of one splat with harmonics,
+ *                 src/render/splats.js:335-381 (coefficient, harmonics,
+ *                 shade), halves decoded exactly, in FP64; the `sh` layout
+ *                 of src/scene/splats.js:306-332
  *
  * Pinned by: tests/test_oracle.py (known answers from test/splats.test.js:51-71,
  * test/core.test.js:214-223,343-360, test/render.test.js:428-440 and
@@ -286,6 +294,89 @@ void or_look_at(float out[16], const double eye[3], const double center[3], cons
   out[13] = (float)(-(yx * eye[0] + yy * eye[1] + yz * eye[2]));
   out[14] = (float)(-(zx * eye[0] + zy * eye[1] + zz * eye[2]));
   out[15] = 1;
+}
+
+/* The eye: the translation column of mat4Invert(mv) (src/core/math/mat4.js:186-228), cofactors in f64 in
+ * the reference's expression order, stored as f32. Returns 1 (eye left at the origin) for a singular matrix. */
+int or_eye(const float m[16], float eye[3]) {
+  double a00 = m[0], a01 = m[1], a02 = m[2], a03 = m[3], a10 = m[4], a11 = m[5], a12 = m[6], a13 = m[7];
+  double a20 = m[8], a21 = m[9], a22 = m[10], a23 = m[11], a30 = m[12], a31 = m[13], a32 = m[14], a33 = m[15];
+  double b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10, b02 = a00 * a13 - a03 * a10;
+  double b03 = a01 * a12 - a02 * a11, b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+  double b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30, b08 = a20 * a33 - a23 * a30;
+  double b09 = a21 * a32 - a22 * a31, b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+  double det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+  eye[0] = eye[1] = eye[2] = 0;
+  if (det == 0) return 1;
+  det = 1 / det;
+  eye[0] = (float)((a11 * b07 - a10 * b09 - a12 * b06) * det);
+  eye[1] = (float)((a00 * b09 - a01 * b07 + a02 * b06) * det);
+  eye[2] = (float)((a31 * b01 - a30 * b03 - a32 * b00) * det);
+  return 0;
+}
+
+/* An IEEE half's value, exactly. */
+static double half_value(uint16_t h) {
+  int e = (h >> 10) & 31, m = h & 1023;
+  double v = e == 0 ? ldexp((double)m, -24) : e == 31 ? (m ? NAN : INFINITY) : ldexp((double)(1024 + m), e - 25);
+  return (h & 0x8000) ? -v : v;
+}
+
+static const int OR_SH_COEFFICIENTS[4] = {0, 3, 8, 15};
+
+/* Words a splat's harmonics take (shWords, src/scene/splats.js:47). */
+int or_sh_words(int degree) { return (OR_SH_COEFFICIENTS[degree] * 3 + 1) / 2; }
+
+/* coefficient(): the three halves of coefficient k of a splat whose words start at w (src/render/splats.js:338-344). */
+static void coefficient(const uint32_t *w, int k, double c[3]) {
+  int h = k * 3;
+  uint32_t a = w[h >> 1], b = w[(h >> 1) + 1];
+  if (h & 1) {
+    c[0] = half_value((uint16_t)(a >> 16)); c[1] = half_value((uint16_t)b); c[2] = half_value((uint16_t)(b >> 16));
+  } else {
+    c[0] = half_value((uint16_t)a); c[1] = half_value((uint16_t)(a >> 16)); c[2] = half_value((uint16_t)b);
+  }
+}
+
+/* The shade pass for one splat (src/render/splats.js:346-381), FP64, the f32 literals widened: colour bytes,
+ * alpha kept. rec: the 32-byte record; w: the splat's sh_words(degree) words; eye as or_eye gives it. */
+uint32_t or_shade(const uint8_t *rec, const uint32_t *w, int degree, const float eye[3]) {
+  double dx = (double)rd_f32(rec) - eye[0], dy = (double)rd_f32(rec + 4) - eye[1], dz = (double)rd_f32(rec + 8) - eye[2];
+  double len = sqrt(dx * dx + dy * dy + dz * dz);
+  double x = dx / len, y = dy / len, z = dz / len;
+  double c[3], k[15][3];
+  for (int j = 0; j < OR_SH_COEFFICIENTS[degree]; j++) coefficient(w, j, k[j]);
+  for (int ch = 0; ch < 3; ch++) c[ch] = (double)0.4886025119029199f * (-y * k[0][ch] + z * k[1][ch] - x * k[2][ch]);
+  if (degree >= 2) {
+    double xx = x * x, yy = y * y, zz = z * z;
+    for (int ch = 0; ch < 3; ch++)
+      c[ch] += (double)1.0925484305920792f * x * y * k[3][ch] - (double)1.0925484305920792f * y * z * k[4][ch] +
+               (double)0.31539156525252005f * (2.0 * zz - xx - yy) * k[5][ch] -
+               (double)1.0925484305920792f * x * z * k[6][ch] + (double)0.5462742152960396f * (xx - yy) * k[7][ch];
+    if (degree >= 3)
+      for (int ch = 0; ch < 3; ch++)
+        c[ch] += -(double)0.5900435899266435f * y * (3.0 * xx - yy) * k[8][ch] +
+                 (double)2.890611442640554f * x * y * z * k[9][ch] -
+                 (double)0.4570457994644658f * y * (4.0 * zz - xx - yy) * k[10][ch] +
+                 (double)0.3731763325901154f * z * (2.0 * zz - 3.0 * xx - 3.0 * yy) * k[11][ch] -
+                 (double)0.4570457994644658f * x * (4.0 * zz - xx - yy) * k[12][ch] +
+                 (double)1.445305721320277f * z * (xx - yy) * k[13][ch] -
+                 (double)0.5900435899266435f * x * (xx - 3.0 * yy) * k[14][ch];
+  }
+  uint32_t out = (uint32_t)rec[27] << 24;
+  for (int ch = 0; ch < 3; ch++) {
+    double seen = rec[24 + ch] / 255.0 + c[ch];
+    seen = seen < 0 ? 0 : seen > 1 ? 1 : seen;
+    out |= (uint32_t)floor(0.5 + 255.0 * seen) << (8 * ch);
+  }
+  return out;
+}
+
+/* or_shade for splats 0..n-1 of a cloud (sh: sh_words(degree) words a splat). */
+void or_shade_all(const uint8_t *records, uint64_t n, const uint32_t *sh, int degree, const float eye[3],
+                  uint32_t *out) {
+  int W = or_sh_words(degree);
+  for (uint64_t i = 0; i < n; i++) out[i] = or_shade(records + 32 * i, sh + (uint64_t)W * i, degree, eye);
 }
 
 /* mat4PerspectiveReverseZInfinite: src/core/math/mat4.js:455-468. */

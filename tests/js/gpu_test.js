@@ -155,5 +155,33 @@ test('learned sort: a predictor trained on the exact sort, then 4 views, each eq
   splats.destroy();
 });
 
+test('a capture with harmonics changes colour with the view, in its own space (test/gpu.test.js:1968-2035)', () => {
+  // A grey splat whose only harmonic is red's third coefficient, the -x term, at 0.6 (half 0x38cd).
+  const bytes = record([0, 0, 0], [0.4, 0.4, 0.4], [128, 128, 128, 255], [255, 128, 128, 128]);
+  const sh = new Uint32Array(wd.shWords(1));
+  sh[3] = 0x38cd;  // halves: coefficient k, channel c at 3k + c; red of k = 2 is half 6, word 3's low half
+  const splats = new wd.Splats(gpu, { count: 1, degree: 1, records: bytes, sh });
+  const colourFrom = (x, turned) => {
+    const scene = new wd.SplatScene();
+    const entity = scene.addSplats({ splats, matrix: turned ? [-1, 0, 0, 0, 0, 1, 0, 0, 0, 0, -1, 0, 0, 0, 0, 1] : undefined });
+    const cam = new wd.Camera({ fovY: 0.8, near: 0.1 });
+    cam.position.set([x, 0, 0]);
+    cam.update(160 / 120);
+    const pass = new wd.SplatPass(gpu);
+    pass.cull = 0;
+    pass.prepare(scene, cam, null, 160, 120);
+    pass.sort();
+    const got = pass.read(entity, { cov2d: true });
+    pass.destroy();
+    const bits = new Uint32Array(got.cov2d.buffer, got.cov2d.byteOffset, 4)[3];
+    return [bits & 255, (bits >> 8) & 255, (bits >> 16) & 255];
+  };
+  const plusX = colourFrom(5, false), minusX = colourFrom(-5, false), turned = colourFrom(-5, true);
+  splats.destroy();
+  assert.ok(plusX[0] > plusX[1] + 40 && minusX[0] < minusX[1] - 40, plusX + ' / ' + minusX);
+  assert.ok(Math.abs(plusX[1] - minusX[1]) <= 2 && Math.abs(turned[0] - plusX[0]) <= 2, turned + ' / ' + plusX);
+  console.log('      from +x ' + plusX + ', from -x ' + minusX + ', turned ' + turned);
+});
+
 gpu.destroy();
 console.log(passed + ' checks passed');

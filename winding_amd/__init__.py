@@ -24,7 +24,7 @@ WD_POSE_FLOATS = 12
 EXPORTS = [
     "wd_last_error", "wd_open", "wd_close", "wd_sync", "wd_stream", "wd_capacity",
     "wd_cloud_create", "wd_cloud_create_device", "wd_cloud_info", "wd_cloud_records", "wd_cloud_destroy",
-    "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
+    "wd_cloud_sh", "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
     "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_profile", "wd_profile_read",
     "wd_batch_create", "wd_batch_schedule", "wd_batch_tune", "wd_batch_set", "wd_batch_run", "wd_batch_stuck",
     "wd_batch_destroy",
@@ -95,6 +95,7 @@ def lib():
             "wd_cloud_create_device": (I, [P, P, U64, C.POINTER(P), FP, FP]),
             "wd_cloud_info": (I, [P, C.POINTER(U64), FP, FP]),
             "wd_cloud_records": (I, [P, C.POINTER(P)]),
+            "wd_cloud_sh": (I, [P, P, P, U64, U32]),
             "wd_cloud_destroy": (I, [P]),
             "wd_view_prepare": (I, [FP, FP, FP, C.c_float, C.c_float, FP, FP, C.c_float, C.POINTER(View)]),
             "wd_state_create": (I, [P, U64, C.POINTER(P)]),
@@ -279,6 +280,12 @@ class Cloud:
         if self.ptr:
             check(lib().wd_cloud_destroy(self.ptr))
             self.ptr = None
+
+    def set_sh(self, words, degree):
+        """Spherical harmonics of degree 1-3 in the reference's `sh` layout (u32 words, halves), 0 to remove."""
+        words = np.ascontiguousarray(words, dtype=np.uint32) if degree else np.zeros(1, np.uint32)
+        check(lib().wd_cloud_sh(self.ctx.ptr, self.ptr, words.ctypes.data, words.size if degree else 0, degree))
+        self.degree = degree
 
     def canonical(self, view):
         out = np.zeros((self.count, 5), np.float64)

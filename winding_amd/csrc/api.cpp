@@ -214,6 +214,28 @@ int wd_cloud_info(const wd_cloud *cloud, uint64_t *count, float min_out[3], floa
   return WD_OK;
 }
 
+int wd_cloud_sh(wd_context *ctx, wd_cloud *cloud, const uint32_t *words, uint64_t n_words, uint32_t degree) {
+  if (!ctx || !cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "splats: these splats were unloaded");
+  if (cloud->ctx != ctx) return fail(WD_E_STATE, "splats: of another device");
+  if (degree > 3) return fail(WD_E_ARG, "splats: harmonics of degree %u; this path takes 0 to 3", degree);
+  const uint64_t need = degree ? cloud->n * wd::sh_words(degree) : 0;
+  if (degree && (!words || n_words < need))
+    return fail(WD_E_ARG, "splats: harmonics of degree %u take %llu words for %llu splats, got %llu", degree,
+                (unsigned long long)need, (unsigned long long)cloud->n, (unsigned long long)n_words);
+  WD_HIP(hipSetDevice(ctx->device), "splats: set device");
+  WD_HIP(hipStreamSynchronize(ctx->stream), "splats: harmonics");  // no sort in flight reads the old ones
+  if (cloud->sh) hipFree(cloud->sh);
+  cloud->sh = nullptr;
+  cloud->degree = 0;
+  if (!degree) return WD_OK;
+  // one word over, as the reference allocates: a two-word read of the last coefficient never falls off
+  WD_HIP(hipMalloc(&cloud->sh, (need + 1) * sizeof(uint32_t)), "splats: harmonics allocation");
+  WD_HIP(hipMemcpy(cloud->sh, words, need * sizeof(uint32_t), hipMemcpyHostToDevice), "splats: harmonics upload");
+  WD_HIP(hipMemset(cloud->sh + need, 0, sizeof(uint32_t)), "splats: harmonics upload");
+  cloud->degree = degree;
+  return WD_OK;
+}
+
 int wd_cloud_records(const wd_cloud *cloud, const void **device_records) {
   if (!cloud || cloud->magic != 0x53504c54 || !device_records)
     return fail(WD_E_STATE, "splats: these splats were unloaded");
@@ -226,6 +248,7 @@ int wd_cloud_destroy(wd_cloud *cloud) {
   hipSetDevice(cloud->ctx->device);
   hipStreamSynchronize(cloud->ctx->stream);
   hipFree(cloud->records);
+  if (cloud->sh) hipFree(cloud->sh);
   cloud->ctx->live.erase(cloud);
   cloud->magic = 0;
   delete cloud;

@@ -91,6 +91,7 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
     return fail(WD_E_ARG, "learned: %u views; this one takes 1 to %u", n_views, l->max_views);
   if (!cloud || !ctx->live.count(cloud)) return fail(WD_E_STATE, "learned: the splats were unloaded");
   if (cloud->ctx != ctx) return fail(WD_E_STATE, "learned: the splats are of another device");
+  if (cloud->degree) return fail(WD_E_ARG, "learned: the splats have harmonics, which only a single sort (wd_sort) shades");
   const uint32_t n = (uint32_t)cloud->n;
   if (cloud->n > wd::kLearnedMaxSplats)
     return fail(WD_E_CAPACITY, "learned: %llu splats are more than a learned sort takes, %u",

@@ -322,15 +322,86 @@ struct ProjectSmem {
   uint32_t recomp;
 };
 
+// ---------------------------------------------------------------------------
+// The reference's shade pass for one survivor (src/render/splats.js:315-381),
+// fused into the projection: the record's colour plus the harmonics past the
+// constant term along the direction from the eye to the splat, clamped to
+// [0, 1] and packed as unorm bytes, floor(0.5 + 255 x) (pack4x8unorm), the
+// alpha byte kept. f32 in the reference's expression order (no contraction);
+// the coefficients are IEEE halves, SHD = 1, 2, 3 (3, 8, 15 coefficients a
+// channel), sh_words(SHD) words a splat, coefficient by coefficient, red,
+// green and blue.
+struct Rgb {
+  float r, g, b;
+};
+
+template <int SHD>
+__device__ __forceinline__ uint32_t shade(const ViewK &v, const uint32_t *__restrict__ sh, uint32_t i, float px,
+                                          float py, float pz, uint32_t rgba) {
+  constexpr uint32_t W = sh_words(SHD);
+  const uint32_t *s = sh + (size_t)i * W;
+  uint32_t w[W];
+#pragma unroll
+  for (uint32_t k = 0; k < W; ++k) w[k] = s[k];
+  auto half = [](uint32_t bits) { return (float)__builtin_bit_cast(_Float16, (uint16_t)bits); };
+  // coefficient k: three halves from the two words they lie across
+  auto coef = [&](uint32_t k) -> Rgb {
+    const uint32_t h = k * 3u, a = w[h >> 1], b = w[(h >> 1) + 1];
+    if (h & 1u) return Rgb{half(a >> 16), half(b & 0xFFFFu), half(b >> 16)};
+    return Rgb{half(a & 0xFFFFu), half(a >> 16), half(b & 0xFFFFu)};
+  };
+  const float dx = px - v.eye[0], dy = py - v.eye[1], dz = pz - v.eye[2];
+  const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float x = dx / len, y = dy / len, z = dz / len;
+  Rgb c;
+  {
+    const Rgb c0 = coef(0), c1 = coef(1), c2 = coef(2);
+    const float k = 0.4886025119029199f;
+    c.r = k * (-y * c0.r + z * c1.r - x * c2.r);
+    c.g = k * (-y * c0.g + z * c1.g - x * c2.g);
+    c.b = k * (-y * c0.b + z * c1.b - x * c2.b);
+  }
+  if (SHD >= 2) {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float a3 = 1.0925484305920792f * x * y, a4 = 1.0925484305920792f * y * z;
+    const float a5 = 0.31539156525252005f * (2.0f * zz - xx - yy), a6 = 1.0925484305920792f * x * z;
+    const float a7 = 0.5462742152960396f * (xx - yy);
+    const Rgb c3 = coef(3), c4 = coef(4), c5 = coef(5), c6 = coef(6), c7 = coef(7);
+    c.r += a3 * c3.r - a4 * c4.r + a5 * c5.r - a6 * c6.r + a7 * c7.r;
+    c.g += a3 * c3.g - a4 * c4.g + a5 * c5.g - a6 * c6.g + a7 * c7.g;
+    c.b += a3 * c3.b - a4 * c4.b + a5 * c5.b - a6 * c6.b + a7 * c7.b;
+    if (SHD >= 3) {
+      const float a8 = -0.5900435899266435f * y * (3.0f * xx - yy), a9 = 2.890611442640554f * x * y * z;
+      const float a10 = 0.4570457994644658f * y * (4.0f * zz - xx - yy);
+      const float a11 = 0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+      const float a12 = 0.4570457994644658f * x * (4.0f * zz - xx - yy), a13 = 1.445305721320277f * z * (xx - yy);
+      const float a14 = 0.5900435899266435f * x * (xx - 3.0f * yy);
+      const Rgb c8 = coef(8), c9 = coef(9), c10 = coef(10), c11 = coef(11), c12 = coef(12), c13 = coef(13),
+                c14 = coef(14);
+      c.r += a8 * c8.r + a9 * c9.r - a10 * c10.r + a11 * c11.r - a12 * c12.r + a13 * c13.r - a14 * c14.r;
+      c.g += a8 * c8.g + a9 * c9.g - a10 * c10.g + a11 * c11.g - a12 * c12.g + a13 * c13.g - a14 * c14.g;
+      c.b += a8 * c8.b + a9 * c9.b - a10 * c10.b + a11 * c11.b - a12 * c12.b + a13 * c13.b - a14 * c14.b;
+    }
+  }
+  auto byte = [](uint32_t word, int k, float add) {
+    const float base = (float)((word >> (8 * k)) & 255u) / 255.0f;
+    const float seen = fminf(fmaxf(base + add, 0.0f), 1.0f);
+    return (uint32_t)floorf(0.5f + 255.0f * seen);
+  };
+  return byte(rgba, 0, c.r) | (byte(rgba, 1, c.g) << 8) | (byte(rgba, 2, c.b) << 16) | (rgba & 0xFF000000u);
+}
+
 // BINS (the learned sort): the chunk's histogram is over the view's 512 bins
 // instead of over the two key bytes; a survivor's bin comes from the view's
 // block table, already in LDS (bin_key).
-template <bool STD, bool BINS = false>
+// SHD > 0: the survivors' colour is shaded by the cloud's harmonics (shade).
+template <bool STD, bool BINS = false, int SHD = 0>
 __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
                                               const ViewK &v, uint16_t *__restrict__ keys,
                                               uint64_t *__restrict__ mask, float4 *__restrict__ cov,
                                               uint16_t *__restrict__ tilehist, uint32_t *__restrict__ recomp_acc,
-                                              uint32_t chunk, const uint16_t *blk = nullptr) {
+                                              uint32_t chunk, const uint16_t *blk = nullptr,
+                                              const uint32_t *__restrict__ sh = nullptr) {
   const uint32_t tid = threadIdx.x;
   __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 512; k += 256) sm.h[k] = 0;
@@ -364,7 +435,10 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       // Streaming stores for the keys and cov2d (the sort never reads cov2d back; the
       // keys are read once, by the next pass): lines that sit dirty in L2 at the end
       // of the launch are written back before the next launch starts.
-      const f32x4 c = p.keep ? f32x4{p.xx, p.xy, p.yy, __uint_as_float(b.z)} : f32x4{0.f, 0.f, 0.f, 0.f};
+      uint32_t colour = b.z;
+      if constexpr (SHD > 0)
+        if (p.keep) colour = shade<SHD>(v, sh, i, f32_of(a.x), f32_of(a.y), f32_of(a.z), b.z);
+      const f32x4 c = p.keep ? f32x4{p.xx, p.xy, p.yy, __uint_as_float(colour)} : f32x4{0.f, 0.f, 0.f, 0.f};
       __builtin_nontemporal_store(c, (f32x4 *)&cov[i]);
     }
     recomp += (p.recomputed && in) ? 1u : 0u;

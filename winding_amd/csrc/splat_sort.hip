@@ -37,14 +37,16 @@
 namespace wd {
 namespace {
 
-// Pass 1: a workgroup a chunk of kTile splats.
-template <bool STD>
+// Pass 1: a workgroup a chunk of kTile splats. SHD > 0: a cloud with
+// harmonics of that degree, each survivor's colour shaded.
+template <bool STD, int SHD>
 __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, uint32_t n, ViewK v,
                                                  uint16_t *__restrict__ keys, uint64_t *__restrict__ mask,
                                                  float4 *__restrict__ cov, uint16_t *__restrict__ tilehist,
-                                                 Ctl *__restrict__ ctl) {
+                                                 Ctl *__restrict__ ctl, const uint32_t *__restrict__ sh) {
   __shared__ ProjectSmem sm;
-  project_chunk<STD>(sm, rec, n, v, keys, mask, cov, tilehist, &ctl->recomputed_acc, blockIdx.x);
+  project_chunk<STD, false, SHD>(sm, rec, n, v, keys, mask, cov, tilehist, &ctl->recomputed_acc, blockIdx.x,
+                                 nullptr, sh);
 }
 
 // Scan of pass 2: low-byte tile prefixes, and both digits' totals. Also moves
@@ -220,6 +222,31 @@ ViewK make_viewk(const wd_view &v) {
   k.ddmin = (double)v.depth_min;
   k.ddscale = (double)v.depth_scale;
   k.dcull = (double)v.cull;
+  // The eye in the capture's space: the translation of the model-view's
+  // inverse, in FP64 from the f32 matrix and stored as f32, as
+  // SplatPass.prepare writes it (src/render/splats.js:580) with mat4Invert's
+  // cofactor expansion (src/core/math/mat4.js:186-228). A singular model-view
+  // leaves it at the origin.
+  const float *m = v.model_view;
+  const double a00 = m[0], a01 = m[1], a02 = m[2], a03 = m[3], a10 = m[4], a11 = m[5], a12 = m[6], a13 = m[7];
+  const double a20 = m[8], a21 = m[9], a22 = m[10], a23 = m[11], a30 = m[12], a31 = m[13], a32 = m[14], a33 = m[15];
+  const double b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10, b02 = a00 * a13 - a03 * a10;
+  const double b03 = a01 * a12 - a02 * a11, b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+  const double b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30, b08 = a20 * a33 - a23 * a30;
+  const double b09 = a21 * a32 - a22 * a31, b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+  double det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+  k.eye[0] = k.eye[1] = k.eye[2] = 0.f;
+  if (det != 0) {
+    det = 1 / det;
+    const double e0 = (a11 * b07 - a10 * b09 - a12 * b06) * det;
+    const double e1 = (a00 * b09 - a01 * b07 + a02 * b06) * det;
+    const double e2 = (a31 * b01 - a30 * b03 - a32 * b00) * det;
+    if (std::isfinite(e0) && std::isfinite(e1) && std::isfinite(e2)) {
+      k.eye[0] = (float)e0;
+      k.eye[1] = (float)e1;
+      k.eye[2] = (float)e2;
+    }
+  }
   return k;
 }
 
@@ -254,12 +281,15 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
     WD_HIP(hipGetLastError(), what);              \
     if (ctx->prof_end(id, ev)) return WD_E_HIP;   \
   } while (0)
-  if (std_form)
-    WD_LAUNCH(kKProject, "sort: project launch", k_project<true>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk,
-              st->keys, st->mask, (float4 *)st->cov, st->tilehist, st->ctl);
-  else
-    WD_LAUNCH(kKProject, "sort: project launch", k_project<false>, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk,
-              st->keys, st->mask, (float4 *)st->cov, st->tilehist, st->ctl);
+  // the projection's form: the fast reverse-Z one or the general one, times the harmonics' degree
+  using ProjectKernel = void (*)(const uint4 *, uint32_t, ViewK, uint16_t *, uint64_t *, float4 *, uint16_t *, Ctl *,
+                                 const uint32_t *);
+  static const ProjectKernel kForms[2][4] = {
+      {k_project<false, 0>, k_project<false, 1>, k_project<false, 2>, k_project<false, 3>},
+      {k_project<true, 0>, k_project<true, 1>, k_project<true, 2>, k_project<true, 3>}};
+  const ProjectKernel project = kForms[std_form ? 1 : 0][cloud->degree <= 3 ? cloud->degree : 0];
+  WD_LAUNCH(kKProject, "sort: project launch", project, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk, st->keys,
+            st->mask, (float4 *)st->cov, st->tilehist, st->ctl, (const uint32_t *)cloud->sh);
   WD_LAUNCH(kKScan, "sort: scan launch", k_scan, dim3(512 / kScanCols), dim3(256), 0, ctx->stream, tiles,
             st->tilehist, st->prefix_lo, st->ctl);
   WD_LAUNCH(kKScatterLo, "sort: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0, ctx->stream,

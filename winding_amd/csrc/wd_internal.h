@@ -61,6 +61,7 @@ struct ViewK {
   float limx, limy;   // 1.3 / P00, 1.3 / P11 (f32)
   float n0, n1;       // row norms of T's bound (see k_project)
   int persp;          // P[15] == 0
+  float eye[3];       // the eye in the capture's space, for the harmonics
 };
 
 // Builds ViewK from a wd_view (host).
@@ -175,8 +176,17 @@ struct wd_cloud {
   uint8_t *records = nullptr;  // device, n * 32 bytes
   uint64_t n = 0;
   float min[3] = {0, 0, 0}, max[3] = {0, 0, 0};
+  uint32_t *sh = nullptr;       // device: spherical harmonics, sh_words(degree) words a splat (+1)
+  uint32_t degree = 0;          // 0: none
   uint32_t magic = 0x53504c54;  // cleared by destroy
 };
+
+namespace wd {
+// Coefficients a colour channel has past the constant one, by degree, and the
+// 32-bit words a splat's three channels of half floats take.
+constexpr uint32_t kShCoefficients[4] = {0, 3, 8, 15};
+constexpr uint32_t sh_words(uint32_t degree) { return (kShCoefficients[degree] * 3 + 1) / 2; }
+}  // namespace wd
 
 struct wd_batch {
   wd_context *ctx = nullptr;

@@ -5,7 +5,7 @@ const require = createRequire(import.meta.url);
 const api = require('./index.js');
 
 export const {
-  SPLAT_BYTES, BUCKETS, SPLAT_CULL, CLOUD_FLOATS, KEEP_FRAMES,
+  SPLAT_BYTES, BUCKETS, SPLAT_CULL, CLOUD_FLOATS, KEEP_FRAMES, SH_COEFFICIENTS, shWords,
   native, Gpu, parseSplats, Splats, SplatScene, SplatPass, cloudUniform,
   mat4Create, mat4Multiply, mat4LookAt, mat4PerspectiveReverseZInfinite, mat4OrthographicReverseZ, Camera,
   POSE_FLOATS, BINS, Predictor, LearnedSort, poseFeatures, trainingSet,

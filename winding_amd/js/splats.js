@@ -13,6 +13,10 @@
 const path = require('path');
 
 const SPLAT_BYTES = 32;
+/** Coefficients a colour channel has past the constant one, by degree (src/scene/splats.js:43-47). */
+const SH_COEFFICIENTS = [0, 3, 8, 15];
+/** 32-bit words a splat's harmonics take: three channels of half floats, packed. */
+const shWords = (degree) => Math.ceil(SH_COEFFICIENTS[degree] * 3 / 2);
 const BUCKETS = 65536;
 /** Opacity x pixels a splat must add to be drawn (src/render/splats.js:78). */
 const SPLAT_CULL = 0.5;
@@ -82,13 +86,23 @@ function parseSplats(bytes) {
  */
 class Splats {
   constructor(gpu, data, label) {
-    if (data.degree > 0) throw new Error('splats: harmonics of degree ' + data.degree + ' are not drawn by this path');
+    const degree = data.degree || 0;
+    if (!(degree >= 0 && degree <= 3)) throw new Error('splats: harmonics of degree ' + degree + '; this path takes 0 to 3');
+    if (degree > 0 && !(data.sh instanceof Uint32Array)) throw new Error('splats: harmonics of degree ' + degree + ' need data.sh (Uint32Array)');
     const made = gpu.native.cloudCreate(gpu.handle, data.records);
+    if (degree > 0) {
+      try {
+        gpu.native.cloudSH(gpu.handle, made.handle, data.sh, degree);
+      } catch (error) {
+        gpu.native.destroy(made.handle);
+        throw error;
+      }
+    }
     this.gpu = gpu;
     this.label = label === undefined ? 'splats' : label;
     this.handle = made.handle;
     this.count = made.count;
-    this.degree = 0;
+    this.degree = degree;
     this.min = made.min;
     this.max = made.max;
     this.unloaded = false;
@@ -241,17 +255,25 @@ class SplatPass {
   sort() {
     const marked = this._items.filter((i) => i.sort);
     for (const item of marked) item.state.last.set(item.state.next);
-    if (marked.length === 1) {
-      const state = marked[0].state;
+    // Clouds with harmonics are shaded by their own sort; the rest share one batch launch.
+    const plain = marked.filter((i) => !(i.state.splats.degree > 0));
+    for (const item of marked) {
+      if (item.state.splats.degree > 0) {
+        const state = item.state;
+        this.native.sort(this.gpu.handle, state.splats.handle, state.uniform, state.handle);
+      }
+    }
+    if (plain.length === 1) {
+      const state = plain[0].state;
       this.native.sort(this.gpu.handle, state.splats.handle, state.uniform, state.handle);
-    } else if (marked.length > 1) {
-      if (this._batch === undefined || this._batchJobs < marked.length) {
+    } else if (plain.length > 1) {
+      if (this._batch === undefined || this._batchJobs < plain.length) {
         if (this._batch !== undefined) this.native.destroy(this._batch);
-        this._batchJobs = Math.max(8, marked.length);
+        this._batchJobs = Math.max(8, plain.length);
         this._batch = this.native.batchCreate(this.gpu.handle, this._batchJobs);
       }
-      this.native.batchSet(this._batch, marked.map((i) => i.state.splats.handle),
-        marked.map((i) => i.state.uniform), marked.map((i) => i.state.handle));
+      this.native.batchSet(this._batch, plain.map((i) => i.state.splats.handle),
+        plain.map((i) => i.state.uniform), plain.map((i) => i.state.handle));
       this.native.batchRun(this.gpu.handle, this._batch);
     }
     for (const item of marked) item.state.sortedOnce = true;
@@ -304,6 +326,6 @@ class SplatPass {
 }
 
 module.exports = {
-  SPLAT_BYTES, BUCKETS, SPLAT_CULL, CLOUD_FLOATS, KEEP_FRAMES,
+  SPLAT_BYTES, BUCKETS, SPLAT_CULL, CLOUD_FLOATS, KEEP_FRAMES, SH_COEFFICIENTS, shWords,
   native, Gpu, parseSplats, Splats, SplatScene, SplatPass, cloudUniform,
 };

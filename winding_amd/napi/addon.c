@@ -196,6 +196,23 @@ static napi_value js_cloud_create(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* cloudSH(context, cloud, Uint32Array sh, degree): the capture's harmonics, degree 1-3 (0 removes them) */
+static napi_value js_cloud_sh(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *cl = get_box(env, argv[1], K_CLOUD);
+  if (!cl) return NULL;
+  void *data;
+  size_t len;
+  if (!typed(env, argv[2], napi_uint32_array, &data, &len)) return NULL;
+  uint32_t degree = 0;
+  CHECK_NAPI(napi_get_value_uint32(env, argv[3], &degree));
+  int rc = wd_cloud_sh((wd_context *)c->ptr, (wd_cloud *)cl->ptr, (const uint32_t *)data, len, degree);  /* len: words */
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
 static napi_value js_destroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   void *data = NULL;
@@ -665,6 +682,7 @@ static napi_value init(napi_env env, napi_value exports) {
   } fns[] = {
       {"open", js_open},           {"close", js_close},           {"sync", js_sync},
       {"cloudCreate", js_cloud_create}, {"destroy", js_destroy},  {"viewPrepare", js_view_prepare},
+      {"cloudSH", js_cloud_sh},
       {"stateCreate", js_state_create}, {"sort", js_sort},        {"stateRead", js_state_read},
       {"synth", js_synth},         {"profile", js_profile},       {"profileRead", js_profile_read},
       {"batchCreate", js_batch_create}, {"batchSet", js_batch_set}, {"batchRun", js_batch_run},
