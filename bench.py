@@ -244,7 +244,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="slab-8m",
-                    choices=["plain-64k", "plain-1m", "slab-8m", "batch-64", "learned-4m"])
+                    choices=["plain-64k", "plain-1m", "slab-8m", "batch-64", "learned-4m", "sh3-1m"])
     ap.add_argument("--cull", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -264,15 +264,23 @@ def main():
               "and the multi-GPU split is a design note, not code)" % args.gpus, file=sys.stderr)
         return 2
 
+    import numpy as np
     import winding_amd as wa
     if args.workload == "batch-64":
         return bench_batch(args)
     if args.workload == "learned-4m":
         return bench_learned(args)
-    recs, view, spec = wa.workloads.plain(args.workload, cull=args.cull)
+    # sh3-1m: the 1,048,576-splat cloud with degree-3 harmonics (seeded), shaded in the projection
+    sh_degree = 3 if args.workload == "sh3-1m" else 0
+    recs, view, spec = wa.workloads.plain("plain-1m" if sh_degree else args.workload, cull=args.cull)
     n = len(recs)
     ctx = wa.Context(0)
     cloud = wa.Cloud(ctx, recs)
+    sh_bytes = 0
+    if sh_degree:
+        words = wa.workloads.sh_words(n, sh_degree, seed=7)
+        cloud.set_sh(words, sh_degree)
+        sh_bytes = 4.0 * (words.size // n)
     st = wa.State(ctx, n)
     for _ in range(args.warmup):
         st.sort(cloud, view)
@@ -297,10 +305,29 @@ def main():
     ctx.profile(False)
     kernels = {k: {"ms_mean": msum / cnt, "launches": cnt} for k, (msum, cnt) in prof.items()}
     proj_ms = kernels["k_project"]["ms_mean"]
-    alg_proj = 32.0 * n + 16.0 * v
+    alg_proj = 32.0 * n + (16.0 + sh_bytes) * v  # the survivors' harmonics are read once, by the projection
     achieved = alg_proj / (proj_ms * 1e-3) / 1e9
-    path_bytes = 32.0 * n + 20.0 * v
+    path_bytes = 32.0 * n + (20.0 + sh_bytes) * v
     path_gbs = path_bytes / (ms * 1e-3) / 1e9
+
+    # The spread of single sorts: HIP events around each, on the library's stream.
+    each = np.sort(st.time_sorts(cloud, view, max(100, args.steps)))
+    latency = {"median_ms": float(np.median(each)), "p95_ms": float(each[int(0.95 * (len(each) - 1))]),
+               "sorts": int(len(each))}
+
+    # The same cloud and view with the area cull off (cull = 0): more survivors, the same path.
+    view0 = wa.workloads.plain("plain-1m" if sh_degree else args.workload, cull=0.0)[1]
+    for _ in range(args.warmup):
+        st.sort(cloud, view0)
+    ctx.sync()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        st.sort(cloud, view0)
+    ctx.sync()
+    ms0 = (time.perf_counter() - t2) * 1e3 / args.steps
+    v0 = st.stats().count
+    cull0 = {"value": n / (ms0 * 1e-3) / 1e6, "ms_per_step": ms0, "survivors": v0,
+             "path_frac_of_peak": (32.0 * n + (20.0 + sh_bytes) * v0) / (ms0 * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
     line = {
         "metric": "M splats projected+sorted/s and % HBM peak",
@@ -316,7 +343,7 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded generator, winding_amd/workloads.py)",
         "config": {"workload": args.workload, "splats": n, "survivors": v, "width": spec["width"],
-                   "height": spec["height"], "cull": args.cull, "parallelism": "single GPU"},
+                   "height": spec["height"], "cull": args.cull, "sh_degree": sh_degree, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, "k_project"),
                      "kernel": "k_project",
@@ -324,6 +351,8 @@ def main():
         "path": {"bytes_per_step": path_bytes, "achieved_gbs": path_gbs, "frac_of_peak": path_gbs / HBM_PEAK_GBS,
                  "recomputed_fp64": stats.recomputed, "profiled_ms_per_step": prof_elapsed * 1e3 / args.steps},
         "kernels": kernels,
+        "latency": latency,
+        "cull0": cull0,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(recs, view)

@@ -202,6 +202,13 @@ int wd_profile(wd_context *ctx, int on);
  */
 int wd_profile_read(wd_context *ctx, const char **names, double *ms,
                     uint64_t *launches, int capacity, int *n);
+/*
+ * Runs `reps` sorts of `cloud` for `view` into `state`, each bracketed by HIP
+ * events on the context's stream, and returns each one's milliseconds in
+ * ms[reps] (the per-sort spread a benchmark reports as median and p95).
+ */
+int wd_time_sorts(wd_context *ctx, const wd_cloud *cloud, const wd_view *view,
+                  wd_state *state, uint32_t reps, double *ms);
 
 /* ---- batches: many (cloud, view) sorts in one graph launch ------------ */
 

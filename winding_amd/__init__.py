@@ -25,7 +25,7 @@ EXPORTS = [
     "wd_last_error", "wd_open", "wd_close", "wd_sync", "wd_stream", "wd_capacity",
     "wd_cloud_create", "wd_cloud_create_device", "wd_cloud_info", "wd_cloud_records", "wd_cloud_destroy",
     "wd_cloud_sh", "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
-    "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_profile", "wd_profile_read",
+    "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_profile", "wd_profile_read", "wd_time_sorts",
     "wd_batch_create", "wd_batch_schedule", "wd_batch_tune", "wd_batch_set", "wd_batch_run", "wd_batch_stuck",
     "wd_batch_destroy",
     "wd_mlp_create", "wd_mlp_destroy", "wd_mlp_set", "wd_mlp_get", "wd_mlp_save", "wd_mlp_load",
@@ -108,6 +108,7 @@ def lib():
             "wd_profile": (I, [P, I]),
             "wd_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(U64), I,
                                     C.POINTER(I)]),
+            "wd_time_sorts": (I, [P, P, P, P, U32, P]),
             "wd_batch_create": (I, [P, U32, C.POINTER(P)]),
             "wd_batch_schedule": (I, [P, I, U32]),
             "wd_batch_tune": (I, [P, P, U32, C.POINTER(Tuning)]),
@@ -310,6 +311,12 @@ class State:
 
     def sort(self, cloud, view):
         check(lib().wd_sort(self.ctx.ptr, cloud.ptr, C.byref(view), self.ptr))
+
+    def time_sorts(self, cloud, view, reps):
+        """Milliseconds of each of `reps` sorts, HIP events around each on the library's stream."""
+        ms = np.zeros(reps, np.float64)
+        check(lib().wd_time_sorts(self.ctx.ptr, cloud.ptr, C.byref(view), self.ptr, reps, ms.ctypes.data))
+        return ms
 
     def stats(self):
         s = Stats()

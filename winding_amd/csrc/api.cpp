@@ -442,6 +442,33 @@ int wd_profile_read(wd_context *ctx, const char **names, double *ms, uint64_t *l
   return WD_OK;
 }
 
+int wd_time_sorts(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_state *s, uint32_t reps,
+                  double *ms) {
+  if (!ms || reps == 0) return fail(WD_E_ARG, "time sorts: no repetitions or no output");
+  int rc = check_sort_args(ctx, cloud, view, s);
+  if (rc) return rc;
+  WD_HIP(hipSetDevice(ctx->device), "time sorts: set device");
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  hipError_t e = hipSuccess;
+  for (size_t k = 0; k < ev.size() && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
+  for (uint32_t r = 0; r < reps && e == hipSuccess && rc == WD_OK; ++r) {
+    e = hipEventRecord(ev[2 * r], ctx->stream);
+    if (e == hipSuccess) rc = wd::launch_sort(ctx, cloud, *view, s);
+    if (e == hipSuccess && rc == WD_OK) e = hipEventRecord(ev[2 * r + 1], ctx->stream);
+  }
+  if (e == hipSuccess && rc == WD_OK) e = hipStreamSynchronize(ctx->stream);
+  for (uint32_t r = 0; r < reps && e == hipSuccess && rc == WD_OK; ++r) {
+    float t = 0;
+    e = hipEventElapsedTime(&t, ev[2 * r], ev[2 * r + 1]);
+    ms[r] = t;
+  }
+  for (hipEvent_t x : ev)
+    if (x) hipEventDestroy(x);
+  if (rc) return rc;
+  if (e != hipSuccess) return wd::hip_fail(e, "time sorts");
+  return WD_OK;
+}
+
 // ---- debugging entry points used by the exactness tests ----
 
 int wd_debug_canonical(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, double *out5_host) {

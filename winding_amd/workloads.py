@@ -129,3 +129,19 @@ def training_set(ctx, cloud, state, eyes, width, height, cull=WD_SPLAT_CULL):
         X.append(pose_features(view))
         Y.append(widths_from_edges(state.quantiles()))
     return np.array(X, np.float32), np.array(Y, np.float32)
+
+
+SH_COEFFICIENTS = (0, 3, 8, 15)
+
+
+def sh_words(n, degree, seed=7, sigma=0.2):
+    """Harmonics of `degree` for n splats in the reference's `sh` layout (src/scene/splats.js:306-332):
+    coefficients N(0, sigma) as IEEE halves, coefficient by coefficient, red, green and blue, ceil(3k / 2)
+    32-bit words a splat (k = 3, 8, 15). numpy's seeded generator."""
+    import numpy as np
+    k = SH_COEFFICIENTS[degree]
+    w = (3 * k + 1) // 2
+    halves = np.zeros((n, 2 * w), np.uint16)
+    vals = np.random.default_rng(seed).normal(0.0, sigma, (n, 3 * k)).astype(np.float16)
+    halves[:, :3 * k] = vals.view(np.uint16)
+    return halves.reshape(-1).view(np.uint32)
