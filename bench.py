@@ -73,6 +73,55 @@ def cpu_baseline(recs, view, budget_s=15.0):
             "sample": "first %d splats of the same cloud and view, median of %d runs, oracle/splat_oracle.c" % (n, len(times))}
 
 
+def bench_particles(args):
+    """particles-8m: one emitter ring of 8,388,608 particles (a quarter dead, 5% never used), sorted far to near
+    (wd_particles_sort): the key pass, then the splat sort's five passes."""
+    import numpy as np
+    import winding_amd as wa
+    n = 8 << 20
+    pool = wa.workloads.particle_pool(n, seed=11)
+    ctx = wa.Context(0)
+    p = wa.Pool(ctx, pool)
+    st = wa.State(ctx, n)
+    view = wa.particle_view(wa.workloads.PARTICLE_EYE, wa.workloads.PARTICLE_FORWARD)
+    for _ in range(args.warmup):
+        st.sort_particles(p, view)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st.sort_particles(p, view)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    v = st.stats().count
+    ctx.profile(True)
+    for _ in range(args.steps):
+        st.sort_particles(p, view)
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    kernels = {k: {"ms_mean": msum / cnt, "launches": cnt} for k, (msum, cnt) in prof.items()}
+    key_ms = kernels["k_particle_keys"]["ms_mean"]
+    alg_key = 32.0 * n + 2.0 * n  # the 32-byte particles read, the 16-bit keys written
+    line = {
+        "metric": "M particles keyed+sorted/s", "value": n / (ms * 1e-3) / 1e6, "unit": "M particles/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic (seeded generator, winding_amd/workloads.py particle_pool)",
+        "config": {"workload": "particles-8m", "particles": n, "live": v, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": alg_key / (key_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": alg_key / (key_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_particle_keys", "kernel_ms": key_ms, "bytes_per_launch": alg_key},
+        "path": {"bytes_per_step": 32.0 * n + 4.0 * v,
+                 "frac_of_peak": (32.0 * n + 4.0 * v) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "kernels": kernels,
+    }
+    print(json.dumps(line), flush=True)
+    st.destroy()
+    p.destroy()
+    ctx.close()
+    return 0
+
+
 def bench_batch(args):
     """64 scenes of log-uniform size (about 32M splats), one random orbit view each: one hipGraph launch
     of the batch (schedule picked by the autotuner) against the same 64 sorts as single-view launches."""
@@ -244,7 +293,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="slab-8m",
-                    choices=["plain-64k", "plain-1m", "slab-8m", "batch-64", "learned-4m", "sh3-1m"])
+                    choices=["plain-64k", "plain-1m", "slab-8m", "batch-64", "learned-4m", "sh3-1m", "particles-8m"])
     ap.add_argument("--cull", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -270,6 +319,8 @@ def main():
         return bench_batch(args)
     if args.workload == "learned-4m":
         return bench_learned(args)
+    if args.workload == "particles-8m":
+        return bench_particles(args)
     # sh3-1m: the 1,048,576-splat cloud with degree-3 harmonics (seeded), shaded in the projection
     sh_degree = 3 if args.workload == "sh3-1m" else 0
     recs, view, spec = wa.workloads.plain("plain-1m" if sh_degree else args.workload, cull=args.cull)
@@ -329,6 +380,28 @@ def main():
     cull0 = {"value": n / (ms0 * 1e-3) / 1e6, "ms_per_step": ms0, "survivors": v0,
              "path_frac_of_peak": (32.0 * n + (20.0 + sh_bytes) * v0) / (ms0 * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
+    # The same sort with the draw's ellipse set-up written beside cov2d (32 more bytes a splat).
+    st.set_ellipses(True)
+    for _ in range(args.warmup):
+        st.sort(cloud, view)
+    ctx.sync()
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        st.sort(cloud, view)
+    ctx.sync()
+    ms_e = (time.perf_counter() - t3) * 1e3 / args.steps
+    ctx.profile(True)
+    for _ in range(args.steps):
+        st.sort(cloud, view)
+    ctx.sync()
+    proj_e = ctx.profile_read()["k_project"]
+    ctx.profile(False)
+    st.set_ellipses(False)
+    proj_e_ms = proj_e[0] / proj_e[1]
+    alg_e = alg_proj + 32.0 * v  # the survivors' set-up (culled slots are written too, as zeros)
+    ellipses = {"value": n / (ms_e * 1e-3) / 1e6, "ms_per_step": ms_e, "k_project_ms": proj_e_ms,
+                "k_project_frac_of_peak": alg_e / (proj_e_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
     line = {
         "metric": "M splats projected+sorted/s and % HBM peak",
         "value": n / (ms * 1e-3) / 1e6,
@@ -352,7 +425,7 @@ def main():
                  "recomputed_fp64": stats.recomputed, "profiled_ms_per_step": prof_elapsed * 1e3 / args.steps},
         "kernels": kernels,
         "latency": latency,
-        "cull0": cull0,
+        "cull0": cull0, "ellipses": ellipses,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(recs, view)

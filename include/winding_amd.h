@@ -57,12 +57,27 @@ extern "C" {
 #define WD_COARSE_BINS 256
 /* Pose features the predictor reads: the model-view's top three rows. */
 #define WD_POSE_FLOATS 12
+/* Bytes in one particle: position, age, velocity, lifetime. src/render/particles.js:51-56 */
+#define WD_PARTICLE_BYTES 32
+/* The particle sort's log-depth buckets. src/render/particles.js:45 */
+#define WD_PARTICLE_BUCKETS 4096
 
 typedef struct wd_context wd_context;
 typedef struct wd_cloud wd_cloud;
 typedef struct wd_state wd_state;
 typedef struct wd_batch wd_batch;
 typedef struct wd_mlp wd_mlp;
+typedef struct wd_pool wd_pool;
+
+/*
+ * The particle sort's view: the reference's 32-byte View uniform
+ * (src/render/particles.js:171-174, written at :688-693): the camera's
+ * position, and its forward axis, -(view[2], view[6], view[10]).
+ */
+typedef struct wd_particle_view {
+  float eye[4];     /* xyz used */
+  float forward[4]; /* xyz used */
+} wd_particle_view;
 
 /*
  * What one sort depends on: the first 37 floats of the reference's Cloud
@@ -134,12 +149,76 @@ int wd_cloud_destroy(wd_cloud *cloud);
  * From then on wd_sort writes, as each survivor's colour, what the reference's
  * shade pass computes (src/render/splats.js:315-381): the record's colour plus
  * the harmonics along the direction from the eye (the inverse model-view's
- * translation, as SplatPass.prepare sets it, :580), clamped to [0, 1] and
+ * translation, as SplatPass.prepare sets it, :585-586), clamped to [0, 1] and
  * packed as unorm bytes, alpha kept; in f32, within one unit of the byte of
  * the FP64 evaluation. Batches and learned runs refuse such a cloud.
  */
 int wd_cloud_sh(wd_context *ctx, wd_cloud *cloud, const uint32_t *words,
                 uint64_t n_words, uint32_t degree);
+
+/*
+ * A cloud given as the reference's Splats data (new Splats(rhi, data),
+ * src/render/splats.js:389-416, from parseSplats' output, src/scene/splats.js:
+ * 305-370): `count` centres of 4 floats (x, y, z, and the rgba8 colour's bits
+ * in the fourth) and 6 covariance floats each (Sigma's xx xy xz yy yz zz), host
+ * memory. Refuses a non-finite position by index, as put() does
+ * (src/scene/splats.js:336-338), and fills the box. Its sorts decide the
+ * footprint and the cull in FP64 from the f32 Sigma. Batches and learned runs
+ * refuse such a cloud; wd_cloud_records gives null for it.
+ */
+int wd_cloud_create_splats(wd_context *ctx, const float *centers,
+                           const float *covariances, uint64_t count,
+                           wd_cloud **out, float min_out[3], float max_out[3]);
+/*
+ * Copies a cloud's centres (4 floats a splat), covariances (6) and harmonics
+ * (sh words, the reference's layout) to the host, and its degree; any pointer
+ * may be null. A cloud of .splat records has no centres to read.
+ */
+int wd_cloud_read(wd_context *ctx, const wd_cloud *cloud, float *centers,
+                  float *covariances, uint32_t *sh, uint32_t *degree);
+
+/* ---- capture decoders on the GPU: src/scene/splats.js:58-303,390-468 -- */
+
+/*
+ * Decodes a capture into a cloud of centres and covariances, with its
+ * harmonics attached (degree from the file): the header on the host, every
+ * splat on the GPU, with put()'s formulas in FP64 as the reference runs them
+ * in JavaScript numbers. The reference's refusals, with its messages.
+ *   wd_decode_ply  a binary_little_endian .ply as training writes it
+ *                  (parsePly, src/scene/splats.js:398-468)
+ *   wd_decode_spz  a .spz of version 1 to 3, gzipped or already gunzipped
+ *                  (readSplats + parseSpz, src/scene/splats.js:84-92,225-303)
+ *   wd_decode_sog  a .sog of version 2 whose zip, meta.json and WebP images the
+ *                  caller has unpacked (readSog, src/scene/splats.js:128-203):
+ *                  each image as rgba8 pixels, as decodeImage hands them over
+ */
+int wd_decode_ply(wd_context *ctx, const void *bytes, uint64_t len,
+                  wd_cloud **out, float min_out[3], float max_out[3]);
+int wd_decode_spz(wd_context *ctx, const void *bytes, uint64_t len,
+                  wd_cloud **out, float min_out[3], float max_out[3]);
+
+typedef struct wd_sog_image {
+  const uint8_t *rgba; /* width * height * 4 bytes, host memory */
+  uint32_t width, height;
+} wd_sog_image;
+
+typedef struct wd_sog {
+  uint32_t count;    /* meta.count */
+  uint32_t bands;    /* meta.shN.bands: the harmonics' degree, 0 when none */
+  uint32_t sh_count; /* meta.shN.count: palette entries; a label at or past it has no harmonics */
+  double mins[3], maxs[3];             /* meta.means.mins, maxs */
+  const double *scale_codebook;        /* meta.scales.codebook */
+  uint32_t scale_codebook_size;
+  const double *colour_codebook;       /* meta.sh0.codebook */
+  uint32_t colour_codebook_size;
+  const double *sh_codebook;           /* meta.shN.codebook (bands > 0) */
+  uint32_t sh_codebook_size;
+  wd_sog_image means_l, means_u, quats, scales, sh0;
+  wd_sog_image centroids, labels;      /* meta.shN.files (bands > 0) */
+} wd_sog;
+
+int wd_decode_sog(wd_context *ctx, const wd_sog *sog, wd_cloud **out,
+                  float min_out[3], float max_out[3]);
 
 /* ---- the per-view uniform: SplatPass.prepare, src/render/splats.js:559-587 */
 
@@ -191,6 +270,59 @@ int wd_state_stats(wd_context *ctx, wd_state *state, wd_stats *stats);
  */
 int wd_state_read_keys(wd_context *ctx, wd_state *state, uint16_t *keys,
                        uint64_t *mask, uint64_t n);
+
+/*
+ * The draw's per-splat ellipse set-up, made by the projection beside cov2d.
+ * Replaces the per-instance work of the splat vertex shader `vs`
+ * (src/render/splats.js:263-291): with it a rasteriser reads each splat once
+ * instead of projecting it again. wd_state_ellipses(ctx, state, 1) allocates
+ * two planes of capacity float4s and from then on wd_sort writes, for splat i:
+ *   plane 0, [i]  its clip position (x, y, z, w), the FP64 value rounded to f32;
+ *   plane 1, [i]  (along e, across e) in pixels: the eigen-axes of the footprint plus
+ *          0.3 px^2 each way, each sqrt(2 lambda) long and capped at 1024,
+ *          along the larger first, across = along turned a quarter clockwise;
+ *          e = sqrt(min(log(255 alpha), 4)). Corner q of the quad, q in
+ *          {-1, 1}^2, lies at clip.xy + (q.x along e + q.y across e) 2 /
+ *          viewport clip.w; the Gaussian's offset there is q e.
+ * Both are zero for a splat the shader would not draw (culled, or a minor
+ * axis or reach not above zero). 0 frees them. f32, within 1e-5 of the
+ * largest term of the FP64 evaluation. Batches and learned runs refuse such
+ * a state. wd_state_ellipse_buffer gives the device address of plane 0
+ * (plane 1 follows at capacity float4s; null when off); wd_state_read_ellipses
+ * copies the first n splats' 8 floats (clip position, then axes) to the host.
+ */
+int wd_state_ellipses(wd_context *ctx, wd_state *state, int on);
+int wd_state_ellipse_buffer(wd_state *state, void **ellipses);
+int wd_state_read_ellipses(wd_context *ctx, wd_state *state, float *out,
+                           uint64_t n);
+
+/* ---- particles: the particle sort, src/render/particles.js:169-240 ---- */
+
+/*
+ * A pool of `count` particles (WD_PARTICLE_BYTES each, host memory: position
+ * xyz, age, velocity xyz, lifetime, all f32) uploaded to the device, as the
+ * reference's `particles` buffer (src/render/particles.js:720-730). A slot
+ * whose age is not below its lifetime is dead (a zeroed slot is). The device
+ * address (wd_pool_device) is for a simulation that writes the pool in place.
+ */
+int wd_pool_create(wd_context *ctx, const void *particles, uint64_t count,
+                   wd_pool **out);
+int wd_pool_device(const wd_pool *pool, void **device_particles);
+int wd_pool_destroy(wd_pool *pool);
+/*
+ * Sorts one emitter's ring, slots [first, first + count) of the pool, far to
+ * near (makeKeys -> scanCounts -> place, src/render/particles.js:198-240):
+ * each live particle's key is 4095 - bucket, bucket = floor((log2(depth) + 10)
+ * 4095 / 30) clamped to [0, 4095], depth = max(dot(position - eye, forward),
+ * 1e-6), decided in FP64 against the 4095 bucket edges 2^((30 b - 40950) /
+ * 4095). The state's order gets the live particles' slots (first + index),
+ * key ascending and, unlike the reference's atomics, slot ascending within a
+ * key; its count the live particles (the draw's instanceCount). Enqueued on
+ * the context's stream. Read back with wd_state_read (cov2d is not written).
+ */
+int wd_particles_sort(wd_context *ctx, const wd_pool *pool, uint64_t first,
+                      uint64_t count, const wd_particle_view *view,
+                      wd_state *state);
 
 /* ---- timing: HIP events around each kernel on the context's stream ---- */
 

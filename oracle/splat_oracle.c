@@ -21,6 +21,8 @@
  *                 ulp of f64 (never visible after the f32 rounding put does).
  *   or_footprint  footprint(): J W Sigma W^T J^T, src/render/splats.js:102-125
  *   or_splat      makeKeys() cull and key, src/render/splats.js:155-186
+ *   or_splat_sigma, or_run_sigma  the same for a splat given as its centre
+ *                 and f32 Sigma (the reference's Splats data), and the sort
  *   or_sort       stable one-pass counting sort over 65536 buckets, in the
  *                 idiom of DrawList.sort (src/render/drawlist.js:158-203);
  *                 ties keep splat index order (the reference's place() uses
@@ -32,9 +34,16 @@
  *   or_look_at, or_perspective  mat4LookAt / mat4PerspectiveReverseZInfinite
  *                 src/core/math/mat4.js:243-273,455-468
  *   or_eye        the eye in a capture's space, as SplatPass.prepare sets it
- *                 (src/render/splats.js:580): the translation of mat4Invert
+ *                 (src/render/splats.js:585-586): the translation of mat4Invert
  *                 (src/core/math/mat4.js:186-228) of the f32 model-view, f64
  *                 maths stored as f32
+ *   or_ellipse, or_ellipses  the splat vertex shader's per-instance set-up
+ *                 (src/render/splats.js:263-291): eigen-axes of the footprint
+ *                 plus 0.3, their lengths and cap, the reach; with the clip
+ *                 position of each splat
+ *   or_particle_keys, or_particle_edges  the particle sort's live test and
+ *                 log-depth key (src/render/particles.js:198-211), the bucket
+ *                 decided against FP64 edges (definition at the function)
  *   or_shade      the shade pass's colour of one splat with harmonics,
  *                 src/render/splats.js:335-381 (coefficient, harmonics,
  *                 shade), halves decoded exactly, in FP64; the `sh` layout
@@ -59,6 +68,8 @@ typedef struct or_view {
 } or_view;
 
 #define OR_BUCKETS 65536
+
+uint32_t or_sort(const uint16_t *keys, const uint8_t *keep, uint64_t n, uint32_t *order);
 
 static const double C13 = (double)1.3f;
 static const double C03 = (double)0.3f;
@@ -176,6 +187,53 @@ int or_splat(const uint8_t *rec, const or_view *v, uint16_t *key, double cov[3],
   uint32_t bucket = (uint32_t)t;
   *key = (uint16_t)(65535u - bucket);
   return 1;
+}
+
+/*
+ * makeKeys() for a splat given as its centre (4 floats, colour bits in the fourth) and its f32
+ * Sigma (xx xy xz yy yz zz), as the reference's Splats data holds it: the same decisions as
+ * or_splat with Sigma read instead of built from a record.
+ */
+int or_splat_sigma(const float *centre, const float *sigma6, const or_view *v, uint16_t *key, double cov[3]) {
+  double view[4], clip[4];
+  view_of(v, centre[0], centre[1], centre[2], view);
+  const float *P = v->projection;
+  for (int r = 0; r < 4; r++)
+    clip[r] = (double)P[r] * view[0] + (double)P[4 + r] * view[1] + (double)P[8 + r] * view[2] + (double)P[12 + r] * view[3];
+  double w = clip[3];
+  if (view[2] >= 0 || clip[2] > w || fabs(clip[0]) > C13 * w || fabs(clip[1]) > C13 * w) return 0;
+  uint32_t word;
+  memcpy(&word, &centre[3], 4);
+  unsigned a = word >> 24;
+  if (a <= 1) return 0;
+  double sg[6], f[3];
+  for (int k = 0; k < 6; k++) sg[k] = sigma6[k];
+  or_footprint(v, view, sg, f);
+  if (cov) memcpy(cov, f, sizeof f);
+  if (v->cull > 0) {
+    double det = (f[0] + C03) * (f[2] + C03) - f[1] * f[1];
+    double area = C2PI * sqrt(det > 0 ? det : 0);
+    if (a / 255.0 * area < (double)v->cull) return 0;
+  }
+  double t = (-view[2] - (double)v->depth_min) * (double)v->depth_scale;
+  if (t < 0) t = 0;
+  if (t > 65535.0) t = 65535.0;
+  *key = (uint16_t)(65535u - (uint32_t)t);
+  return 1;
+}
+
+/* or_splat_sigma over a cloud, then the stable sort: returns V; keys, keep and cov (n x 3) per splat. */
+uint32_t or_run_sigma(const float *centers, const float *covs, uint64_t n, const or_view *v, uint32_t *order,
+                      uint16_t *keys, uint8_t *keep, double *cov) {
+  for (uint64_t i = 0; i < n; i++) {
+    uint16_t k = 0;
+    double c[3] = {NAN, NAN, NAN};
+    int s = or_splat_sigma(centers + 4 * i, covs + 6 * i, v, &k, c);
+    keys[i] = s ? k : 0;
+    keep[i] = (uint8_t)s;
+    if (cov) memcpy(cov + 3 * i, c, sizeof c);
+  }
+  return or_sort(keys, keep, n, order);
 }
 
 /* Every splat of a cloud: keys (culled get 0), keep flags, FP64 cov (n x 3, may be null), value (may be null). */
@@ -377,6 +435,107 @@ void or_shade_all(const uint8_t *records, uint64_t n, const uint32_t *sh, int de
                   uint32_t *out) {
   int W = or_sh_words(degree);
   for (uint64_t i = 0; i < n; i++) out[i] = or_shade(records + 32 * i, sh + (uint64_t)W * i, degree, eye);
+}
+
+/*
+ * The splat vertex shader's per-instance set-up (src/render/splats.js:263-291)
+ * for a splat with FP64 footprint f = (xx, xy, yy) and alpha byte abyte, in
+ * FP64, the f32 literals widened: out = (along e, across e), the eigen-axes
+ * of [[xx + 0.3, xy], [xy, yy + 0.3]] as long as sqrt(2 lambda) capped at 1024,
+ * across = along turned a quarter clockwise, e = sqrt(min(log(255 alpha), 4)).
+ * Returns 0 (out zeros) when the shader draws nothing: minor axis or reach
+ * not above zero.
+ */
+int or_ellipse(const double f[3], unsigned abyte, double out[4]) {
+  double a = f[0] + C03, b = f[1], c = f[2] + C03;
+  double mid = 0.5 * (a + c);
+  double h = 0.5 * (a - c);
+  double radius = sqrt(h * h + b * b);
+  double major = mid + radius, minor = mid - radius;
+  double reach = log(255.0 * (abyte / 255.0));
+  out[0] = out[1] = out[2] = out[3] = 0;
+  if (!(minor > 0) || !(reach > 0)) return 0;
+  double ax = 1, ay = 0;
+  if (fabs(b) > (double)1e-12f) {
+    double uy = major - a, len = sqrt(b * b + uy * uy);
+    ax = b / len;
+    ay = uy / len;
+  } else if (c > a) {
+    ax = 0;
+    ay = 1;
+  }
+  double e = sqrt(reach < 4 ? reach : 4);
+  double la = sqrt(2 * major), lc = sqrt(2 * minor);
+  la = (la < 1024 ? la : 1024) * e;
+  lc = (lc < 1024 ? lc : 1024) * e;
+  out[0] = la * ax;
+  out[1] = la * ay;
+  out[2] = lc * ay;
+  out[3] = -(lc * ax);
+  return 1;
+}
+
+/* The draw set-up of every splat of a cloud, 8 doubles a splat: its clip position (modelView, then projection,
+ * as vs and makeKeys compute it), then or_ellipse; all zero for a splat culled or not drawn. */
+void or_ellipses(const uint8_t *records, uint64_t n, const or_view *v, double *out) {
+  for (uint64_t i = 0; i < n; i++) {
+    const uint8_t *rec = records + 32 * i;
+    double *o = out + 8 * i;
+    uint16_t k;
+    double f[3];
+    memset(o, 0, 8 * sizeof(double));
+    if (!or_splat(rec, v, &k, f, NULL)) continue;
+    if (!or_ellipse(f, rec[27], o + 4)) continue;
+    double view[4];
+    view_of(v, rd_f32(rec), rd_f32(rec + 4), rd_f32(rec + 8), view);
+    const float *P = v->projection;
+    for (int r = 0; r < 4; r++)
+      o[r] = (double)P[r] * view[0] + (double)P[4 + r] * view[1] + (double)P[8 + r] * view[2] + (double)P[12 + r] * view[3];
+  }
+}
+
+/*
+ * Particles (src/render/particles.js:169-240). A particle is 32 bytes: position (3 f32), age,
+ * velocity (3 f32), lifetime. A live one (age < lifetime) gets the key 4095 - bucket, far
+ * first, with bucket = floor((log2(depth) - LOG_NEAR) (BUCKETS - 1) / (LOG_FAR - LOG_NEAR))
+ * clamped to [0, 4095] (LOG_NEAR = -10, LOG_FAR = 20, BUCKETS = 4096), and
+ * depth = max(dot(position - eye, forward), 1e-6), makeKeys at :198-211.
+ *
+ * Canonical definition: depth in FP64 from the f32 inputs (the three products summed left to
+ * right, the clamp at the f32 literal 1e-6f; a NaN depth takes the clamp). The bucket is the
+ * number of edges E[b], b = 1..4095, at or below depth, where E[b] = exp2((30 b - 40950) / 4095)
+ * in FP64: floor of the formula above with log2 taken exactly, up to the rounding of each edge
+ * to a double, so that the key does not depend on how a log2 rounds.
+ */
+#define OR_PARTICLE_BUCKETS 4096
+
+void or_particle_edges(double *edges) {
+  edges[0] = 0;
+  for (int b = 1; b < OR_PARTICLE_BUCKETS; b++) edges[b] = exp2((30.0 * b - 40950.0) / 4095.0);
+}
+
+/* Per particle of a pool of n: key (0 when dead) and keep (1 when live). view: eye (3 f32), forward (3 f32). */
+void or_particle_keys(const uint8_t *pool, uint64_t n, const float eye[3], const float forward[3], uint16_t *keys,
+                      uint8_t *keep) {
+  double edges[OR_PARTICLE_BUCKETS];
+  or_particle_edges(edges);
+  for (uint64_t i = 0; i < n; i++) {
+    const uint8_t *p = pool + 32 * i;
+    float age = rd_f32(p + 12), lifetime = rd_f32(p + 28);
+    keys[i] = 0;
+    keep[i] = 0;
+    if (age >= lifetime) continue; /* as the shader tests it: a NaN age or lifetime is live */
+    double d = ((double)rd_f32(p) - eye[0]) * forward[0] + ((double)rd_f32(p + 4) - eye[1]) * forward[1] +
+               ((double)rd_f32(p + 8) - eye[2]) * forward[2];
+    if (!(d >= (double)1e-6f)) d = (double)1e-6f;
+    int lo = 0, hi = OR_PARTICLE_BUCKETS - 1; /* the largest b with edges[b] <= d */
+    while (lo < hi) {
+      int mid = (lo + hi + 1) / 2;
+      if (edges[mid] <= d) lo = mid; else hi = mid - 1;
+    }
+    keys[i] = (uint16_t)(OR_PARTICLE_BUCKETS - 1 - lo);
+    keep[i] = 1;
+  }
 }
 
 /* mat4PerspectiveReverseZInfinite: src/core/math/mat4.js:455-468. */

@@ -40,7 +40,9 @@ test('parseSplats refuses what is not .splat records, by what is wrong with it',
   const gz = new Uint8Array(64); gz.set([0x1f, 0x8b]);
   assert.throws(() => wd.parseSplats(gz), /this is gzipped/);
   const ply = new TextEncoder().encode('ply\nformat binary_little_endian 1.0\n');
-  assert.throws(() => wd.parseSplats(ply), /\.ply capture is not read here/);
+  assert.strictEqual(wd.parseSplats(ply).format, 'ply');  // decoded on the GPU by new Splats
+  const v4 = new Uint8Array(64); new DataView(v4.buffer).setUint32(0, 0x5053474e, true); new DataView(v4.buffer).setUint32(4, 4, true);
+  assert.throws(() => wd.parseSplats(v4), /version 4, compressed with zstd/);
   const ok = wd.parseSplats(record([1, 2, 3], [1, 1, 1], [1, 2, 3, 4], [255, 128, 128, 128]));
   assert.strictEqual(ok.count, 1);
   assert.strictEqual(ok.degree, 0);

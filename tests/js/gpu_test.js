@@ -183,5 +183,99 @@ test('a capture with harmonics changes colour with the view, in its own space (t
   console.log('      from +x ' + plusX + ', from -x ' + minusX + ', turned ' + turned);
 });
 
+test('the draw\'s ellipse set-up beside cov2d: a round splat seen head on (src/render/splats.js:263-291)', () => {
+  const bytes = record([0, 0, 0], [0.4, 0.4, 0.4], [200, 100, 50, 255], [255, 128, 128, 128]);
+  const splats = new wd.Splats(gpu, wd.parseSplats(bytes));
+  const scene = new wd.SplatScene();
+  const entity = scene.addSplats({ splats });
+  const cam = new wd.Camera({ fovY: 0.8, near: 0.1 });
+  cam.position.set([0, 0, 5]);
+  cam.update(160 / 120);
+  const pass = new wd.SplatPass(gpu);
+  pass.cull = 0;
+  pass.ellipses = true;
+  pass.prepare(scene, cam, null, 160, 120);
+  pass.sort();
+  const got = pass.read(entity, { cov2d: true, ellipses: true });
+  pass.destroy();
+  splats.destroy();
+  assert.strictEqual(got.count, 1);
+  const [xx, xy, yy] = got.cov2d;
+  const e = got.ellipses;
+  assert.ok(Math.abs(xy) < 1e-6 * xx && Math.abs(xx - yy) < 1e-6 * xx, xx + ' ' + xy + ' ' + yy);
+  // alpha 255 reaches log(255) > 4, so e = 2; a circle's axes run along x, then down y
+  const r = Math.sqrt(2 * (xx + 0.3)) * 2;
+  assert.ok(Math.abs(e[3] - 5) < 1e-5 && Math.abs(e[0]) < 1e-6 && Math.abs(e[1]) < 1e-6, 'clip ' + Array.from(e.subarray(0, 4)));
+  const close = (a, b) => Math.abs(a - b) <= 1e-5 * r;
+  assert.ok(close(Math.hypot(e[4], e[5]), r) && close(Math.hypot(e[6], e[7]), r), Array.from(e.subarray(4)));
+  assert.ok(Math.abs(e[4] * e[6] + e[5] * e[7]) <= 1e-5 * r * r, 'axes at right angles');
+  console.log('      clip w ' + e[3] + ', axes ' + Array.from(e.subarray(4)).map((x) => x.toFixed(3)));
+});
+
+test('an emitter\'s particles sort far to near, whatever slots they were born in (test/gpu.test.js:2666-2760)', () => {
+  const depths = [3, 40, 0.5, 12, 200, 7];
+  const bytes = new Uint8Array(32 * (depths.length + 2));
+  const f = new Float32Array(bytes.buffer);
+  depths.forEach((d, k) => {
+    const s = 8 * (k + 2);  // the ring starts two slots in
+    f.set([0.1 * k, 0, 100 - d, 0.5, 0, 0, 0, 2], s);
+  });
+  const native = gpu.native;
+  const pool = native.poolCreate(gpu.handle, bytes);
+  const state = native.stateCreate(gpu.handle, depths.length);
+  native.particlesSort(gpu.handle, pool, 2, depths.length, new Float32Array([0, 0, 100, 0, 0, 0, -1, 0]), state);
+  const got = native.stateRead(gpu.handle, state, 0);
+  native.destroy(state);
+  native.destroy(pool);
+  const want = depths.map((d, k) => [d, k + 2]).sort((a, b) => b[0] - a[0]).map((x) => x[1]);
+  assert.deepStrictEqual(Array.from(got.order), want);
+  assert.strictEqual(got.count, depths.length);
+});
+
+test('captures decoded on the GPU: a .ply, a .spz and a .sog through parseSplats / readSplats and new Splats', () => {
+  const zlib = require('zlib');
+  const SH_C0 = 0.28209479177387814;
+  const props = ['x', 'y', 'z', 'f_dc_0', 'f_dc_1', 'f_dc_2', 'opacity', 'scale_0', 'scale_1', 'scale_2', 'rot_0', 'rot_1', 'rot_2', 'rot_3'];
+  const rows = [
+    { x: -1, y: 0, z: 4, f_dc_0: (1 - 0.5) / SH_C0, f_dc_1: 0, f_dc_2: (0 - 0.5) / SH_C0, opacity: 0, scale_0: Math.log(2), scale_1: Math.log(3), scale_2: 0, rot_0: 2 },
+    { x: 5, y: -2, z: 0, rot_0: 1, opacity: 2 },
+  ];
+  const head = Buffer.from('ply\nformat binary_little_endian 1.0\nelement vertex 2\n' + props.map((p) => 'property float ' + p + '\n').join('') + 'end_header\n');
+  const body = new Float32Array(rows.length * props.length);
+  rows.forEach((row, i) => props.forEach((p, k) => { body[i * props.length + k] = row[p] || 0; }));
+  const ply = new Uint8Array(Buffer.concat([head, Buffer.from(body.buffer)]));
+  const splats = new wd.Splats(gpu, wd.parseSplats(ply));
+  assert.strictEqual(splats.count, 2);
+  assert.strictEqual(splats.degree, 0);
+  assert.deepStrictEqual(Array.from(splats.min), [-1, -2, 0]);
+  assert.deepStrictEqual(Array.from(splats.max), [5, 0, 4]);
+  const scene = new wd.SplatScene();
+  const entity = scene.addSplats({ splats });
+  const cam = new wd.Camera({ fovY: 0.8, near: 0.1 });
+  cam.position.set([0, 0, 20]);
+  cam.update(160 / 120);
+  const pass = new wd.SplatPass(gpu);
+  pass.cull = 0;
+  pass.prepare(scene, cam, null, 160, 120);
+  pass.sort();
+  const got = pass.read(entity, { cov2d: true });
+  pass.destroy();
+  splats.destroy();
+  assert.strictEqual(got.count, 2);
+  assert.deepStrictEqual(Array.from(got.order), [1, 0]);  // z 0 is farther from the eye at z 20 than z 4
+  const bits = new Uint32Array(got.cov2d.buffer, got.cov2d.byteOffset, 8)[3];
+  assert.deepStrictEqual([bits & 255, (bits >> 8) & 255, (bits >> 16) & 255, bits >>> 24], [255, 128, 0, 128]);
+  // a .spz of version 2: one splat, gzipped; the library gunzips it
+  const spz = new Uint8Array(16 + 9 + 1 + 3 + 3 + 3);
+  const sv = new DataView(spz.buffer);
+  sv.setUint32(0, 0x5053474e, true); sv.setUint32(4, 2, true); sv.setUint32(8, 1, true); spz[13] = 12;
+  spz.set([0, 0x10, 0, 0, 0xf0, 0xff, 0, 0xf8, 0xff], 16);  // x = 1, y = -1 and z = -0.5, turned to 1 and 0.5
+  spz[25] = 200; spz.set([128, 128, 128], 26); spz.set([100, 100, 100], 29); spz.set([128, 128, 128], 32);
+  const s = new wd.Splats(gpu, { format: 'spz', bytes: new Uint8Array(zlib.gzipSync(Buffer.from(spz))) });
+  assert.strictEqual(s.count, 1);
+  assert.deepStrictEqual(Array.from(s.min), [1, 1, 0.5]);
+  s.destroy();
+});
+
 gpu.destroy();
 console.log(passed + ' checks passed');

@@ -19,13 +19,19 @@ WD_SPLAT_CULL = 0.5
 WD_MAX_SORT = 1 << 24
 WD_COARSE_BINS = 256
 WD_POSE_FLOATS = 12
+WD_SH_COEFFICIENTS = (0, 3, 8, 15)
+WD_E_ARG = -1
+WD_E_DATA = -4
 
 # Every symbol include/winding_amd.h declares, in header order.
 EXPORTS = [
     "wd_last_error", "wd_open", "wd_close", "wd_sync", "wd_stream", "wd_capacity",
     "wd_cloud_create", "wd_cloud_create_device", "wd_cloud_info", "wd_cloud_records", "wd_cloud_destroy",
-    "wd_cloud_sh", "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
-    "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_profile", "wd_profile_read", "wd_time_sorts",
+    "wd_cloud_sh", "wd_cloud_create_splats", "wd_cloud_read", "wd_decode_ply", "wd_decode_spz", "wd_decode_sog",
+    "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
+    "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_state_ellipses", "wd_state_ellipse_buffer",
+    "wd_state_read_ellipses", "wd_pool_create", "wd_pool_device", "wd_pool_destroy", "wd_particles_sort",
+    "wd_profile", "wd_profile_read", "wd_time_sorts",
     "wd_batch_create", "wd_batch_schedule", "wd_batch_tune", "wd_batch_set", "wd_batch_run", "wd_batch_stuck",
     "wd_batch_destroy",
     "wd_mlp_create", "wd_mlp_destroy", "wd_mlp_set", "wd_mlp_get", "wd_mlp_save", "wd_mlp_load",
@@ -33,6 +39,26 @@ EXPORTS = [
     "wd_learned_create", "wd_learned_run", "wd_learned_destroy", "wd_state_quantiles",
     "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest", "wd_debug_copy",
 ]
+
+
+class SogImage(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class Sog(C.Structure):
+    """wd_sog: a .sog's meta.json numbers and its decoded images."""
+    _fields_ = [("count", C.c_uint32), ("bands", C.c_uint32), ("sh_count", C.c_uint32),
+                ("mins", C.c_double * 3), ("maxs", C.c_double * 3),
+                ("scale_codebook", C.c_void_p), ("scale_codebook_size", C.c_uint32),
+                ("colour_codebook", C.c_void_p), ("colour_codebook_size", C.c_uint32),
+                ("sh_codebook", C.c_void_p), ("sh_codebook_size", C.c_uint32),
+                ("means_l", SogImage), ("means_u", SogImage), ("quats", SogImage), ("scales", SogImage),
+                ("sh0", SogImage), ("centroids", SogImage), ("labels", SogImage)]
+
+
+class ParticleView(C.Structure):
+    """wd_particle_view: the particle sort's View uniform (src/render/particles.js:171-174)."""
+    _fields_ = [("eye", C.c_float * 4), ("forward", C.c_float * 4)]
 
 
 class View(C.Structure):
@@ -96,6 +122,11 @@ def lib():
             "wd_cloud_info": (I, [P, C.POINTER(U64), FP, FP]),
             "wd_cloud_records": (I, [P, C.POINTER(P)]),
             "wd_cloud_sh": (I, [P, P, P, U64, U32]),
+            "wd_cloud_create_splats": (I, [P, P, P, U64, C.POINTER(P), P, P]),
+            "wd_cloud_read": (I, [P, P, P, P, P, P]),
+            "wd_decode_ply": (I, [P, P, U64, C.POINTER(P), P, P]),
+            "wd_decode_spz": (I, [P, P, U64, C.POINTER(P), P, P]),
+            "wd_decode_sog": (I, [P, C.POINTER(Sog), C.POINTER(P), P, P]),
             "wd_cloud_destroy": (I, [P]),
             "wd_view_prepare": (I, [FP, FP, FP, C.c_float, C.c_float, FP, FP, C.c_float, C.POINTER(View)]),
             "wd_state_create": (I, [P, U64, C.POINTER(P)]),
@@ -105,6 +136,13 @@ def lib():
             "wd_state_read": (I, [P, P, P, U64, P, U64, C.POINTER(Stats)]),
             "wd_state_stats": (I, [P, P, C.POINTER(Stats)]),
             "wd_state_read_keys": (I, [P, P, P, P, U64]),
+            "wd_state_ellipses": (I, [P, P, I]),
+            "wd_state_ellipse_buffer": (I, [P, C.POINTER(P)]),
+            "wd_state_read_ellipses": (I, [P, P, P, U64]),
+            "wd_pool_create": (I, [P, P, U64, C.POINTER(P)]),
+            "wd_pool_device": (I, [P, C.POINTER(P)]),
+            "wd_pool_destroy": (I, [P]),
+            "wd_particles_sort": (I, [P, P, U64, U64, C.POINTER(ParticleView), P]),
             "wd_profile": (I, [P, I]),
             "wd_profile_read": (I, [P, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(U64), I,
                                     C.POINTER(I)]),
@@ -277,6 +315,41 @@ class Cloud:
         self.ctx = ctx
         self.ptr = p
 
+    @classmethod
+    def _wrap(cls, ctx, p, mn, mx):
+        self = cls.__new__(cls)
+        self.ctx, self.ptr, self.min, self.max = ctx, p, mn, mx
+        n = C.c_uint64()
+        check(lib().wd_cloud_info(p, C.byref(n), None, None))
+        self.count = n.value
+        d = C.c_uint32()
+        check(lib().wd_cloud_read(ctx.ptr, p, None, None, None, C.byref(d)))
+        self.degree = d.value
+        return self
+
+    @classmethod
+    def from_splats(cls, ctx, centers, covariances):
+        """The reference's Splats data (parseSplats' output): centres (n, 4) f32 with the colour bits in the
+        fourth, covariances (n, 6) f32 (new Splats(rhi, data), src/render/splats.js:389-416)."""
+        centers = np.ascontiguousarray(centers, np.float32).reshape(-1, 4)
+        covariances = np.ascontiguousarray(covariances, np.float32).reshape(-1, 6)
+        assert len(centers) == len(covariances)
+        p = C.c_void_p()
+        mn, mx = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        check(lib().wd_cloud_create_splats(ctx.ptr, centers.ctypes.data, covariances.ctypes.data, len(centers),
+                                           C.byref(p), _fp(mn), _fp(mx)))
+        return cls._wrap(ctx, p, mn, mx)
+
+    def read(self):
+        """(centres (n, 4) f32, covariances (n, 6) f32, sh words (n, W) u32 or None, degree) of a decoded cloud."""
+        centers = np.zeros((self.count, 4), np.float32)
+        covs = np.zeros((self.count, 6), np.float32)
+        w = (WD_SH_COEFFICIENTS[self.degree] * 3 + 1) // 2
+        sh = np.zeros((self.count, w), np.uint32) if self.degree else None
+        check(lib().wd_cloud_read(self.ctx.ptr, self.ptr, centers.ctypes.data, covs.ctypes.data,
+                                  sh.ctypes.data if sh is not None else None, None))
+        return centers, covs, sh, self.degree
+
     def destroy(self):
         if self.ptr:
             check(lib().wd_cloud_destroy(self.ptr))
@@ -292,6 +365,129 @@ class Cloud:
         out = np.zeros((self.count, 5), np.float64)
         check(lib().wd_debug_canonical(self.ctx.ptr, self.ptr, C.byref(view), out.ctypes.data))
         return out
+
+
+def _decoded(ctx, fn, data):
+    data = np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else data
+    data = np.ascontiguousarray(data, np.uint8)
+    p = C.c_void_p()
+    mn, mx = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    check(fn(ctx.ptr, data.ctypes.data, data.nbytes, C.byref(p), _fp(mn), _fp(mx)))
+    return Cloud._wrap(ctx, p, mn, mx)
+
+
+def decode_ply(ctx, data):
+    """A binary .ply as training writes it, decoded on the GPU (parsePly, src/scene/splats.js:398-468)."""
+    return _decoded(ctx, lib().wd_decode_ply, data)
+
+
+def decode_spz(ctx, data):
+    """A .spz of version 1 to 3, gzipped or not, decoded on the GPU (parseSpz, src/scene/splats.js:225-303)."""
+    return _decoded(ctx, lib().wd_decode_spz, data)
+
+
+def _unzip(data):
+    """A zip's files by name, stored and deflated entries, not zip64 (unzip, src/scene/splats.js:98-126)."""
+    import io
+    import zipfile
+    try:
+        z = zipfile.ZipFile(io.BytesIO(bytes(data)))
+    except zipfile.BadZipFile as e:
+        raise WindingError(WD_E_DATA, "splats: a zip with no end record: cut short?") from e
+    files = {}
+    for info in z.infolist():
+        if info.compress_type not in (zipfile.ZIP_STORED, zipfile.ZIP_DEFLATED):
+            raise WindingError(WD_E_DATA, "splats: %s is compressed with method %d; stored and deflate are read"
+                               % (info.filename, info.compress_type))
+        files[info.filename] = (lambda name: (lambda: z.read(name)))(info.filename)
+    return files
+
+
+def read_sog(ctx, data, decode_image=None):
+    """A .sog (a zip of meta.json and WebP images), unpacked here and decoded on the GPU (readSog,
+    src/scene/splats.js:128-203). decode_image(bytes) -> (width, height, rgba uint8), the bytes as stored."""
+    import json
+    if not callable(decode_image):
+        raise WindingError(WD_E_ARG, "splats: a .sog needs decodeImage, for its WebP images")
+    files = _unzip(data)
+
+    def file(name):
+        if name not in files:
+            raise WindingError(WD_E_DATA, "splats: the .sog has no %s" % name)
+        return files[name]()
+
+    meta = json.loads(file("meta.json").decode())
+    if meta.get("version") != 2:
+        raise WindingError(WD_E_DATA, "splats: a .sog of version %s; version 2 is read (re-save it with splat-transform)"
+                           % meta.get("version", 1))
+    count = meta.get("count")
+    if not (isinstance(count, int) and count > 0):
+        raise WindingError(WD_E_DATA, "splats: a .sog of %s splats" % count)
+    keep = []
+
+    def image(name, check_count=True):
+        w, h, rgba = decode_image(file(name))
+        rgba = np.ascontiguousarray(rgba, np.uint8).reshape(-1)
+        if rgba.size < 4 * w * h:
+            raise WindingError(WD_E_DATA, "splats: %s decoded to %d bytes, not %d x %d rgba" % (name, rgba.size, w, h))
+        if check_count and w * h < count:
+            raise WindingError(WD_E_DATA, "splats: %s has %d pixels, for %d splats" % (name, w * h, count))
+        keep.append(rgba)
+        return SogImage(rgba.ctypes.data, w, h)
+
+    def book(values):
+        a = np.ascontiguousarray(values, np.float64)
+        keep.append(a)
+        return a.ctypes.data, a.size
+
+    s = Sog()
+    s.count = count
+    s.means_l, s.means_u = image(meta["means"]["files"][0]), image(meta["means"]["files"][1])
+    s.quats = image(meta["quats"]["files"][0])
+    s.scales = image(meta["scales"]["files"][0])
+    s.sh0 = image(meta["sh0"]["files"][0])
+    bands = meta["shN"]["bands"] if meta.get("shN") else 0
+    degree = bands if bands in (1, 2, 3) else 0
+    for a in range(3):
+        s.mins[a] = meta["means"]["mins"][a]
+        s.maxs[a] = meta["means"]["maxs"][a]
+    s.scale_codebook, s.scale_codebook_size = book(meta["scales"]["codebook"])
+    s.colour_codebook, s.colour_codebook_size = book(meta["sh0"]["codebook"])
+    if degree:
+        s.bands = degree
+        s.sh_count = meta["shN"].get("count", 0xFFFFFFFF)
+        s.sh_codebook, s.sh_codebook_size = book(meta["shN"]["codebook"])
+        s.centroids = image(meta["shN"]["files"][0], check_count=False)
+        s.labels = image(meta["shN"]["files"][1])
+    p = C.c_void_p()
+    mn, mx = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    check(lib().wd_decode_sog(ctx.ptr, C.byref(s), C.byref(p), _fp(mn), _fp(mx)))
+    return Cloud._wrap(ctx, p, mn, mx)
+
+
+class Pool:
+    """A particle pool on the device (32-byte particles: position, age, velocity, lifetime)."""
+
+    def __init__(self, ctx, particles):
+        particles = np.ascontiguousarray(particles, np.uint8)
+        assert particles.nbytes % 32 == 0
+        self.ctx = ctx
+        self.count = particles.nbytes // 32
+        p = C.c_void_p()
+        check(lib().wd_pool_create(ctx.ptr, particles.ctypes.data, self.count, C.byref(p)))
+        self.ptr = p
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().wd_pool_destroy(self.ptr))
+            self.ptr = None
+
+
+def particle_view(eye, forward):
+    v = ParticleView()
+    v.eye[:3] = [float(x) for x in eye]
+    v.forward[:3] = [float(x) for x in forward]
+    return v
 
 
 class State:
@@ -343,6 +539,21 @@ class State:
         o, c, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
         check(lib().wd_state_buffers(self.ptr, C.byref(o), C.byref(c), C.byref(v)))
         return o.value, c.value, v.value
+
+    def sort_particles(self, pool, view, first=0, count=None):
+        """Sorts slots [first, first + count) of `pool` far to near (wd_particles_sort)."""
+        count = pool.count - first if count is None else count
+        check(lib().wd_particles_sort(self.ctx.ptr, pool.ptr, first, count, C.byref(view), self.ptr))
+
+    def set_ellipses(self, on):
+        """Whether each sort also writes the draw's ellipse set-up (8 floats a splat) beside cov2d."""
+        check(lib().wd_state_ellipses(self.ctx.ptr, self.ptr, 1 if on else 0))
+
+    def read_ellipses(self, n):
+        """(n, 8) f32: clip x y z w, then (along e, across e) in pixels; zeros where nothing is drawn."""
+        out = np.empty((n, 8), np.float32)
+        check(lib().wd_state_read_ellipses(self.ctx.ptr, self.ptr, out.ctypes.data, n))
+        return out
 
     def read_keys(self, n):
         keys = np.empty(n, np.uint16)

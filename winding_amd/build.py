@@ -32,8 +32,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["splat_sort.hip", "batch.hip", "mlp.hip", "learned.hip"]
-HOST_SOURCES = ["api.cpp", "synth.cpp", "batch_host.cpp", "mlp_host.cpp", "learned_host.cpp"]
+HIP_SOURCES = ["splat_sort.hip", "batch.hip", "mlp.hip", "learned.hip", "decode.hip"]
+HOST_SOURCES = ["api.cpp", "synth.cpp", "batch_host.cpp", "mlp_host.cpp", "learned_host.cpp", "decode_host.cpp"]
 
 
 def _run(cmd, cwd=ROOT):
@@ -66,7 +66,7 @@ def build_library():
     objs = _objects()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-lpthread", "-Wl,-soname,libwinding_amd.so"])
+         ["-lpthread", "-lz", "-Wl,-soname,libwinding_amd.so"])
     os.replace(tmp, LIB)
     return LIB
 
@@ -84,9 +84,9 @@ def build_addon():
 
 def build_oracle():
     os.makedirs(os.path.dirname(ORACLE_LIB), exist_ok=True)
-    src = os.path.join(ROOT, "oracle", "splat_oracle.c")
+    srcs = [os.path.join(ROOT, "oracle", f) for f in ("splat_oracle.c", "decode_oracle.c")]
     tmp = ORACLE_LIB + ".tmp"
-    _run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall", src, "-o", tmp, "-lm"])
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall"] + srcs + ["-o", tmp, "-lm"])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB
 

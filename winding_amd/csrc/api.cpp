@@ -117,6 +117,7 @@ int wd_close(wd_context *ctx) {
     hipEventDestroy(p.b);
   }
   for (auto e : ctx->spare) hipEventDestroy(e);
+  if (ctx->particle_edges) hipFree(ctx->particle_edges);
   hipStreamDestroy(ctx->stream);
   ctx->closed = true;
   delete ctx;
@@ -206,6 +207,47 @@ int wd_cloud_create_device(wd_context *ctx, const void *device_records, uint64_t
   return cloud_from(ctx, device_records, bytes, true, out, min_out, max_out);
 }
 
+int wd_cloud_create_splats(wd_context *ctx, const float *centers, const float *covariances, uint64_t count,
+                           wd_cloud **out, float min_out[3], float max_out[3]) {
+  if (!ctx || !centers || !covariances || !out) return fail(WD_E_ARG, "splats: null argument");
+  *out = nullptr;
+  if (count == 0) return fail(WD_E_DATA, "splats: no splats");
+  if (count > WD_MAX_SORT)
+    return fail(WD_E_CAPACITY, "splats: %llu splats are more than this device holds in one buffer, %u",
+                (unsigned long long)count, WD_MAX_SORT);
+  // put()'s refusal and box (src/scene/splats.js:336-348), over the f32 centres
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint64_t i = 0; i < count; ++i)
+    for (int k = 0; k < 3; ++k) {
+      const float x = centers[4 * i + k];
+      if (!std::isfinite(x))
+        return fail(WD_E_DATA, "splats: splat %llu has a position that is not a number", (unsigned long long)i);
+      mn[k] = std::min(mn[k], x);
+      mx[k] = std::max(mx[k], x);
+    }
+  WD_HIP(hipSetDevice(ctx->device), "splats: set device");
+  wd_cloud *c = new wd_cloud();
+  c->ctx = ctx;
+  c->n = count;
+  hipError_t e = hipMalloc(&c->centers, count * 16);
+  if (e == hipSuccess) e = hipMalloc(&c->covs, count * 24);
+  if (e == hipSuccess) e = hipMemcpy(c->centers, centers, count * 16, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->covs, covariances, count * 24, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (c->centers) hipFree(c->centers);
+    if (c->covs) hipFree(c->covs);
+    delete c;
+    return wd::hip_fail(e, "splats: upload");
+  }
+  std::memcpy(c->min, mn, sizeof mn);
+  std::memcpy(c->max, mx, sizeof mx);
+  if (min_out) std::memcpy(min_out, mn, sizeof mn);
+  if (max_out) std::memcpy(max_out, mx, sizeof mx);
+  ctx->live.insert(c);
+  *out = c;
+  return WD_OK;
+}
+
 int wd_cloud_info(const wd_cloud *cloud, uint64_t *count, float min_out[3], float max_out[3]) {
   if (!cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "splats: these splats were unloaded");
   if (count) *count = cloud->n;
@@ -222,6 +264,10 @@ int wd_cloud_sh(wd_context *ctx, wd_cloud *cloud, const uint32_t *words, uint64_
   if (degree && (!words || n_words < need))
     return fail(WD_E_ARG, "splats: harmonics of degree %u take %llu words for %llu splats, got %llu", degree,
                 (unsigned long long)need, (unsigned long long)cloud->n, (unsigned long long)n_words);
+  // src/render/splats.js:398-399: the harmonics must fit one buffer of this device
+  if (degree && cloud->n > ctx->mem_free_at_open / (wd::sh_words(degree) * sizeof(uint32_t)))
+    return fail(WD_E_CAPACITY, "splats: %llu splats with harmonics of degree %u are more than this device holds in one buffer",
+                (unsigned long long)cloud->n, degree);
   WD_HIP(hipSetDevice(ctx->device), "splats: set device");
   WD_HIP(hipStreamSynchronize(ctx->stream), "splats: harmonics");  // no sort in flight reads the old ones
   if (cloud->sh) hipFree(cloud->sh);
@@ -247,7 +293,9 @@ int wd_cloud_destroy(wd_cloud *cloud) {
   if (!cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "splats: destroyed twice");
   hipSetDevice(cloud->ctx->device);
   hipStreamSynchronize(cloud->ctx->stream);
-  hipFree(cloud->records);
+  if (cloud->records) hipFree(cloud->records);
+  if (cloud->centers) hipFree(cloud->centers);
+  if (cloud->covs) hipFree(cloud->covs);
   if (cloud->sh) hipFree(cloud->sh);
   cloud->ctx->live.erase(cloud);
   cloud->magic = 0;
@@ -332,7 +380,7 @@ int wd_state_destroy(wd_state *s) {
   hipStreamSynchronize(s->ctx->stream);
   for (void *p : {(void *)s->ctl, (void *)s->keys, (void *)s->mask, (void *)s->packed, (void *)s->order,
                   (void *)s->cov, (void *)s->tilehist, (void *)s->tilehist3, (void *)s->prefix_lo,
-                  (void *)s->prefix_hi, (void *)s->lut, (void *)s->prefix_bin})
+                  (void *)s->prefix_hi, (void *)s->lut, (void *)s->prefix_bin, (void *)s->ellipse})
     if (p) hipFree(p);
   s->ctx->live.erase(s);
   s->magic = 0;
@@ -345,6 +393,116 @@ int wd_state_buffers(wd_state *s, void **order, void **count, void **cov2d) {
   if (order) *order = s->order;
   if (count) *count = &s->ctl->count;
   if (cov2d) *cov2d = s->cov;
+  return WD_OK;
+}
+
+// ---- particles: the particle sort (src/render/particles.js:169-240) ----
+
+int wd_pool_create(wd_context *ctx, const void *particles, uint64_t count, wd_pool **out) {
+  if (!ctx || !particles || !out) return fail(WD_E_ARG, "particles: null argument");
+  *out = nullptr;
+  if (count == 0 || count > WD_MAX_SORT)
+    return fail(WD_E_CAPACITY, "particles: a pool of %llu particles; this sort takes 1 to %u", (unsigned long long)count,
+                WD_MAX_SORT);
+  WD_HIP(hipSetDevice(ctx->device), "particles: set device");
+  wd_pool *p = new wd_pool();
+  p->ctx = ctx;
+  p->n = count;
+  hipError_t e = hipMalloc(&p->particles, count * WD_PARTICLE_BYTES);
+  if (e == hipSuccess) e = hipMemcpy(p->particles, particles, count * WD_PARTICLE_BYTES, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (p->particles) hipFree(p->particles);
+    delete p;
+    return wd::hip_fail(e, "particles: upload");
+  }
+  ctx->live.insert(p);
+  *out = p;
+  return WD_OK;
+}
+
+int wd_pool_device(const wd_pool *pool, void **device_particles) {
+  if (!pool || pool->magic != 0x504f4f4c || !device_particles) return fail(WD_E_STATE, "particles: the pool was destroyed");
+  *device_particles = pool->particles;
+  return WD_OK;
+}
+
+int wd_pool_destroy(wd_pool *pool) {
+  if (!pool || pool->magic != 0x504f4f4c) return fail(WD_E_STATE, "particles: destroyed twice");
+  hipSetDevice(pool->ctx->device);
+  hipStreamSynchronize(pool->ctx->stream);
+  hipFree(pool->particles);
+  pool->ctx->live.erase(pool);
+  pool->magic = 0;
+  delete pool;
+  return WD_OK;
+}
+
+int wd_particles_sort(wd_context *ctx, const wd_pool *pool, uint64_t first, uint64_t count,
+                      const wd_particle_view *view, wd_state *s) {
+  if (!ctx || !view) return fail(WD_E_ARG, "particles: null argument");
+  if (!pool || !ctx->live.count(pool)) return fail(WD_E_STATE, "particles: the pool was destroyed");
+  if (!s || !ctx->live.count(s)) return fail(WD_E_STATE, "particles: the sort state was destroyed");
+  if (pool->ctx != ctx || s->ctx != ctx) return fail(WD_E_STATE, "particles: pool, state and context are of different devices");
+  if (count == 0 || first > pool->n || count > pool->n - first)
+    return fail(WD_E_ARG, "particles: slots %llu to %llu of a pool of %llu", (unsigned long long)first,
+                (unsigned long long)(first + count), (unsigned long long)pool->n);
+  if (count > s->capacity)
+    return fail(WD_E_CAPACITY, "particles: %llu particles are more than this state holds, %llu",
+                (unsigned long long)count, (unsigned long long)s->capacity);
+  if (first + count > 0xFFFFFFFFull) return fail(WD_E_ARG, "particles: slots past 2^32 are not addressed");
+  WD_HIP(hipSetDevice(ctx->device), "particles: set device");
+  if (!ctx->particle_edges) {
+    // bucket b's lower edge, 2^(b 30 / 4095 - 10) (LOG_NEAR = -10, LOG_FAR = 20, 4096 buckets)
+    double edges[WD_PARTICLE_BUCKETS];
+    edges[0] = 0;
+    for (int b = 1; b < WD_PARTICLE_BUCKETS; ++b) edges[b] = std::exp2((30.0 * b - 40950.0) / 4095.0);
+    WD_HIP(hipMalloc(&ctx->particle_edges, sizeof edges), "particles: edges");
+    WD_HIP(hipMemcpy(ctx->particle_edges, edges, sizeof edges, hipMemcpyHostToDevice), "particles: edges");
+  }
+  const double v6[6] = {view->eye[0], view->eye[1], view->eye[2], view->forward[0], view->forward[1], view->forward[2]};
+  return wd::launch_particle_sort(ctx, pool->particles, first, (uint32_t)count, v6, ctx->particle_edges, s);
+}
+
+int wd_state_ellipses(wd_context *ctx, wd_state *s, int on) {
+  if (!ctx || !s || !ctx->live.count(s)) return fail(WD_E_STATE, "ellipses: the sort state was destroyed");
+  if (s->ctx != ctx) return fail(WD_E_STATE, "ellipses: state and context are of different devices");
+  WD_HIP(hipSetDevice(ctx->device), "ellipses: set device");
+  if (on && !s->ellipse) {
+    // 8 floats a splat, then the reach per alpha byte: sqrt(min(log(255 (k / 255)), 4)), 0 where not above zero
+    float reach[256];
+    for (int k = 0; k < 256; ++k) {
+      const double r = std::log(255.0 * (k / 255.0));
+      reach[k] = r > 0 ? (float)std::sqrt(r < 4 ? r : 4.0) : 0.0f;
+    }
+    WD_HIP(hipMalloc(&s->ellipse, (size_t)s->capacity * 8 * sizeof(float) + sizeof reach), "ellipses: allocation");
+    WD_HIP(hipMemcpy(s->ellipse + (size_t)s->capacity * 8, reach, sizeof reach, hipMemcpyHostToDevice),
+           "ellipses: reach table");
+  } else if (!on && s->ellipse) {
+    WD_HIP(hipStreamSynchronize(ctx->stream), "ellipses: sync");  // no sort in flight writes them
+    hipFree(s->ellipse);
+    s->ellipse = nullptr;
+  }
+  return WD_OK;
+}
+
+int wd_state_ellipse_buffer(wd_state *s, void **ellipses) {
+  if (!s || s->magic != 0x53544154 || !ellipses) return fail(WD_E_STATE, "ellipses: destroyed state or null argument");
+  *ellipses = s->ellipse;
+  return WD_OK;
+}
+
+int wd_state_read_ellipses(wd_context *ctx, wd_state *s, float *out, uint64_t n) {
+  if (!ctx || !s || s->magic != 0x53544154 || !out) return fail(WD_E_ARG, "ellipses: null argument or destroyed state");
+  if (!s->ellipse) return fail(WD_E_STATE, "ellipses: this state does not take them; turn them on first");
+  if (n > s->last_n)
+    return fail(WD_E_ARG, "ellipses: %llu asked, the last sort had %llu splats", (unsigned long long)n,
+                (unsigned long long)s->last_n);
+  WD_HIP(hipStreamSynchronize(ctx->stream), "ellipses: sync");
+  // two planes on the device (clip positions, then axes); 8 floats a splat on the host
+  WD_HIP(hipMemcpy2D(out, 8 * sizeof(float), s->ellipse, 4 * sizeof(float), 4 * sizeof(float), (size_t)n,
+                     hipMemcpyDeviceToHost), "ellipses: read");
+  WD_HIP(hipMemcpy2D(out + 4, 8 * sizeof(float), s->ellipse + (size_t)s->capacity * 4, 4 * sizeof(float),
+                     4 * sizeof(float), (size_t)n, hipMemcpyDeviceToHost), "ellipses: read");
   return WD_OK;
 }
 
@@ -473,6 +631,7 @@ int wd_time_sorts(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, w
 
 int wd_debug_canonical(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, double *out5_host) {
   if (!ctx || !cloud || !view || !out5_host) return fail(WD_E_ARG, "canonical debug: null argument");
+  if (!cloud->records) return fail(WD_E_ARG, "canonical debug: takes a cloud of .splat records");
   double *dev = nullptr;
   const size_t bytes = cloud->n * 5 * sizeof(double);
   WD_HIP(hipMalloc(&dev, bytes), "canonical debug: alloc");

@@ -54,6 +54,10 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     if (J.cloud->ctx != b->ctx || J.state->ctx != b->ctx) return fail(WD_E_STATE, "batch: job %u is of another device", i);
     if (J.cloud->degree)
       return fail(WD_E_ARG, "batch: job %u: its splats have harmonics, which only a single sort (wd_sort) shades", i);
+    if (J.state->ellipse)
+      return fail(WD_E_ARG, "batch: job %u: its state takes the draw's ellipses, which only a single sort (wd_sort) writes", i);
+    if (!J.cloud->records)
+      return fail(WD_E_ARG, "batch: job %u: its splats are centres and covariances, which only a single sort (wd_sort) takes", i);
     if (J.cloud->n > J.state->capacity)
       return fail(WD_E_CAPACITY, "batch: job %u: %llu splats are more than its state holds, %llu", i,
                   (unsigned long long)J.cloud->n, (unsigned long long)J.state->capacity);
@@ -127,6 +131,9 @@ int wd_batch_run(wd_context *ctx, wd_batch *b) {
     if (!ctx->live.count(p)) return fail(WD_E_STATE, "batch: a job's splats or sort state was destroyed; set the jobs again");
   for (const wd_job &J : b->api_jobs)
     if (J.cloud->degree) return fail(WD_E_ARG, "batch: a job's splats have harmonics, which only a single sort (wd_sort) shades");
+  for (const wd_job &J : b->api_jobs)
+    if (J.state->ellipse)
+      return fail(WD_E_ARG, "batch: a job's state takes the draw's ellipses, which only a single sort (wd_sort) writes");
   WD_HIP(hipSetDevice(ctx->device), "batch: set device");
   if (ctx->profiling) return wd::launch_batch(ctx, b);  // timed launches go straight to the stream
   if (!b->exec) {

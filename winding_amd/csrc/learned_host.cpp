@@ -92,6 +92,8 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
   if (!cloud || !ctx->live.count(cloud)) return fail(WD_E_STATE, "learned: the splats were unloaded");
   if (cloud->ctx != ctx) return fail(WD_E_STATE, "learned: the splats are of another device");
   if (cloud->degree) return fail(WD_E_ARG, "learned: the splats have harmonics, which only a single sort (wd_sort) shades");
+  if (!cloud->records)
+    return fail(WD_E_ARG, "learned: the splats are centres and covariances, which only a single sort (wd_sort) takes");
   const uint32_t n = (uint32_t)cloud->n;
   if (cloud->n > wd::kLearnedMaxSplats)
     return fail(WD_E_CAPACITY, "learned: %llu splats are more than a learned sort takes, %u",
@@ -100,6 +102,8 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
     wd_state *s = states[v];
     if (!s || !ctx->live.count(s)) return fail(WD_E_STATE, "learned: view %u: its sort state was destroyed", v);
     if (s->ctx != ctx) return fail(WD_E_STATE, "learned: view %u: its sort state is of another device", v);
+    if (s->ellipse)
+      return fail(WD_E_ARG, "learned: view %u: its state takes the draw's ellipses, which only a single sort (wd_sort) writes", v);
     if (cloud->n > s->capacity)
       return fail(WD_E_CAPACITY, "learned: view %u: %u splats are more than its state holds, %llu", v, n,
                   (unsigned long long)s->capacity);

@@ -101,24 +101,10 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) { re
 // put() (src/scene/splats.js:350-368), footprint() (src/render/splats.js:
 // 102-125) and the area test (:174-178), in the oracle's expression order.
 // Returns alpha * area.
-__device__ __forceinline__ double canonical_value(const ViewK &v, uint4 a, uint4 b, double vx, double vy, double vz) {
-  const double sx = f32_of(a.w), sy = f32_of(b.x), sz = f32_of(b.y);
-  const uint32_t q = b.w;
-  const double qw = ((int)(q & 255u) - 128) / 128.0, qx = ((int)((q >> 8) & 255u) - 128) / 128.0;
-  const double qy = ((int)((q >> 16) & 255u) - 128) / 128.0, qz = ((int)(q >> 24) - 128) / 128.0;
-  double length = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
-  if (length == 0) length = 1;
-  const double w = qw / length, u = qx / length, vq = qy / length, t = qz / length;
-  const double r00 = 1 - 2 * (vq * vq + t * t), r01 = 2 * (u * vq - w * t), r02 = 2 * (u * t + w * vq);
-  const double r10 = 2 * (u * vq + w * t), r11 = 1 - 2 * (u * u + t * t), r12 = 2 * (vq * t - w * u);
-  const double r20 = 2 * (u * t - w * vq), r21 = 2 * (vq * t + w * u), r22 = 1 - 2 * (u * u + vq * vq);
-  const double m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
-  const double m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
-  const double m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
-  const double S[3][3] = {
-      {m00 * m00 + m01 * m01 + m02 * m02, m00 * m10 + m01 * m11 + m02 * m12, m00 * m20 + m01 * m21 + m02 * m22},
-      {m00 * m10 + m01 * m11 + m02 * m12, m10 * m10 + m11 * m11 + m12 * m12, m10 * m20 + m11 * m21 + m12 * m22},
-      {m00 * m20 + m01 * m21 + m02 * m22, m10 * m20 + m11 * m21 + m12 * m22, m20 * m20 + m21 * m21 + m22 * m22}};
+// The FP64 footprint (footprint(), src/render/splats.js:102-125) of a splat
+// whose Sigma is S, at view position (vx, vy, vz): T = J W, cov = (T S) T^T.
+__device__ __forceinline__ void footprint64(const ViewK &v, const double (&S)[3][3], double vx, double vy, double vz,
+                                            double &xx, double &xy, double &yy) {
   const double p00 = v.dP[0], p11 = v.dP[5];
   const double fx = p00 * (double)v.vpw * 0.5, fy = p11 * (double)v.vph * 0.5;
   double j00 = fx, j11 = fy, j02 = 0, j12 = 0;
@@ -147,13 +133,39 @@ __device__ __forceinline__ double canonical_value(const ViewK &v, uint4 a, uint4
     u0[c] = t0[0] * S[0][c] + t0[1] * S[1][c] + t0[2] * S[2][c];
     u1[c] = t1[0] * S[0][c] + t1[1] * S[1][c] + t1[2] * S[2][c];
   }
-  const double xx = u0[0] * t0[0] + u0[1] * t0[1] + u0[2] * t0[2];
-  const double xy = u0[0] * t1[0] + u0[1] * t1[1] + u0[2] * t1[2];
-  const double yy = u1[0] * t1[0] + u1[1] * t1[1] + u1[2] * t1[2];
+  xx = u0[0] * t0[0] + u0[1] * t0[1] + u0[2] * t0[2];
+  xy = u0[0] * t1[0] + u0[1] * t1[1] + u0[2] * t1[2];
+  yy = u1[0] * t1[0] + u1[1] * t1[1] + u1[2] * t1[2];
+}
+
+// alpha * area, the cull's value (src/render/splats.js:169-176), in FP64.
+__device__ __forceinline__ double cull_value64(double xx, double xy, double yy, uint32_t abyte) {
   const double det = (xx + kC03) * (yy + kC03) - xy * xy;
   const double area = kC2PI * sqrt(det > 0 ? det : 0);
-  const double alpha = (double)(b.z >> 24) / 255.0;
-  return alpha * area;
+  return (double)abyte / 255.0 * area;
+}
+
+__device__ __forceinline__ double canonical_value(const ViewK &v, uint4 a, uint4 b, double vx, double vy, double vz) {
+  const double sx = f32_of(a.w), sy = f32_of(b.x), sz = f32_of(b.y);
+  const uint32_t q = b.w;
+  const double qw = ((int)(q & 255u) - 128) / 128.0, qx = ((int)((q >> 8) & 255u) - 128) / 128.0;
+  const double qy = ((int)((q >> 16) & 255u) - 128) / 128.0, qz = ((int)(q >> 24) - 128) / 128.0;
+  double length = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+  if (length == 0) length = 1;
+  const double w = qw / length, u = qx / length, vq = qy / length, t = qz / length;
+  const double r00 = 1 - 2 * (vq * vq + t * t), r01 = 2 * (u * vq - w * t), r02 = 2 * (u * t + w * vq);
+  const double r10 = 2 * (u * vq + w * t), r11 = 1 - 2 * (u * u + t * t), r12 = 2 * (vq * t - w * u);
+  const double r20 = 2 * (u * t - w * vq), r21 = 2 * (vq * t + w * u), r22 = 1 - 2 * (u * u + vq * vq);
+  const double m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
+  const double m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
+  const double m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
+  const double S[3][3] = {
+      {m00 * m00 + m01 * m01 + m02 * m02, m00 * m10 + m01 * m11 + m02 * m12, m00 * m20 + m01 * m21 + m02 * m22},
+      {m00 * m10 + m01 * m11 + m02 * m12, m10 * m10 + m11 * m11 + m12 * m12, m10 * m20 + m11 * m21 + m12 * m22},
+      {m00 * m20 + m01 * m21 + m02 * m22, m10 * m20 + m11 * m21 + m12 * m22, m20 * m20 + m21 * m21 + m22 * m22}};
+  double xx, xy, yy;
+  footprint64(v, S, vx, vy, vz, xx, xy, yy);
+  return cull_value64(xx, xy, yy, b.z >> 24);
 }
 
 // One splat through makeKeys. STD: the projection is the reverse-Z infinite
@@ -165,6 +177,7 @@ struct Projected {
   bool recomputed;
   uint32_t key;
   float xx, xy, yy;
+  float cx, cy, cz, cw;  // clip position, the FP64 value rounded to f32
 };
 
 // What the footprint needs of a splat whatever the view: its rotation from
@@ -200,14 +213,15 @@ __device__ __forceinline__ SplatLocal splat_local(uint4 a, uint4 b) {
   return L;
 }
 
-template <bool STD, bool HAVE_LOCAL>
-__device__ __forceinline__ Projected project_with(const ViewK &v, uint4 a, uint4 b, const SplatLocal *Lp,
-                                                  float bn0w0, float bn1w1, float bw2) {
-  const double x = f32_of(a.x), y = f32_of(a.y), z = f32_of(a.z);
+// The part of makeKeys every splat form shares (src/render/splats.js:157-181):
+// view and clip position, key, and the frustum and alpha tests, all FP64.
+template <bool STD>
+__device__ __forceinline__ Projected project_front(const ViewK &v, double x, double y, double z, uint32_t abyte,
+                                                   double &vx, double &vy, double &vz) {
   // view = modelView * (p, 1); each product is exact, so fma is the sum as written.
-  const double vx = fma(v.dmv[8], z, fma(v.dmv[4], y, v.dmv[0] * x)) + v.dmv[12];
-  const double vy = fma(v.dmv[9], z, fma(v.dmv[5], y, v.dmv[1] * x)) + v.dmv[13];
-  const double vz = fma(v.dmv[10], z, fma(v.dmv[6], y, v.dmv[2] * x)) + v.dmv[14];
+  vx = fma(v.dmv[8], z, fma(v.dmv[4], y, v.dmv[0] * x)) + v.dmv[12];
+  vy = fma(v.dmv[9], z, fma(v.dmv[5], y, v.dmv[1] * x)) + v.dmv[13];
+  vz = fma(v.dmv[10], z, fma(v.dmv[6], y, v.dmv[2] * x)) + v.dmv[14];
   double cx, cy, cz, cw;
   if (STD) {
     cx = v.dP[0] * vx + v.dP[8] * vz;
@@ -226,7 +240,6 @@ __device__ __forceinline__ Projected project_with(const ViewK &v, uint4 a, uint4
     cz = c[2];
     cw = c[3];
   }
-  const uint32_t abyte = b.z >> 24;
   // Far first: the farthest key is the smallest (src/render/splats.js:179-181).
   double t = (-vz - v.ddmin) * v.ddscale;
   t = t < 0.0 ? 0.0 : t;
@@ -234,8 +247,41 @@ __device__ __forceinline__ Projected project_with(const ViewK &v, uint4 a, uint4
   Projected o;
   o.key = 65535u - (uint32_t)t;
   o.recomputed = false;
+  o.cx = (float)cx;
+  o.cy = (float)cy;
+  o.cz = (float)cz;
+  o.cw = (float)cw;
   o.xx = o.xy = o.yy = 0.f;
   o.keep = !((vz >= 0) | (cz > cw) | (fabs(cx) > kC13 * cw) | (fabs(cy) > kC13 * cw) | (abyte <= 1u));
+  return o;
+}
+
+// A splat given as its centre and its f32 Sigma (the reference's Splats data,
+// src/render/splats.js:389-416: centres with the colour in .w, six covariance
+// floats xx xy xz yy yz zz): makeKeys with the footprint and cull in FP64.
+template <bool STD>
+__device__ __forceinline__ Projected project_sigma(const ViewK &v, uint4 centre, const float (&c6)[6]) {
+  double vx, vy, vz;
+  const uint32_t abyte = centre.w >> 24;
+  Projected o = project_front<STD>(v, f32_of(centre.x), f32_of(centre.y), f32_of(centre.z), abyte, vx, vy, vz);
+  if (o.keep) {
+    const double S[3][3] = {{c6[0], c6[1], c6[2]}, {c6[1], c6[3], c6[4]}, {c6[2], c6[4], c6[5]}};
+    double xx, xy, yy;
+    footprint64(v, S, vx, vy, vz, xx, xy, yy);
+    o.xx = (float)xx;
+    o.xy = (float)xy;
+    o.yy = (float)yy;
+    if (v.cull > 0.f) o.keep = !(cull_value64(xx, xy, yy, abyte) < v.dcull);
+  }
+  return o;
+}
+
+template <bool STD, bool HAVE_LOCAL>
+__device__ __forceinline__ Projected project_with(const ViewK &v, uint4 a, uint4 b, const SplatLocal *Lp,
+                                                  float bn0w0, float bn1w1, float bw2) {
+  double vx, vy, vz;
+  const uint32_t abyte = b.z >> 24;
+  Projected o = project_front<STD>(v, f32_of(a.x), f32_of(a.y), f32_of(a.z), abyte, vx, vy, vz);
   if (o.keep) {
     // f32 from here to the decision: fusing multiply-adds only tightens the bound below.
 #pragma clang fp contract(fast)
@@ -391,17 +437,63 @@ __device__ __forceinline__ uint32_t shade(const ViewK &v, const uint32_t *__rest
   return byte(rgba, 0, c.r) | (byte(rgba, 1, c.g) << 8) | (byte(rgba, 2, c.b) << 16) | (rgba & 0xFF000000u);
 }
 
+// The splat vertex shader's per-instance set-up (src/render/splats.js:263-291)
+// for a survivor, made once here beside cov2d so that a rasteriser reads it
+// instead of projecting the splat again. From the footprint plus a third of a
+// pixel each way, C = [[a, b], [b, c]]: the eigen-axes, each as long as
+// sqrt(2 lambda) capped at 1024 px, scaled by how far the splat reaches,
+// e = sqrt(min(log(255 alpha), 4)). Writes (along e, across e) in pixels; the
+// quad's corner q in {-1, 1}^2 is at clip.xy + (q.x along e + q.y across e)
+// 2 / viewport clip.w, with the Gaussian's offset q e. Zeros when the shader
+// would draw nothing (minor axis not > 0, or reach not > 0: alpha byte < 2).
+// f32 with the hardware square root and reciprocal square root (about 1 ulp
+// each); e comes from a 256-entry table of the FP64 value per alpha byte
+// (reach[0] = reach[1] = 0). The shader's direction (b, major - a) loses its
+// second component to cancellation when |b| << a - c, so for a >= c it is
+// taken as the same direction written without the difference,
+// (sign(b) (radius + h), |b|), since major - a = radius - h = b^2 / (radius + h).
+__device__ __forceinline__ f32x4 ellipse_axes(float xx, float xy, float yy, uint32_t abyte,
+                                              const float *__restrict__ reach) {
+  const float a = xx + 0.3f, b = xy, c = yy + 0.3f;
+  const float mid = 0.5f * (a + c);
+  const float h = 0.5f * (a - c);
+  const float radius = __builtin_amdgcn_sqrtf(h * h + b * b);
+  const float major = mid + radius, minor = mid - radius;
+  const float e = reach[abyte];
+  if (!(minor > 0.0f) || !(e > 0.0f)) return f32x4{0.f, 0.f, 0.f, 0.f};
+  float ax = 1.0f, ay = 0.0f;
+  if (fabsf(b) > 1e-12f) {
+    const float ux = h >= 0.0f ? copysignf(radius + h, b) : b;
+    const float uy = h >= 0.0f ? fabsf(b) : radius - h;
+    const float inv = __builtin_amdgcn_rsqf(ux * ux + uy * uy);
+    ax = ux * inv;
+    ay = uy * inv;
+  } else if (c > a) {
+    ax = 0.0f;
+    ay = 1.0f;
+  }
+  const float la = fminf(__builtin_amdgcn_sqrtf(2.0f * major), 1024.0f) * e;
+  const float lc = fminf(__builtin_amdgcn_sqrtf(2.0f * minor), 1024.0f) * e;
+  return f32x4{la * ax, la * ay, lc * ay, -(lc * ax)};
+}
+
 // BINS (the learned sort): the chunk's histogram is over the view's 512 bins
 // instead of over the two key bytes; a survivor's bin comes from the view's
 // block table, already in LDS (bin_key).
 // SHD > 0: the survivors' colour is shaded by the cloud's harmonics (shade).
-template <bool STD, bool BINS = false, int SHD = 0>
+// ELL: each splat's draw set-up goes to two planes of n_cap float4s, its clip
+// position to ell[i] and its ellipse_axes to ell[n_cap + i] (each store of a
+// wave covers whole lines), zeros for one not drawn.
+template <bool STD, bool BINS = false, int SHD = 0, bool ELL = false, bool SIG = false>
 __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
                                               const ViewK &v, uint16_t *__restrict__ keys,
                                               uint64_t *__restrict__ mask, float4 *__restrict__ cov,
                                               uint16_t *__restrict__ tilehist, uint32_t *__restrict__ recomp_acc,
                                               uint32_t chunk, const uint16_t *blk = nullptr,
-                                              const uint32_t *__restrict__ sh = nullptr) {
+                                              const uint32_t *__restrict__ sh = nullptr,
+                                              float4 *__restrict__ ell = nullptr,
+                                              const float *__restrict__ reach = nullptr,
+                                              const uint2 *__restrict__ sig = nullptr, uint64_t n_cap = 0) {
   const uint32_t tid = threadIdx.x;
   __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 512; k += 256) sm.h[k] = 0;
@@ -416,17 +508,41 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
   // Three register buffers: two tiles' records are in flight while a tile is
   // worked on, which hides the load latency at four waves a SIMD better than
   // one tile in flight does at five.
-  auto load = [&](uint32_t tile, uint4 &a, uint4 &b) {
+  // SIG: rec holds the centres (16 bytes a splat) and sig the six covariance
+  // floats (24 bytes a splat, three 8-byte loads); a is the centre, b and c Sigma.
+  struct Slot {
+    uint4 a, b;
+    uint2 c;
+  };
+  auto load = [&](uint32_t tile, Slot &s) {
     const uint32_t i = (tile << 8) + tid;
     if (tile < end && i < n) {
-      a = rec[2 * (size_t)i];
-      b = rec[2 * (size_t)i + 1];
+      if constexpr (SIG) {
+        s.a = rec[i];
+        const uint2 s0 = sig[3 * (size_t)i], s1 = sig[3 * (size_t)i + 1];
+        s.b = make_uint4(s0.x, s0.y, s1.x, s1.y);
+        s.c = sig[3 * (size_t)i + 2];
+      } else {
+        s.a = rec[2 * (size_t)i];
+        s.b = rec[2 * (size_t)i + 1];
+      }
     }
   };
-  auto work = [&](uint32_t tile, const uint4 &a, const uint4 &b) {
+  auto work = [&](uint32_t tile, const Slot &s) {
+    const uint4 &a = s.a, &b = s.b;
     const uint32_t i = (tile << 8) + tid;
     const bool in = i < n;
-    Projected p = project_one<STD>(v, a, b, bw0, bw1, bw2);
+    Projected p;
+    uint32_t rgba;
+    if constexpr (SIG) {
+      const float c6[6] = {__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z),
+                           __uint_as_float(b.w), __uint_as_float(s.c.x), __uint_as_float(s.c.y)};
+      p = project_sigma<STD>(v, a, c6);
+      rgba = a.w;
+    } else {
+      p = project_one<STD>(v, a, b, bw0, bw1, bw2);
+      rgba = b.z;
+    }
     p.keep = p.keep && in;
     const uint64_t bal = __ballot(p.keep);
     if ((tid & 63u) == 0 && in) mask[i >> 6] = bal;
@@ -435,11 +551,18 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       // Streaming stores for the keys and cov2d (the sort never reads cov2d back; the
       // keys are read once, by the next pass): lines that sit dirty in L2 at the end
       // of the launch are written back before the next launch starts.
-      uint32_t colour = b.z;
+      uint32_t colour = rgba;
       if constexpr (SHD > 0)
-        if (p.keep) colour = shade<SHD>(v, sh, i, f32_of(a.x), f32_of(a.y), f32_of(a.z), b.z);
+        if (p.keep) colour = shade<SHD>(v, sh, i, f32_of(a.x), f32_of(a.y), f32_of(a.z), rgba);
       const f32x4 c = p.keep ? f32x4{p.xx, p.xy, p.yy, __uint_as_float(colour)} : f32x4{0.f, 0.f, 0.f, 0.f};
       __builtin_nontemporal_store(c, (f32x4 *)&cov[i]);
+      if constexpr (ELL) {
+        const f32x4 zero{0.f, 0.f, 0.f, 0.f};
+        const f32x4 axes = p.keep ? ellipse_axes(p.xx, p.xy, p.yy, rgba >> 24, reach) : zero;
+        const bool drawn = axes[0] != 0.f || axes[1] != 0.f;
+        __builtin_nontemporal_store(drawn ? f32x4{p.cx, p.cy, p.cz, p.cw} : zero, (f32x4 *)&ell[i]);
+        __builtin_nontemporal_store(axes, (f32x4 *)&ell[n_cap + i]);
+      }
     }
     recomp += (p.recomputed && in) ? 1u : 0u;
     if (BINS) {
@@ -449,16 +572,16 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       atomicAdd(&sm.h[256u + (p.key >> 8)], 1u);
     }
   };
-  uint4 a0 = make_uint4(0, 0, 0, 0), b0 = a0, a1 = a0, b1 = a0, a2 = a0, b2 = a0;
-  load(first, a0, b0);
-  load(first + 1, a1, b1);
+  Slot s0{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint2(0, 0)}, s1 = s0, s2 = s0;
+  load(first, s0);
+  load(first + 1, s1);
   for (uint32_t tile = first; tile < end; tile += 3) {
-    load(tile + 2, a2, b2);
-    work(tile, a0, b0);
-    load(tile + 3, a0, b0);
-    if (tile + 1 < end) work(tile + 1, a1, b1);
-    load(tile + 4, a1, b1);
-    if (tile + 2 < end) work(tile + 2, a2, b2);
+    load(tile + 2, s2);
+    work(tile, s0);
+    load(tile + 3, s0);
+    if (tile + 1 < end) work(tile + 1, s1);
+    load(tile + 4, s1);
+    if (tile + 2 < end) work(tile + 2, s2);
   }
   if (recomp) atomicAdd(&sm.recomp, recomp);
   __syncthreads();
@@ -626,7 +749,8 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
                                               const uint32_t *__restrict__ in_packed, uint32_t n,
                                               const uint16_t *__restrict__ hist, uint32_t hstride,
                                               const uint32_t *__restrict__ prefix, const uint32_t *__restrict__ total,
-                                              Ctl *__restrict__ ctl, uint32_t *__restrict__ out) {
+                                              Ctl *__restrict__ ctl, uint32_t *__restrict__ out,
+                                              uint32_t out_base = 0) {
   constexpr int KPT = kKeysPerThread;
   constexpr uint32_t kLocalBits = S * kTile <= 8192 ? 13u : 14u;
   static_assert(S * kTile <= 16384, "index within the super-tile in 14 bits");
@@ -754,7 +878,7 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const uint32_t p = p0 + (uint32_t)k * 256u + tid;
-      const uint32_t value = HI ? (e[k] & 0xFFFFFFu)
+      const uint32_t value = HI ? (e[k] & 0xFFFFFFu) + out_base
                                 : (((e[k] >> kLocalBits) & 255u) << 24) | (sbase + (e[k] & ((1u << kLocalBits) - 1u)));
       // n bounds both outputs; a guard against a miscount. The draw order is the
       // path's output: streamed out, so that the next launch does not start behind

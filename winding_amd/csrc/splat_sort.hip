@@ -38,15 +38,96 @@ namespace wd {
 namespace {
 
 // Pass 1: a workgroup a chunk of kTile splats. SHD > 0: a cloud with
-// harmonics of that degree, each survivor's colour shaded.
-template <bool STD, int SHD>
+// harmonics of that degree, each survivor's colour shaded. ELL: the draw's
+// ellipse set-up is written beside cov2d.
+// SIG: a cloud given as centres and Sigma (rec = the centres, sig = the
+// covariances), its footprint and cull in FP64.
+template <bool STD, int SHD, bool ELL, bool SIG>
 __global__ __launch_bounds__(256) void k_project(const uint4 *__restrict__ rec, uint32_t n, ViewK v,
                                                  uint16_t *__restrict__ keys, uint64_t *__restrict__ mask,
                                                  float4 *__restrict__ cov, uint16_t *__restrict__ tilehist,
-                                                 Ctl *__restrict__ ctl, const uint32_t *__restrict__ sh) {
+                                                 Ctl *__restrict__ ctl, const uint32_t *__restrict__ sh,
+                                                 float4 *__restrict__ ell, uint64_t n_cap,
+                                                 const uint2 *__restrict__ sig) {
   __shared__ ProjectSmem sm;
-  project_chunk<STD, false, SHD>(sm, rec, n, v, keys, mask, cov, tilehist, &ctl->recomputed_acc, blockIdx.x,
-                                 nullptr, sh);
+  project_chunk<STD, false, SHD, ELL, SIG>(
+      sm, rec, n, v, keys, mask, cov, tilehist, &ctl->recomputed_acc, blockIdx.x, nullptr, sh, ell,
+      ELL ? reinterpret_cast<const float *>(ell + 2 * (size_t)n_cap) : nullptr, sig, n_cap);
+}
+
+// The particle sort's key pass (makeKeys, src/render/particles.js:198-211) over
+// a ring of n particles (32 bytes each: position, age, velocity, lifetime), a
+// workgroup a chunk of kTile, in the layout the splat passes after it take:
+// every particle's key and survivor bit, and the chunk's two digit histograms.
+// A live particle (!(age >= lifetime)) gets 4095 - bucket, far first. The
+// depth is FP64 from the f32 inputs, clamped below at 1e-6f (a NaN depth
+// takes the clamp); the bucket is the number of the 4095 FP64 edges at or
+// below it (edges[b] = 2^((30 b - 40950) / 4095), edges[0] = 0), so no key
+// depends on how a log2 rounds.
+struct ParticleK {
+  double eye[3], fwd[3];
+};
+
+__global__ __launch_bounds__(256) void k_particle_keys(const uint4 *__restrict__ pool, uint32_t n, ParticleK v,
+                                                       const double *__restrict__ edges, uint16_t *__restrict__ keys,
+                                                       uint64_t *__restrict__ mask, uint16_t *__restrict__ tilehist) {
+  __shared__ uint32_t h[512];
+  const uint32_t tid = threadIdx.x, chunk = blockIdx.x;
+  for (uint32_t k = tid; k < 512; k += 256) h[k] = 0;
+  __syncthreads();
+  const uint32_t first = chunk * (kTile / 256), end = min(first + kTile / 256, (n + 255u) >> 8);
+  auto load = [&](uint32_t tile, uint4 &a, uint32_t &life) {
+    const uint32_t i = (tile << 8) + tid;
+    if (tile < end && i < n) {
+      a = pool[2 * (size_t)i];
+      life = reinterpret_cast<const uint32_t *>(pool + 2 * (size_t)i + 1)[3];
+    }
+  };
+  auto work = [&](uint32_t tile, const uint4 &a, uint32_t life) {
+    const uint32_t i = (tile << 8) + tid;
+    const bool in = i < n;
+    const bool live = in && !(__uint_as_float(a.w) >= __uint_as_float(life));
+    uint32_t key = 0;
+    if (live) {
+      double d = ((double)__uint_as_float(a.x) - v.eye[0]) * v.fwd[0];
+      d = d + ((double)__uint_as_float(a.y) - v.eye[1]) * v.fwd[1];
+      d = d + ((double)__uint_as_float(a.z) - v.eye[2]) * v.fwd[2];
+      if (!(d >= (double)1e-6f)) d = (double)1e-6f;
+      // The bucket from the FP64 log2 (within a few ulps): it equals the edges'
+      // answer unless t lies within 1e-9 of a whole number, far beyond both the
+      // log2's error and the edges' (8 ulps of each true edge); only there are
+      // the edges read. The loops end by 4095 steps in any case.
+      const double t = (log2(d) + 10.0) * 4095.0 / 30.0;
+      const double f = floor(t);
+      int b = (int)fmin(fmax(f, 0.0), 4095.0);
+      if (t - f < 1e-9 || t - f > 1.0 - 1e-9) {
+        while (b < 4095 && d >= edges[b + 1]) ++b;
+        while (b > 0 && d < edges[b]) --b;
+      }
+      key = 4095u - (uint32_t)b;
+    }
+    const uint64_t bal = __ballot(live);
+    if ((tid & 63u) == 0 && in) mask[i >> 6] = bal;
+    if (in) __builtin_nontemporal_store((uint16_t)key, &keys[i]);
+    if (live) {
+      atomicAdd(&h[key & 255u], 1u);
+      atomicAdd(&h[256u + (key >> 8)], 1u);
+    }
+  };
+  uint4 a0 = make_uint4(0, 0, 0, 0), a1 = a0, a2 = a0;
+  uint32_t l0 = 0, l1 = 0, l2 = 0;
+  load(first, a0, l0);
+  load(first + 1, a1, l1);
+  for (uint32_t tile = first; tile < end; tile += 3) {
+    load(tile + 2, a2, l2);
+    work(tile, a0, l0);
+    load(tile + 3, a0, l0);
+    if (tile + 1 < end) work(tile + 1, a1, l1);
+    load(tile + 4, a1, l1);
+    if (tile + 2 < end) work(tile + 2, a2, l2);
+  }
+  __syncthreads();
+  for (uint32_t k = tid; k < 512; k += 256) tilehist[(size_t)chunk * 512 + k] = (uint16_t)h[k];
 }
 
 // Scan of pass 2: low-byte tile prefixes, and both digits' totals. Also moves
@@ -69,10 +150,10 @@ __global__ __launch_bounds__(256) void k_scatter(const uint16_t *__restrict__ ke
                                                  const uint32_t *__restrict__ in_packed, uint32_t n,
                                                  const uint16_t *__restrict__ hist, uint32_t hstride,
                                                  const uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl,
-                                                 uint32_t *__restrict__ out) {
+                                                 uint32_t *__restrict__ out, uint32_t out_base) {
   __shared__ ScatterSmem<kSuper> sm;
   scatter_super<HI, kSuper>(sm, xcd_item(blockIdx.x, gridDim.x), keys, mask, in_packed, n, hist, hstride, prefix,
-                            HI ? ctl->total + 256 : ctl->total, ctl, out);
+                            HI ? ctl->total + 256 : ctl->total, ctl, out, out_base);
 }
 
 // Upsweep of pass 3: a workgroup a tile of pass 2's output.
@@ -224,7 +305,7 @@ ViewK make_viewk(const wd_view &v) {
   k.dcull = (double)v.cull;
   // The eye in the capture's space: the translation of the model-view's
   // inverse, in FP64 from the f32 matrix and stored as f32, as
-  // SplatPass.prepare writes it (src/render/splats.js:580) with mat4Invert's
+  // SplatPass.prepare writes it (src/render/splats.js:585-586) with mat4Invert's
   // cofactor expansion (src/core/math/mat4.js:186-228). A singular model-view
   // leaves it at the origin.
   const float *m = v.model_view;
@@ -252,7 +333,7 @@ ViewK make_viewk(const wd_view &v) {
 
 const char *const kKernelNames[kKernelCount] = {
     "k_bounds", "k_project", "k_scan", "k_scatter_lo", "k_hist",        "k_scan2",
-    "k_scatter_hi", "k_batch", "k_mlp", "k_bin_table", "k_bin_scatter", "k_bin_sort"};
+    "k_scatter_hi", "k_batch", "k_mlp", "k_bin_table", "k_bin_scatter", "k_bin_sort", "k_particle_keys"};
 
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8) {
   static const uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0, 0xFFFFFFFFu, 0};
@@ -281,26 +362,75 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
     WD_HIP(hipGetLastError(), what);              \
     if (ctx->prof_end(id, ev)) return WD_E_HIP;   \
   } while (0)
-  // the projection's form: the fast reverse-Z one or the general one, times the harmonics' degree
+  // the projection's form: the fast reverse-Z one or the general one, times the
+  // harmonics' degree, times whether the state takes the draw's ellipses
   using ProjectKernel = void (*)(const uint4 *, uint32_t, ViewK, uint16_t *, uint64_t *, float4 *, uint16_t *, Ctl *,
-                                 const uint32_t *);
-  static const ProjectKernel kForms[2][4] = {
-      {k_project<false, 0>, k_project<false, 1>, k_project<false, 2>, k_project<false, 3>},
-      {k_project<true, 0>, k_project<true, 1>, k_project<true, 2>, k_project<true, 3>}};
-  const ProjectKernel project = kForms[std_form ? 1 : 0][cloud->degree <= 3 ? cloud->degree : 0];
-  WD_LAUNCH(kKProject, "sort: project launch", project, dim3(tiles), dim3(256), 0, ctx->stream, rec, n, vk, st->keys,
-            st->mask, (float4 *)st->cov, st->tilehist, st->ctl, (const uint32_t *)cloud->sh);
+                                 const uint32_t *, float4 *, uint64_t, const uint2 *);
+#define WD_FORMS(STD, SHD) \
+  {{k_project<STD, SHD, false, false>, k_project<STD, SHD, false, true>}, \
+   {k_project<STD, SHD, true, false>, k_project<STD, SHD, true, true>}}
+  // [reverse-Z form][harmonics degree][ellipses][centres and Sigma]
+  static const ProjectKernel kForms[2][4][2][2] = {
+      {WD_FORMS(false, 0), WD_FORMS(false, 1), WD_FORMS(false, 2), WD_FORMS(false, 3)},
+      {WD_FORMS(true, 0), WD_FORMS(true, 1), WD_FORMS(true, 2), WD_FORMS(true, 3)}};
+#undef WD_FORMS
+  const bool sig = cloud->covs != nullptr;
+  const ProjectKernel project =
+      kForms[std_form ? 1 : 0][cloud->degree <= 3 ? cloud->degree : 0][st->ellipse ? 1 : 0][sig ? 1 : 0];
+  const uint4 *first = sig ? (const uint4 *)cloud->centers : rec;
+  WD_LAUNCH(kKProject, "sort: project launch", project, dim3(tiles), dim3(256), 0, ctx->stream, first, n, vk, st->keys,
+            st->mask, (float4 *)st->cov, st->tilehist, st->ctl, (const uint32_t *)cloud->sh, (float4 *)st->ellipse,
+            st->capacity, (const uint2 *)cloud->covs);
   WD_LAUNCH(kKScan, "sort: scan launch", k_scan, dim3(512 / kScanCols), dim3(256), 0, ctx->stream, tiles,
             st->tilehist, st->prefix_lo, st->ctl);
   WD_LAUNCH(kKScatterLo, "sort: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0, ctx->stream,
-            st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->ctl, st->packed);
+            st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->ctl, st->packed, 0u);
   WD_LAUNCH(kKHist, "sort: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, st->packed, st->ctl,
             st->tilehist3);
   WD_LAUNCH(kKScan2, "sort: second scan launch", k_scan2, dim3(256 / kScanCols), dim3(256), 0, ctx->stream,
             st->tilehist3, st->prefix_hi, st->ctl);
   WD_LAUNCH(kKScatterHi, "sort: high-byte pass launch", k_scatter<true>, dim3(supers), dim3(256), 0, ctx->stream,
             (const uint16_t *)nullptr, (const uint64_t *)nullptr, st->packed, n, st->tilehist3, 256u, st->prefix_hi,
-            st->ctl, st->order);
+            st->ctl, st->order, 0u);
+#undef WD_LAUNCH
+  return WD_OK;
+}
+
+// The particle sort: the key pass above, then the splat sort's five passes
+// unchanged (a 12-bit key is a 16-bit key whose high byte is below 16). The
+// draw order holds slots: first + the particle's index in the ring.
+int launch_particle_sort(wd_context *ctx, const uint8_t *pool, uint64_t first, uint32_t n, const double *view6,
+                         const double *edges, wd_state *st) {
+  st->last_n = n;
+  const uint32_t tiles = (n + kTile - 1) / kTile;
+  const uint32_t supers = (tiles + kSuper - 1) / kSuper;
+  ParticleK v;
+  for (int k = 0; k < 3; ++k) {
+    v.eye[k] = view6[k];
+    v.fwd[k] = view6[3 + k];
+  }
+  hipEvent_t ev = nullptr;
+#define WD_LAUNCH(id, what, ...)                  \
+  do {                                            \
+    if (ctx->prof_begin(id, &ev)) return WD_E_HIP; \
+    hipLaunchKernelGGL(__VA_ARGS__);              \
+    WD_HIP(hipGetLastError(), what);              \
+    if (ctx->prof_end(id, ev)) return WD_E_HIP;   \
+  } while (0)
+  WD_LAUNCH(kKParticleKeys, "particles: key launch", k_particle_keys, dim3(tiles), dim3(256), 0, ctx->stream,
+            (const uint4 *)(pool + 32 * first), n, v, edges, st->keys, st->mask, st->tilehist);
+  WD_LAUNCH(kKScan, "particles: scan launch", k_scan, dim3(512 / kScanCols), dim3(256), 0, ctx->stream, tiles,
+            st->tilehist, st->prefix_lo, st->ctl);
+  WD_LAUNCH(kKScatterLo, "particles: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0,
+            ctx->stream, st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->ctl,
+            st->packed, 0u);
+  WD_LAUNCH(kKHist, "particles: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, st->packed,
+            st->ctl, st->tilehist3);
+  WD_LAUNCH(kKScan2, "particles: second scan launch", k_scan2, dim3(256 / kScanCols), dim3(256), 0, ctx->stream,
+            st->tilehist3, st->prefix_hi, st->ctl);
+  WD_LAUNCH(kKScatterHi, "particles: high-byte pass launch", k_scatter<true>, dim3(supers), dim3(256), 0,
+            ctx->stream, (const uint16_t *)nullptr, (const uint64_t *)nullptr, st->packed, n, st->tilehist3, 256u,
+            st->prefix_hi, st->ctl, st->order, (uint32_t)first);
 #undef WD_LAUNCH
   return WD_OK;
 }

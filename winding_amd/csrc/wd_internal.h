@@ -166,14 +166,24 @@ struct wd_context {
   std::vector<hipEvent_t> spare;
   double prof_ms[16] = {};
   uint64_t prof_launches[16] = {};
+  double *particle_edges = nullptr;  // device: the particle sort's 4096 bucket edges, made at first use
   int prof_begin(int kernel, hipEvent_t *a);
   int prof_end(int kernel, hipEvent_t a);
   int prof_collect();
 };
 
+struct wd_pool {
+  wd_context *ctx = nullptr;
+  uint8_t *particles = nullptr;  // device, n * 32 bytes
+  uint64_t n = 0;
+  uint32_t magic = 0x504f4f4c;  // cleared by destroy
+};
+
 struct wd_cloud {
   wd_context *ctx = nullptr;
-  uint8_t *records = nullptr;  // device, n * 32 bytes
+  uint8_t *records = nullptr;  // device, n * 32 bytes (null for a cloud given as centres and Sigma)
+  float *centers = nullptr;    // device, n * 4 floats: x, y, z, rgba8 bits (null for .splat records)
+  float *covs = nullptr;       // device, n * 6 floats: Sigma's xx xy xz yy yz zz (null for .splat records)
   uint64_t n = 0;
   float min[3] = {0, 0, 0}, max[3] = {0, 0, 0};
   uint32_t *sh = nullptr;       // device: spherical harmonics, sh_words(degree) words a splat (+1)
@@ -221,6 +231,7 @@ struct wd_state {
   uint32_t *packed = nullptr;  // low-digit pass output: high byte << 24 | index
   uint32_t *order = nullptr;   // the draw order
   float *cov = nullptr;        // n x 4 floats
+  float *ellipse = nullptr;    // n x 8 floats when the state takes the draw's ellipses, else null
   uint16_t *tilehist = nullptr;   // per 4096-splat chunk: 256 low-byte then 256 high-byte counts
   uint16_t *tilehist3 = nullptr;  // per tile of the low-byte pass's output: 256 high-byte counts
   uint32_t *prefix_lo = nullptr, *prefix_hi = nullptr;  // [tiles][256]: where a tile's digit run starts
@@ -234,7 +245,7 @@ namespace wd {
 
 enum KernelId {
   kKBounds = 0, kKProject, kKScan, kKScatterLo, kKHist, kKScan2, kKScatterHi, kKBatch, kKMlp, kKBinTable, kKBinScatter,
-  kKBinSort, kKernelCount
+  kKBinSort, kKParticleKeys, kKernelCount
 };
 extern const char *const kKernelNames[kKernelCount];
 
@@ -242,6 +253,55 @@ extern const char *const kKernelNames[kKernelCount];
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8);
 int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_state *st);
 int launch_batch(wd_context *ctx, wd_batch *b);
+// ---- capture decoders (decode.hip, decode_host.cpp) ----
+// Where a decoder writes: the cloud's centres (x, y, z, rgba bits), Sigma (six
+// floats), harmonics (sh_words words a splat, zeroed before), the least index
+// of a non-finite position (UINT32_MAX when none) and the box as ordered bits
+// (min xyz, then max xyz).
+struct DecodeOut {
+  float *centers;
+  float *covs;
+  uint32_t *sh;
+  uint32_t sh_words;
+  uint32_t *bad;
+  uint32_t *box;
+};
+// .ply property types, as PLY_TYPES names them (src/scene/splats.js:392-397).
+enum PlyType : uint8_t { kPlyI8, kPlyU8, kPlyI16, kPlyU16, kPlyI32, kPlyU32, kPlyF32, kPlyF64 };
+// x y z f_dc_0..2 opacity scale_0..2 rot_0..3 (PLY_NEEDS), then f_rest_0.. (up to 45).
+constexpr int kPlyNeeds = 14;
+struct PlyLayout {
+  uint32_t stride;
+  uint32_t rest;
+  uint32_t off[kPlyNeeds + 45];
+  uint8_t type[kPlyNeeds + 45];
+};
+// A gunzipped .spz's sections (byte offsets) and header (src/scene/splats.js:230-256).
+struct SpzLayout {
+  uint32_t version, rotation_bytes, coefficients, file_coefficients;
+  uint64_t positions, alphas, colours, scales, rotations, sh;
+  double fixed;  // 1 / (1 << fraction), with JavaScript's 32-bit shift
+};
+// A .sog's decoded images (device copies, rgba8, a splat a pixel) and tables
+// made on the host from its codebooks: the colour byte, exp of the scale and
+// the harmonics' half bits, per codebook index (src/scene/splats.js:136-203).
+struct SogLayout {
+  uint32_t count, per, sh_count, centroids_width;
+  uint64_t centroids_bytes;
+  double mins[3], maxs[3];
+  const uint8_t *means_l, *means_u, *quats, *scales, *sh0, *labels, *centroids;
+  const uint8_t *colour_byte;
+  const double *scale_exp;
+  const uint16_t *sh_half;
+};
+int launch_decode_ply(wd_context *ctx, const uint8_t *body, uint32_t n, const PlyLayout &L, const DecodeOut &o);
+int launch_decode_spz(wd_context *ctx, const uint8_t *data, uint32_t n, const SpzLayout &L, const DecodeOut &o);
+int launch_decode_sog(wd_context *ctx, const SogLayout &L, const DecodeOut &o);
+
+// The particle sort over slots [first, first + n) of a pool (view6: eye, forward in FP64;
+// edges: the 4096 FP64 bucket edges on the device).
+int launch_particle_sort(wd_context *ctx, const uint8_t *pool, uint64_t first, uint32_t n, const double *view6,
+                         const double *edges, wd_state *st);
 // The persistent batch kernel over a job table; skipped unless *gate != 0 when gate is given.
 int launch_persistent(wd_context *ctx, const BatchJob *jobs, const uint32_t *starts, uint32_t nj, uint32_t max_jobs,
                       BatchCtl *bctl, uint32_t grid, const uint32_t *gate);

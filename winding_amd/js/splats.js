@@ -63,14 +63,20 @@ class Gpu {
 const isZip = (d) => d[0] === 0x50 && d[1] === 0x4b && d[2] === 0x03 && d[3] === 0x04;
 
 /**
- * A capture's splats from the bytes of a .splat file: { count, degree, records, min, max }.
- * `records` are the bytes themselves, 32 a splat; min and max are filled in when the capture
- * is uploaded (new Splats), where the GPU checks every position.
+ * A capture's splats from the bytes of a .splat or .ply file (parseSplats, src/scene/splats.js:58-80).
+ * A .splat gives { count, degree: 0, records, min: null, max: null }: the bytes themselves, 32 a
+ * splat. A .ply gives { format: 'ply', bytes }: it is decoded on the GPU by new Splats. Either way
+ * min and max are filled in when the capture is uploaded, where the GPU checks every position.
  */
 function parseSplats(bytes) {
   const data = bytes instanceof Uint8Array ? bytes : new Uint8Array(bytes);
+  if (data.length >= 8 && data[0] === 0x4e && data[1] === 0x47 && data[2] === 0x53 && data[3] === 0x50
+    && new DataView(data.buffer, data.byteOffset, 8).getUint32(4, true) >= 4) {
+    throw new Error('splats: a .spz of version 4, compressed with zstd, which browsers cannot unpack; '
+      + 'save it as version 3, or as a .ply or .splat');
+  }
   const isPly = data.length >= 4 && data[0] === 0x70 && data[1] === 0x6c && data[2] === 0x79 && (data[3] === 0x0a || data[3] === 0x0d);
-  if (isPly) throw new Error('splats: a .ply capture is not read here; this path takes .splat records');
+  if (isPly) return { format: 'ply', bytes: data, count: null, degree: null, min: null, max: null };
   if (isZip(data)) throw new Error('splats: this is a zip: unpack it with readSplats, which reads a .sog');
   if (data[0] === 0x1f && data[1] === 0x8b) throw new Error('splats: this is gzipped: unpack it with readSplats, which reads a .spz');
   if (data.length === 0 || data.length % SPLAT_BYTES !== 0) {
@@ -79,23 +85,118 @@ function parseSplats(bytes) {
   return { count: data.length / SPLAT_BYTES, degree: 0, records: data, min: null, max: null };
 }
 
+/** A zip's files, by name, each a function that gives its bytes: stored and deflated entries, not zip64. */
+function unzip(data) {
+  const zlib = require('zlib');
+  const view = new DataView(data.buffer, data.byteOffset, data.byteLength);
+  let end = -1;
+  for (let i = data.length - 22; i >= Math.max(0, data.length - 22 - 65535); i--) {
+    if (view.getUint32(i, true) === 0x06054b50) { end = i; break; }
+  }
+  if (end < 0) throw new Error('splats: a zip with no end record: cut short?');
+  const files = new Map();
+  let at = view.getUint32(end + 16, true);
+  for (let n = view.getUint16(end + 10, true); n > 0; n--) {
+    if (at + 46 > data.length || view.getUint32(at, true) !== 0x02014b50) throw new Error('splats: a zip whose directory is damaged');
+    const method = view.getUint16(at + 10, true);
+    const size = view.getUint32(at + 20, true);
+    const nameLength = view.getUint16(at + 28, true);
+    const local = view.getUint32(at + 42, true);
+    const name = Buffer.from(data.subarray(at + 46, at + 46 + nameLength)).toString('utf8');
+    if (size === 0xffffffff || local === 0xffffffff) throw new Error('splats: a zip64 .sog is not read');
+    at += 46 + nameLength + view.getUint16(at + 30, true) + view.getUint16(at + 32, true);
+    const start = local + 30 + view.getUint16(local + 26, true) + view.getUint16(local + 28, true);
+    if (start + size > data.length) throw new Error('splats: the zip is cut short in ' + name);
+    const stored = data.subarray(start, start + size);
+    if (method !== 0 && method !== 8) throw new Error('splats: ' + name + ' is compressed with method ' + method + '; stored and deflate are read');
+    files.set(name, () => (method === 0 ? stored : new Uint8Array(zlib.inflateRawSync(stored))));
+  }
+  return files;
+}
+
+/**
+ * As parseSplats, for every format the reference's readSplats takes (src/scene/splats.js:84-92): a
+ * .spz is handed to the GPU as it is (the library gunzips it); a .sog is unzipped here, its
+ * meta.json read and its images decoded by decodeImage(bytes) -> { width, height, rgba }, and the
+ * splats decoded on the GPU by new Splats.
+ */
+async function readSplats(bytes, options) {
+  const data = bytes instanceof Uint8Array ? bytes : new Uint8Array(bytes);
+  if (isZip(data)) return readSog(data, options && options.decodeImage);
+  if (data[0] === 0x1f && data[1] === 0x8b) return { format: 'spz', bytes: data, count: null, degree: null, min: null, max: null };
+  return parseSplats(data);
+}
+
+/** readSog's host part (src/scene/splats.js:128-160): meta.json and the decoded images. */
+async function readSog(data, decodeImage) {
+  if (typeof decodeImage !== 'function') throw new Error('splats: a .sog needs decodeImage, for its WebP images');
+  const files = unzip(data);
+  const file = (name) => {
+    const get = files.get(name);
+    if (get === undefined) throw new Error('splats: the .sog has no ' + name);
+    return get();
+  };
+  const meta = JSON.parse(Buffer.from(file('meta.json')).toString('utf8'));
+  if (meta.version !== 2) {
+    throw new Error('splats: a .sog of version ' + (meta.version === undefined ? 1 : meta.version) + '; version 2 is read (re-save it with splat-transform)');
+  }
+  const count = meta.count;
+  if (!(Number.isInteger(count) && count > 0)) throw new Error('splats: a .sog of ' + count + ' splats');
+  const image = async (name, checkCount) => {
+    const decoded = await decodeImage(file(name));
+    if (checkCount !== false && decoded.width * decoded.height < count) {
+      throw new Error('splats: ' + name + ' has ' + (decoded.width * decoded.height) + ' pixels, for ' + count + ' splats');
+    }
+    return { width: decoded.width, height: decoded.height, rgba: Uint8Array.from(decoded.rgba) };
+  };
+  const [meansL, meansU, quats, scales, sh0] = await Promise.all([
+    image(meta.means.files[0]), image(meta.means.files[1]), image(meta.quats.files[0]),
+    image(meta.scales.files[0]), image(meta.sh0.files[0]),
+  ]);
+  const bands = meta.shN ? meta.shN.bands : 0;
+  const degree = SH_COEFFICIENTS[bands] > 0 ? bands : 0;
+  const sog = {
+    count, bands: degree, mins: Float64Array.from(meta.means.mins), maxs: Float64Array.from(meta.means.maxs),
+    scaleCodebook: Float64Array.from(meta.scales.codebook), colourCodebook: Float64Array.from(meta.sh0.codebook),
+    images: { means_l: meansL, means_u: meansU, quats, scales, sh0 },
+  };
+  if (degree > 0) {
+    sog.shCount = meta.shN.count === undefined ? 0xffffffff : meta.shN.count;
+    sog.shCodebook = Float64Array.from(meta.shN.codebook);
+    [sog.images.centroids, sog.images.labels] = await Promise.all([image(meta.shN.files[0], false), image(meta.shN.files[1])]);
+  }
+  return { format: 'sog', sog, count: null, degree: null, min: null, max: null };
+}
+
 /**
  * A capture on the GPU. `count` splats; `min` and `max`, the corners of the box around
- * their centres. Refused by name when a position is not a number or the capture is larger
- * than one sort takes.
+ * their centres. Takes what parseSplats or readSplats give (a .ply, .spz or .sog is decoded on
+ * the GPU), or the reference's parsed data { count, degree, centers, covariances, sh }. Refused
+ * by name when a position is not a number or the capture is larger than one sort takes.
  */
 class Splats {
   constructor(gpu, data, label) {
-    const degree = data.degree || 0;
-    if (!(degree >= 0 && degree <= 3)) throw new Error('splats: harmonics of degree ' + degree + '; this path takes 0 to 3');
-    if (degree > 0 && !(data.sh instanceof Uint32Array)) throw new Error('splats: harmonics of degree ' + degree + ' need data.sh (Uint32Array)');
-    const made = gpu.native.cloudCreate(gpu.handle, data.records);
-    if (degree > 0) {
-      try {
-        gpu.native.cloudSH(gpu.handle, made.handle, data.sh, degree);
-      } catch (error) {
-        gpu.native.destroy(made.handle);
-        throw error;
+    const native = gpu.native;
+    let made;
+    let degree = data.degree || 0;
+    if (data.format === 'ply') made = native.decodePly(gpu.handle, data.bytes);
+    else if (data.format === 'spz') made = native.decodeSpz(gpu.handle, data.bytes);
+    else if (data.format === 'sog') made = native.decodeSog(gpu.handle, data.sog);
+    if (made !== undefined) {
+      degree = made.degree;
+    } else {
+      if (!(degree >= 0 && degree <= 3)) throw new Error('splats: harmonics of degree ' + degree + '; this path takes 0 to 3');
+      if (degree > 0 && !(data.sh instanceof Uint32Array)) throw new Error('splats: harmonics of degree ' + degree + ' need data.sh (Uint32Array)');
+      made = data.centers instanceof Float32Array
+        ? native.cloudCreateSplats(gpu.handle, data.centers, data.covariances)
+        : native.cloudCreate(gpu.handle, data.records);
+      if (degree > 0) {
+        try {
+          native.cloudSH(gpu.handle, made.handle, data.sh, degree);
+        } catch (error) {
+          native.destroy(made.handle);
+          throw error;
+        }
       }
     }
     this.gpu = gpu;
@@ -197,6 +298,11 @@ class SplatPass {
     this.sorted = 0;
     /** The least a splat may add to the picture, opacity x pixels, and be drawn. */
     this.cull = SPLAT_CULL;
+    /**
+     * Whether each sort also makes the draw's ellipse set-up beside cov2d (what the splat vertex
+     * shader works out per instance, src/render/splats.js:263-291); read(entity, { ellipses: true }).
+     */
+    this.ellipses = false;
     /** The projection without TAA's jitter, when jittered: what the sort is decided by. */
     this.unjittered = null;
     this._key = new Float32Array(CLOUD_FLOATS);
@@ -225,8 +331,11 @@ class SplatPass {
       const key = this._key;
       key.set(f.subarray(0, CLOUD_FLOATS));
       if (this.unjittered !== null) key.set(this.unjittered, 16);
-      const sort = !sameFloats(state.last, key, CLOUD_FLOATS) || state.cull !== cull;
+      const ellipses = this.ellipses === true;
+      if (state.ellipses !== ellipses) this.native.stateEllipses(this.gpu.handle, state.handle, ellipses);
+      const sort = !sameFloats(state.last, key, CLOUD_FLOATS) || state.cull !== cull || state.ellipses !== ellipses;
       state.cull = cull;
+      state.ellipses = ellipses;
       if (sort) {
         state.next.set(key);
         this.sorted++;
@@ -255,10 +364,12 @@ class SplatPass {
   sort() {
     const marked = this._items.filter((i) => i.sort);
     for (const item of marked) item.state.last.set(item.state.next);
-    // Clouds with harmonics are shaded by their own sort; the rest share one batch launch.
-    const plain = marked.filter((i) => !(i.state.splats.degree > 0));
+    // Clouds with harmonics, or whose state makes the draw's ellipses, have a sort of their own;
+    // the rest share one batch launch.
+    const own = (i) => i.state.splats.degree > 0 || i.state.ellipses;
+    const plain = marked.filter((i) => !own(i));
     for (const item of marked) {
-      if (item.state.splats.degree > 0) {
+      if (own(item)) {
         const state = item.state;
         this.native.sort(this.gpu.handle, state.splats.handle, state.uniform, state.handle);
       }
@@ -287,13 +398,17 @@ class SplatPass {
 
   /**
    * Waits for entity's last sort and reads it back: { order (Uint32Array, back to front),
-   * count, recomputed, cov2d (Float32Array, 4 a splat by index: xx, xy, yy, rgba bits) }.
+   * count, recomputed, cov2d (Float32Array, 4 a splat by index: xx, xy, yy, rgba bits),
+   * ellipses (Float32Array, 8 a splat by index: clip x y z w, then along and across, scaled by
+   * the reach, in pixels; zeros where nothing is drawn) }.
    */
   read(entity, options) {
     const item = this._items.find((i) => i.entity === entity);
     if (item === undefined || !item.state.sortedOnce) throw new Error('splats: entity ' + entity + ' has not been sorted');
     const wantCov = options && options.cov2d;
-    return this.native.stateRead(this.gpu.handle, item.state.handle, wantCov ? item.count : 0);
+    const out = this.native.stateRead(this.gpu.handle, item.state.handle, wantCov ? item.count : 0);
+    if (options && options.ellipses) out.ellipses = this.native.stateReadEllipses(this.gpu.handle, item.state.handle, item.count);
+    return out;
   }
 
   _stateFor(record, splats, scene, entity) {
@@ -304,7 +419,7 @@ class SplatPass {
     last.fill(NaN);   // NaN matches nothing, so the first frame sorts
     state = {
       splats, scene, entity, lastFrame: 0, last, next: new Float32Array(CLOUD_FLOATS),
-      uniform: new Float32Array(CLOUD_FLOATS), cull: undefined, sortedOnce: false,
+      uniform: new Float32Array(CLOUD_FLOATS), cull: undefined, sortedOnce: false, ellipses: false,
       handle: this.native.stateCreate(this.gpu.handle, Math.max(1, splats.count)),
     };
     this._states.set(record, state);
@@ -327,5 +442,5 @@ class SplatPass {
 
 module.exports = {
   SPLAT_BYTES, BUCKETS, SPLAT_CULL, CLOUD_FLOATS, KEEP_FRAMES, SH_COEFFICIENTS, shWords,
-  native, Gpu, parseSplats, Splats, SplatScene, SplatPass, cloudUniform,
+  native, Gpu, parseSplats, readSplats, Splats, SplatScene, SplatPass, cloudUniform,
 };

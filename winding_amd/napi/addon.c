@@ -17,7 +17,7 @@
 
 #include "winding_amd.h"
 
-enum { K_CTX = 1, K_CLOUD = 2, K_STATE = 3, K_BATCH = 4, K_MLP = 5, K_LEARNED = 6 };
+enum { K_CTX = 1, K_CLOUD = 2, K_STATE = 3, K_BATCH = 4, K_MLP = 5, K_LEARNED = 6, K_POOL = 7 };
 
 typedef struct Box {
   int kind;
@@ -49,6 +49,7 @@ static void release_box(napi_env env, Box *b) {
     if (b->kind == K_BATCH) wd_batch_destroy((wd_batch *)b->ptr);
     if (b->kind == K_MLP) wd_mlp_destroy((wd_mlp *)b->ptr);
     if (b->kind == K_LEARNED) wd_learned_destroy((wd_learned *)b->ptr);
+    if (b->kind == K_POOL) wd_pool_destroy((wd_pool *)b->ptr);
     if (b->kind == K_CTX) wd_close((wd_context *)b->ptr);
     b->ptr = NULL;
     if (b->ctx) b->ctx->children--;
@@ -213,6 +214,142 @@ static napi_value js_cloud_sh(napi_env env, napi_callback_info info) {
   return rc ? throw_wd(env, rc) : NULL;
 }
 
+/* A decoded or given cloud as {handle, count, degree, min, max}. */
+static napi_value cloud_result(napi_env env, napi_value ctx_value, Box *c, wd_cloud *cloud, const float *mn,
+                               const float *mx) {
+  napi_value out;
+  napi_value h = make_handle(env, K_CLOUD, cloud, ctx_value, c);
+  if (!h) return NULL;
+  uint64_t count = 0;
+  uint32_t degree = 0;
+  wd_cloud_info(cloud, &count, NULL, NULL);
+  wd_cloud_read((wd_context *)c->ptr, cloud, NULL, NULL, NULL, &degree);
+  CHECK_NAPI(napi_create_object(env, &out));
+  napi_set_named_property(env, out, "handle", h);
+  napi_set_named_property(env, out, "count", num(env, (double)count));
+  napi_set_named_property(env, out, "degree", num(env, degree));
+  napi_set_named_property(env, out, "min", f32_array(env, mn, 3));
+  napi_set_named_property(env, out, "max", f32_array(env, mx, 3));
+  return out;
+}
+
+/* cloudCreateSplats(context, Float32Array centers (4 n), Float32Array covariances (6 n)) -> cloud */
+static napi_value js_cloud_create_splats(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  void *centers, *covs;
+  size_t lc, lv;
+  if (!typed(env, argv[1], napi_float32_array, &centers, &lc)) return NULL;
+  if (!typed(env, argv[2], napi_float32_array, &covs, &lv)) return NULL;
+  if (lc % 4 != 0 || lv != lc / 4 * 6) {
+    napi_throw_range_error(env, NULL, "splats: 4 centre floats and 6 covariance floats a splat");
+    return NULL;
+  }
+  wd_cloud *cloud = NULL;
+  float mn[3], mx[3];
+  int rc = wd_cloud_create_splats((wd_context *)c->ptr, (const float *)centers, (const float *)covs, lc / 4, &cloud,
+                                  mn, mx);
+  if (rc) return throw_wd(env, rc);
+  return cloud_result(env, argv[0], c, cloud, mn, mx);
+}
+
+/* decodePly / decodeSpz(context, Uint8Array bytes) -> cloud, decoded on the GPU */
+static napi_value decode_bytes(napi_env env, napi_callback_info info, int spz) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  void *data;
+  size_t len;
+  if (!typed(env, argv[1], napi_uint8_array, &data, &len)) return NULL;
+  wd_cloud *cloud = NULL;
+  float mn[3], mx[3];
+  int rc = spz ? wd_decode_spz((wd_context *)c->ptr, data, len, &cloud, mn, mx)
+               : wd_decode_ply((wd_context *)c->ptr, data, len, &cloud, mn, mx);
+  if (rc) return throw_wd(env, rc);
+  return cloud_result(env, argv[0], c, cloud, mn, mx);
+}
+static napi_value js_decode_ply(napi_env env, napi_callback_info info) { return decode_bytes(env, info, 0); }
+static napi_value js_decode_spz(napi_env env, napi_callback_info info) { return decode_bytes(env, info, 1); }
+
+static int get_named(napi_env env, napi_value obj, const char *name, napi_value *out) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return 0;
+  return napi_get_named_property(env, obj, name, out) == napi_ok;
+}
+
+static int sog_image(napi_env env, napi_value images, const char *name, wd_sog_image *img) {
+  napi_value o, v;
+  size_t len;
+  void *data;
+  uint32_t w = 0, h = 0;
+  if (!get_named(env, images, name, &o)) return 1;  /* absent: left null, the library says which */
+  if (!get_named(env, o, "rgba", &v) || !typed(env, v, napi_uint8_array, &data, &len)) return 0;
+  if (get_named(env, o, "width", &v)) napi_get_value_uint32(env, v, &w);
+  if (get_named(env, o, "height", &v)) napi_get_value_uint32(env, v, &h);
+  if ((uint64_t)w * h * 4 > len) {
+    napi_throw_range_error(env, NULL, "splats: a .sog image holds fewer bytes than width x height x 4");
+    return 0;
+  }
+  img->rgba = (const uint8_t *)data;
+  img->width = w;
+  img->height = h;
+  return 1;
+}
+
+/*
+ * decodeSog(context, {count, bands, shCount, mins, maxs, scaleCodebook, colourCodebook, shCodebook
+ * (Float64Arrays), images: {means_l, means_u, quats, scales, sh0, centroids, labels: {width, height,
+ * rgba}}}) -> cloud: the numbers of meta.json and the decoded images (readSog's loop on the GPU).
+ */
+static napi_value js_decode_sog(napi_env env, napi_callback_info info) {
+  napi_value argv[2], v, images;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  wd_sog s;
+  memset(&s, 0, sizeof s);
+  napi_value o = argv[1];
+  if (get_named(env, o, "count", &v)) napi_get_value_uint32(env, v, &s.count);
+  if (get_named(env, o, "bands", &v)) napi_get_value_uint32(env, v, &s.bands);
+  s.sh_count = 0xFFFFFFFFu;
+  if (get_named(env, o, "shCount", &v)) napi_get_value_uint32(env, v, &s.sh_count);
+  const char *vec[2] = {"mins", "maxs"};
+  double *dst[2] = {s.mins, s.maxs};
+  for (int k = 0; k < 2; ++k) {
+    void *d;
+    size_t l;
+    if (!get_named(env, o, vec[k], &v) || !typed(env, v, napi_float64_array, &d, &l) || l < 3) return NULL;
+    memcpy(dst[k], d, 3 * sizeof(double));
+  }
+  const char *books[3] = {"scaleCodebook", "colourCodebook", "shCodebook"};
+  const double **bp[3] = {&s.scale_codebook, &s.colour_codebook, &s.sh_codebook};
+  uint32_t *bn[3] = {&s.scale_codebook_size, &s.colour_codebook_size, &s.sh_codebook_size};
+  for (int k = 0; k < 3; ++k) {
+    void *d;
+    size_t l;
+    if (!get_named(env, o, books[k], &v)) continue;
+    if (!typed(env, v, napi_float64_array, &d, &l)) return NULL;
+    *bp[k] = (const double *)d;
+    *bn[k] = (uint32_t)l;
+  }
+  if (!get_named(env, o, "images", &images)) {
+    napi_throw_type_error(env, NULL, "splats: decodeSog needs images");
+    return NULL;
+  }
+  const char *names[7] = {"means_l", "means_u", "quats", "scales", "sh0", "centroids", "labels"};
+  wd_sog_image *imgs[7] = {&s.means_l, &s.means_u, &s.quats, &s.scales, &s.sh0, &s.centroids, &s.labels};
+  for (int k = 0; k < 7; ++k)
+    if (!sog_image(env, images, names[k], imgs[k])) return NULL;
+  wd_cloud *cloud = NULL;
+  float mn[3], mx[3];
+  int rc = wd_decode_sog((wd_context *)c->ptr, &s, &cloud, mn, mx);
+  if (rc) return throw_wd(env, rc);
+  return cloud_result(env, argv[0], c, cloud, mn, mx);
+}
+
 static napi_value js_destroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   void *data = NULL;
@@ -327,6 +464,85 @@ static napi_value js_state_read(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, out, "cov2d", arr);
   }
   return out;
+}
+
+/* stateEllipses(ctx, state, on): whether each sort also writes the draw's ellipse set-up */
+static napi_value js_state_ellipses(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  bool on;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *s = get_box(env, argv[1], K_STATE);
+  if (!s) return NULL;
+  CHECK_NAPI(napi_get_value_bool(env, argv[2], &on));
+  int rc = wd_state_ellipses((wd_context *)c->ptr, (wd_state *)s->ptr, on ? 1 : 0);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
+/* stateReadEllipses(ctx, state, n) -> Float32Array(8 n): clip x y z w, then (along e, across e) */
+static napi_value js_state_read_ellipses(napi_env env, napi_callback_info info) {
+  napi_value argv[3], ab, arr;
+  double n;
+  void *data;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *s = get_box(env, argv[1], K_STATE);
+  if (!s) return NULL;
+  CHECK_NAPI(napi_get_value_double(env, argv[2], &n));
+  if (!(n >= 0)) n = 0;
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)n * 32 + 4, &data, &ab));
+  int rc = wd_state_read_ellipses((wd_context *)c->ptr, (wd_state *)s->ptr, (float *)data, (uint64_t)n);
+  if (rc) return throw_wd(env, rc);
+  CHECK_NAPI(napi_create_typedarray(env, napi_float32_array, (size_t)n * 8, ab, 0, &arr));
+  return arr;
+}
+
+/* poolCreate(context, Uint8Array particles) -> handle: 32-byte particles (position, age, velocity, lifetime) */
+static napi_value js_pool_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  void *data;
+  size_t len;
+  if (!typed(env, argv[1], napi_uint8_array, &data, &len)) return NULL;
+  if (len % WD_PARTICLE_BYTES != 0) {
+    napi_throw_error(env, NULL, "particles: not a whole number of 32-byte particles");
+    return NULL;
+  }
+  wd_pool *pool = NULL;
+  int rc = wd_pool_create((wd_context *)c->ptr, data, len / WD_PARTICLE_BYTES, &pool);
+  if (rc) return throw_wd(env, rc);
+  return make_handle(env, K_POOL, pool, argv[0], c);
+}
+
+/* particlesSort(context, pool, first, count, Float32Array(8) view: eye xyz_, forward xyz_, state) */
+static napi_value js_particles_sort(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  double first, count;
+  void *u;
+  size_t l;
+  if (!get_args(env, info, 6, argv)) return NULL;
+  Box *c = get_box(env, argv[0], K_CTX);
+  if (!c) return NULL;
+  Box *p = get_box(env, argv[1], K_POOL);
+  if (!p) return NULL;
+  CHECK_NAPI(napi_get_value_double(env, argv[2], &first));
+  CHECK_NAPI(napi_get_value_double(env, argv[3], &count));
+  if (!typed(env, argv[4], napi_float32_array, &u, &l) || l < 8) return NULL;
+  Box *s = get_box(env, argv[5], K_STATE);
+  if (!s) return NULL;
+  if (!(first >= 0) || !(count >= 0)) {
+    napi_throw_range_error(env, NULL, "particles: first and count must be 0 or more");
+    return NULL;
+  }
+  wd_particle_view v;
+  memcpy(&v, u, sizeof v);
+  int rc = wd_particles_sort((wd_context *)c->ptr, (wd_pool *)p->ptr, (uint64_t)first, (uint64_t)count, &v,
+                             (wd_state *)s->ptr);
+  return rc ? throw_wd(env, rc) : NULL;
 }
 
 /* synth(kind, seed, n) -> Uint8Array of n seeded .splat records (host only) */
@@ -682,8 +898,11 @@ static napi_value init(napi_env env, napi_value exports) {
   } fns[] = {
       {"open", js_open},           {"close", js_close},           {"sync", js_sync},
       {"cloudCreate", js_cloud_create}, {"destroy", js_destroy},  {"viewPrepare", js_view_prepare},
-      {"cloudSH", js_cloud_sh},
+      {"cloudSH", js_cloud_sh},         {"cloudCreateSplats", js_cloud_create_splats},
+      {"decodePly", js_decode_ply},     {"decodeSpz", js_decode_spz}, {"decodeSog", js_decode_sog},
       {"stateCreate", js_state_create}, {"sort", js_sort},        {"stateRead", js_state_read},
+      {"stateEllipses", js_state_ellipses}, {"stateReadEllipses", js_state_read_ellipses},
+      {"poolCreate", js_pool_create},   {"particlesSort", js_particles_sort},
       {"synth", js_synth},         {"profile", js_profile},       {"profileRead", js_profile_read},
       {"batchCreate", js_batch_create}, {"batchSet", js_batch_set}, {"batchRun", js_batch_run},
       {"batchTune", js_batch_tune},     {"mlpCreate", js_mlp_create}, {"mlpSave", js_mlp_save},

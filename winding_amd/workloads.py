@@ -145,3 +145,32 @@ def sh_words(n, degree, seed=7, sigma=0.2):
     vals = np.random.default_rng(seed).normal(0.0, sigma, (n, 3 * k)).astype(np.float16)
     halves[:, :3 * k] = vals.view(np.uint16)
     return halves.reshape(-1).view(np.uint32)
+
+
+PARTICLE_EYE = (0.0, 0.0, 100.0)
+PARTICLE_FORWARD = (0.0, 0.0, -1.0)
+
+
+def particle_pool(n, seed=11, dead=0.25, unused=0.05):
+    """A pool of n 32-byte particles (position, age, velocity, lifetime; src/render/particles.js:51-56) seen
+    from PARTICLE_EYE along PARTICLE_FORWARD: depths log-uniform over the buckets' whole range and past both
+    ends (2^-12 to 2^21, and some behind the eye), a fraction dead (age past lifetime) and a fraction never
+    used (zeroed, lifetime 0). numpy's seeded generator."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    depth = np.exp2(rng.uniform(-12.0, 21.0, n))
+    depth[rng.random(n) < 0.02] *= -1.0
+    pos = np.zeros((n, 3), np.float64)
+    pos[:, :2] = rng.normal(0.0, 1.0, (n, 2)) * (0.2 * depth[:, None])
+    pos[:, 2] = PARTICLE_EYE[2] - depth
+    rec = np.zeros((n, 8), np.float32)
+    rec[:, :3] = pos
+    rec[:, 4:7] = rng.normal(0.0, 1.0, (n, 3))
+    life = rng.uniform(1.0, 5.0, n)
+    age = rng.uniform(0.0, 1.0, n) * life
+    gone = rng.random(n) < dead
+    age[gone] = life[gone] + rng.uniform(0.0, 2.0, gone.sum())
+    rec[:, 3] = age
+    rec[:, 7] = life
+    rec[rng.random(n) < unused] = 0.0
+    return rec.view(np.uint8).reshape(n, 32)
