@@ -23,10 +23,10 @@ HBM_PEAK_GBS = 8000.0
 
 
 def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload` from the committed counter runs (profiles/*_traffic.md:
-    FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a copy of known size), or None."""
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter runs (the `traffic:` lines of
+    profiles/*.md: FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a copy of known size), or None."""
     import glob
-    for path in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md")):
+    for path in glob.glob(os.path.join(ROOT, "profiles", "*.md")):
         for line in open(path):
             f = line.split()
             if len(f) == 5 and f[0] == "traffic:" and f[1] == workload and f[2] == kernel:
@@ -109,7 +109,8 @@ def bench_particles(args):
         "data": "synthetic (seeded generator, winding_amd/workloads.py particle_pool)",
         "config": {"workload": "particles-8m", "particles": n, "live": v, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": alg_key / (key_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": alg_key / (key_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": alg_key / (key_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("particles-8m", "k_particle_keys"),
                      "kernel": "k_particle_keys", "kernel_ms": key_ms, "bytes_per_launch": alg_key},
         "path": {"bytes_per_step": 32.0 * n + 4.0 * v,
                  "frac_of_peak": (32.0 * n + 4.0 * v) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
@@ -118,6 +119,81 @@ def bench_particles(args):
     print(json.dumps(line), flush=True)
     st.destroy()
     p.destroy()
+    ctx.close()
+    return 0
+
+
+def synth_ply(n, seed=3):
+    """A binary .ply of n splats with degree-3 harmonics, as training writes it: 62 floats a vertex."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    names = ["x", "y", "z", "f_dc_0", "f_dc_1", "f_dc_2", "opacity", "scale_0", "scale_1", "scale_2",
+             "rot_0", "rot_1", "rot_2", "rot_3"] + ["f_rest_%d" % k for k in range(45)] + ["nx", "ny", "nz"]
+    v = np.empty((n, len(names)), np.float32)
+    v[:, :3] = rng.normal(0, 3, (n, 3))
+    v[:, 3:6] = rng.normal(0, 1.5, (n, 3))
+    v[:, 6] = rng.normal(0, 3, n)
+    v[:, 7:10] = rng.normal(-3, 1, (n, 3))
+    v[:, 10:14] = rng.normal(0, 1, (n, 4))
+    v[:, 14:59] = rng.normal(0, 0.3, (n, 45))
+    v[:, 59:] = 0
+    head = "ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % n
+    head += "".join("property float %s\n" % nm for nm in names) + "end_header\n"
+    return np.frombuffer(head.encode() + v.tobytes(), np.uint8)
+
+
+def synth_spz(n, seed=4):
+    """A gunzipped .spz of version 3 with degree-3 harmonics: 65 bytes a splat in its sections."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    head = np.frombuffer(np.array([0x5053474E, 3, n], "<u4").tobytes() + bytes([3, 12, 0, 0]), np.uint8)
+    body = rng.integers(0, 256, n * (9 + 1 + 3 + 3 + 4 + 45), dtype=np.uint8)
+    rot = body[n * 16:n * 20].view("<u4")
+    rot &= np.uint32(0x3FFFFFFF)
+    rot |= (rng.integers(0, 4, n).astype(np.uint32) << np.uint32(30))
+    return np.concatenate([head, body])
+
+
+def bench_decode(args):
+    """decode-1m: 1,048,576 splats with degree-3 harmonics as a .ply (248 bytes a splat) and as a
+    gunzipped .spz of version 3 (65 bytes a splat), each decoded into a cloud of centres, Sigma and
+    harmonics. The kernel is timed with HIP events; the whole call (header, upload, decode, box and
+    check read back, the cloud's allocations) with the host clock."""
+    import winding_amd as wa
+    n = 1 << 20
+    ctx = wa.Context(0)
+    out = {}
+    for kind, data, fn, in_bytes in (("ply", synth_ply(n), wa.decode_ply, 248.0),
+                                     ("spz", synth_spz(n), wa.decode_spz, 65.0)):
+        for _ in range(max(1, args.warmup // 4)):
+            fn(ctx, data).destroy()
+        ctx.profile(True)
+        reps = max(5, args.steps // 10)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(ctx, data).destroy()
+        call_ms = (time.perf_counter() - t0) * 1e3 / reps
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        k_ms = prof["k_decode"][0] / prof["k_decode"][1]
+        alg = (in_bytes + 16 + 24 + 92) * n  # the file's splat bytes read; centre, Sigma and harmonics written
+        out[kind] = {"kernel_ms": k_ms, "call_ms": call_ms, "splats_per_s_kernel": n / (k_ms * 1e-3),
+                     "splats_per_s_call": n / (call_ms * 1e-3), "bytes_per_launch": alg,
+                     "frac_of_peak": alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    ply = out["ply"]
+    line = {
+        "metric": "M splats decoded/s (.ply, degree 3)", "value": ply["splats_per_s_kernel"] / 1e6,
+        "unit": "M splats/s", "n_gpus": 1, "steps": reps, "warmup": max(1, args.warmup // 4),
+        "ms_per_step": ply["kernel_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded numpy .ply and .spz)",
+        "config": {"workload": "decode-1m", "splats": n, "sh_degree": 3, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": ply["bytes_per_launch"] / (ply["kernel_ms"] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ply["frac_of_peak"],
+                     "traffic": measured_traffic("decode-1m", "k_decode_ply"),
+                     "kernel": "k_decode_ply", "kernel_ms": ply["kernel_ms"], "bytes_per_launch": ply["bytes_per_launch"]},
+        "decode": out,
+    }
+    print(json.dumps(line), flush=True)
     ctx.close()
     return 0
 
@@ -293,7 +369,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="slab-8m",
-                    choices=["plain-64k", "plain-1m", "slab-8m", "batch-64", "learned-4m", "sh3-1m", "particles-8m"])
+                    choices=["plain-64k", "plain-1m", "slab-8m", "batch-64", "learned-4m", "sh3-1m", "particles-8m", "decode-1m"])
     ap.add_argument("--cull", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -321,6 +397,8 @@ def main():
         return bench_learned(args)
     if args.workload == "particles-8m":
         return bench_particles(args)
+    if args.workload == "decode-1m":
+        return bench_decode(args)
     # sh3-1m: the 1,048,576-splat cloud with degree-3 harmonics (seeded), shaded in the projection
     sh_degree = 3 if args.workload == "sh3-1m" else 0
     recs, view, spec = wa.workloads.plain("plain-1m" if sh_degree else args.workload, cull=args.cull)

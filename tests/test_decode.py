@@ -311,6 +311,11 @@ def random_ply(n, degree, seed, types=None):
         row["opacity"] = rng.normal(0, 3)
         row.update({r: rng.normal(0, 0.3) for r in rest})
         rows.append(row)
+    # harmonics at the edges of halfBits: ties (half up), both ends of the subnormals, saturation, NaN
+    edges = [1 + 2 ** -11, -(1 + 2 ** -11), 0.5 + 2 ** -12, 2 ** -14 * (1 - 2 ** -11), 3 * 2 ** -25, 2 ** -25,
+             65503.99, 65520.0, 1e9, -1e9, float("nan"), 0.0, -0.0, 2 ** -24]
+    for j, r in enumerate(rest):
+        rows[j % len(rows)][r] = edges[j % len(edges)]
     return ply(props, rows, types)
 
 

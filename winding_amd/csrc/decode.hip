@@ -13,6 +13,7 @@
 // an f32 rounding edge.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "wd_internal.h"
@@ -24,6 +25,10 @@ constexpr double kShC0 = 0.28209479177387814;
 // Turning RUB to RDF flips these harmonics' signs, in 3DGS order
 // (src/scene/splats.js:219): bits 0, 1, 3, 6, 8, 10, 11, 13.
 constexpr uint32_t kSpzShNegative = 0x2D4Bu;
+constexpr uint32_t kShCoeffs[4] = {0, 3, 8, 15};
+// The staged .ply form holds a workgroup's 256 vertices in LDS: up to 64 KB
+// (62 floats a vertex, the layout training writes, take 62 KB).
+constexpr uint32_t kPlyStagedStride = 256;
 
 __device__ __forceinline__ double js_round(double x) {
   const double r = floor(x);
@@ -57,32 +62,59 @@ __device__ __forceinline__ uint32_t half_bits(double v) {
 
 __device__ __forceinline__ uint32_t signed_half(double v) { return v < 0 ? 0x8000u | half_bits(-v) : half_bits(v); }
 
+// signed_half of a value that is an f32, from its bits: the same answer
+// (halfBits' tests and its rounding, half up, are exact on the f32's
+// mantissa) in integer operations instead of FP64 ones.
+__device__ __forceinline__ uint32_t signed_half_f32(float v) {
+  const uint32_t bits = __float_as_uint(v);
+  const uint32_t sign = (bits >> 16) & 0x8000u;
+  const uint32_t a = bits & 0x7FFFFFFFu;  // |v|
+  uint32_t h;
+  if (a > 0x7F800000u || a == 0u) {
+    h = 0u;                                // NaN, zero
+  } else if (a >= 0x477FE000u) {
+    h = 0x7bffu;                           // |v| >= 65504, infinity included
+  } else if (a < 0x33800000u) {
+    h = 0u;                                // |v| < 2^-24
+  } else if (a < 0x38800000u) {
+    // subnormal: round(|v| / 2^-24), half up; |v| 2^24 < 1024 is exact in f32
+    const float x = __uint_as_float(a) * 16777216.0f;
+    const float r = floorf(x);
+    h = (uint32_t)r + ((x - r >= 0.5f) ? 1u : 0u);
+  } else {
+    // normal: the top 10 mantissa bits, plus one when the 13 dropped bits are half or more
+    h = (((a >> 23) - 112u) << 10) + ((a >> 13) & 1023u) + ((a >> 12) & 1u);
+  }
+  return (v < 0.0f ? sign : 0u) | h;
+}
+
 // The float's bits in an order that compares as the float does.
 __device__ __forceinline__ uint32_t ordered(float x) {
   const uint32_t b = __float_as_uint(x);
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-// put() (src/scene/splats.js:335-370) for splat i: centre, colour word, box,
-// Sigma = (R S)(R S)^T from the quaternion made unit length; a non-finite
-// position is recorded in bad (the least such index wins) and left out of the box.
-__device__ __forceinline__ void put(DecodeOut o, uint32_t i, double x, double y, double z, uint32_t r, uint32_t g,
-                                    uint32_t b, uint32_t a, double sx, double sy, double sz, double qw, double qx,
-                                    double qy, double qz) {
-  const bool finite = isfinite(x) && isfinite(y) && isfinite(z);
-  if (!finite) atomicMin(o.bad, i);
-  const float fx = (float)x, fy = (float)y, fz = (float)z;
-  reinterpret_cast<float4 *>(o.centers)[i] = make_float4(fx, fy, fz, __uint_as_float(r | (g << 8) | (b << 16) | (a << 24)));
-  if (finite) {
-    const float p[3] = {fx, fy, fz};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      atomicMin(&o.box[k], ordered(p[k]));
-      atomicMax(&o.box[3 + k], ordered(p[k]));
-    }
-  }
+// One decoded splat in registers: its centre word and Sigma, as put() makes
+// them (src/scene/splats.js:335-370), and its harmonics as packed halves.
+template <int DEG>
+struct Decoded {
+  static constexpr uint32_t W = DEG ? sh_words(DEG) : 1;
+  float4 centre;
+  float cov[6];
+  uint32_t sh[W];
+  bool finite;
+};
+
+// put(): centre, colour word, Sigma = (R S)(R S)^T from the quaternion made
+// unit length (`Math.hypot(...) || 1`).
+template <int DEG>
+__device__ __forceinline__ void put(Decoded<DEG> &d, double x, double y, double z, uint32_t r, uint32_t g, uint32_t b,
+                                    uint32_t a, double sx, double sy, double sz, double qw, double qx, double qy,
+                                    double qz) {
+  d.finite = isfinite(x) && isfinite(y) && isfinite(z);
+  d.centre = make_float4((float)x, (float)y, (float)z, __uint_as_float(r | (g << 8) | (b << 16) | (a << 24)));
   double length = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
-  if (length == 0.0 || length != length) length = 1.0;  // `|| 1`: a zero or NaN length is 1
+  if (length == 0.0 || length != length) length = 1.0;
   const double w = qw / length, u = qx / length, v = qy / length, t = qz / length;
   const double r00 = 1 - 2 * (v * v + t * t), r01 = 2 * (u * v - w * t), r02 = 2 * (u * t + w * v);
   const double r10 = 2 * (u * v + w * t), r11 = 1 - 2 * (u * u + t * t), r12 = 2 * (v * t - w * u);
@@ -90,22 +122,100 @@ __device__ __forceinline__ void put(DecodeOut o, uint32_t i, double x, double y,
   const double m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
   const double m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
   const double m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
-  float *c = o.covs + 6 * (size_t)i;
-  c[0] = (float)(m00 * m00 + m01 * m01 + m02 * m02);
-  c[1] = (float)(m00 * m10 + m01 * m11 + m02 * m12);
-  c[2] = (float)(m00 * m20 + m01 * m21 + m02 * m22);
-  c[3] = (float)(m10 * m10 + m11 * m11 + m12 * m12);
-  c[4] = (float)(m10 * m20 + m11 * m21 + m12 * m22);
-  c[5] = (float)(m20 * m20 + m21 * m21 + m22 * m22);
+  d.cov[0] = (float)(m00 * m00 + m01 * m01 + m02 * m02);
+  d.cov[1] = (float)(m00 * m10 + m01 * m11 + m02 * m12);
+  d.cov[2] = (float)(m00 * m20 + m01 * m21 + m02 * m22);
+  d.cov[3] = (float)(m10 * m10 + m11 * m11 + m12 * m12);
+  d.cov[4] = (float)(m10 * m20 + m11 * m21 + m12 * m22);
+  d.cov[5] = (float)(m20 * m20 + m21 * m21 + m22 * m22);
 }
 
-// putSH (src/scene/splats.js:327-332): coefficient k of splat i, red, green, blue.
-__device__ __forceinline__ void put_sh(DecodeOut o, uint32_t i, uint32_t k, double r, double g, double b) {
-  uint16_t *h = reinterpret_cast<uint16_t *>(o.sh) + (size_t)i * o.sh_words * 2 + k * 3;
-  h[0] = (uint16_t)signed_half(r);
-  h[1] = (uint16_t)signed_half(g);
-  h[2] = (uint16_t)signed_half(b);
+// Half h of a splat's harmonics (coefficient k, channel c at h = 3 k + c: the
+// `sh` layout, two halves a word, the lower first). h is a constant after unrolling.
+template <int DEG>
+__device__ __forceinline__ void set_half(Decoded<DEG> &d, uint32_t h, uint32_t bits) {
+  uint32_t &word = d.sh[h >> 1];
+  word = (h & 1u) ? ((word & 0xFFFFu) | (bits << 16)) : ((word & 0xFFFF0000u) | (bits & 0xFFFFu));
 }
+
+// putSH (src/scene/splats.js:327-332): coefficient k, red, green, blue.
+template <int DEG>
+__device__ __forceinline__ void put_sh(Decoded<DEG> &d, uint32_t k, double r, double g, double b) {
+  set_half(d, 3 * k, signed_half(r));
+  set_half(d, 3 * k + 1, signed_half(g));
+  set_half(d, 3 * k + 2, signed_half(b));
+}
+// The same for values that are f32s (a .ply's float properties, a .spz's byte fractions).
+template <int DEG>
+__device__ __forceinline__ void put_sh_f32(Decoded<DEG> &d, uint32_t k, float r, float g, float b) {
+  set_half(d, 3 * k, signed_half_f32(r));
+  set_half(d, 3 * k + 1, signed_half_f32(g));
+  set_half(d, 3 * k + 2, signed_half_f32(b));
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, s));
+  return x;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, s));
+  return x;
+}
+
+// The workgroup's decoded splats out: the box of the finite centres reduced in
+// the wave and across the workgroup (six atomics a workgroup), the least
+// index of a non-finite position, and centres, Sigma and harmonics staged in
+// LDS so that each goes out as one contiguous run of whole lines. lds holds
+// 256 (10 + W) words; the caller has finished with it.
+template <int DEG>
+__device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t cnt, bool valid, const Decoded<DEG> &d,
+                                     uint32_t *lds) {
+  constexpr uint32_t W = DEG ? sh_words(DEG) : 0;
+  __shared__ uint32_t box[6];
+  const uint32_t t = threadIdx.x;
+  if (t < 6) box[t] = t < 3 ? 0xFFFFFFFFu : 0u;
+  if (valid && !d.finite) atomicMin(o.bad, i0 + t);
+  const bool in_box = valid && d.finite;
+  const float p[3] = {d.centre.x, d.centre.y, d.centre.z};
+  uint32_t lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = wave_min(in_box ? ordered(p[k]) : 0xFFFFFFFFu);
+    hi[k] = wave_max(in_box ? ordered(p[k]) : 0u);
+  }
+  uint32_t *centres = lds, *covs = lds + 256 * 4, *sh = lds + 256 * 10;
+  if (valid) {
+    reinterpret_cast<float4 *>(centres)[t] = d.centre;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) covs[6 * t + k] = __float_as_uint(d.cov[k]);
+    if constexpr (DEG > 0)
+#pragma unroll
+      for (uint32_t k = 0; k < W; ++k) sh[W * t + k] = d.sh[k];
+  }
+  __syncthreads();
+  if ((t & 63u) == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&box[k], lo[k]);
+      atomicMax(&box[3 + k], hi[k]);
+    }
+  }
+  uint32_t *gc = reinterpret_cast<uint32_t *>(o.centers) + (size_t)i0 * 4;
+  for (uint32_t k = t; k < cnt * 4; k += 256) gc[k] = centres[k];
+  uint32_t *gv = reinterpret_cast<uint32_t *>(o.covs) + (size_t)i0 * 6;
+  for (uint32_t k = t; k < cnt * 6; k += 256) gv[k] = covs[k];
+  if constexpr (DEG > 0) {
+    uint32_t *gs = o.sh + (size_t)i0 * W;
+    for (uint32_t k = t; k < cnt * W; k += 256) gs[k] = sh[k];
+  }
+  __syncthreads();
+  if (t < 3 && box[t] != 0xFFFFFFFFu) atomicMin(&o.box[t], box[t]);
+  if (t >= 3 && t < 6 && box[t] != 0u) atomicMax(&o.box[t], box[t]);
+}
+
+constexpr uint32_t emit_words(int DEG) { return 256u * (10u + (DEG ? sh_words(DEG) : 0u)); }
 
 // One .ply property of a vertex, as a JavaScript number.
 __device__ __forceinline__ double ply_get(const uint8_t *base, uint32_t off, uint32_t type) {
@@ -122,144 +232,222 @@ __device__ __forceinline__ double ply_get(const uint8_t *base, uint32_t off, uin
   }
 }
 
-// parsePly's vertex loop (src/scene/splats.js:457-467).
+// parsePly's vertex loop (src/scene/splats.js:457-467), a workgroup 256
+// vertices. STAGED (a stride and offsets of whole words, every property a
+// float, stride <= kPlyStagedStride): the workgroup's vertices are first
+// copied into LDS with 16-byte loads, so the body is read once, in order.
+template <int DEG, bool STAGED>
 __global__ __launch_bounds__(256) void k_decode_ply(const uint8_t *__restrict__ body, uint32_t n, PlyLayout L,
                                                     DecodeOut o) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= n) return;
-  const uint8_t *b = body + (size_t)i * L.stride;
-  auto get = [&](int k) { return ply_get(b, L.off[k], L.type[k]); };
-  const double op = get(6);
-  const double alpha = 1.0 / (1.0 + exp(-op));
-  put(o, i, get(0), get(1), get(2), js_byte(0.5 + kShC0 * get(3)), js_byte(0.5 + kShC0 * get(4)),
-      js_byte(0.5 + kShC0 * get(5)), js_byte(alpha), exp(get(7)), exp(get(8)), exp(get(9)), get(10), get(11),
-      get(12), get(13));
-  const uint32_t per = L.rest / 3;
-  for (uint32_t k = 0; k < per; ++k)
-    put_sh(o, i, k, get(kPlyNeeds + k), get(kPlyNeeds + k + per), get(kPlyNeeds + k + 2 * per));
+  extern __shared__ __align__(16) uint32_t lds[];
+  const uint32_t t = threadIdx.x, i0 = blockIdx.x * 256u, cnt = min(256u, n - i0);
+  const bool valid = t < cnt;
+  const uint8_t *b;
+  if constexpr (STAGED) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(body + (size_t)i0 * L.stride);
+    const uint32_t n16 = (cnt * L.stride + 15u) / 16u;  // the body is padded to 16 bytes past its end
+    for (uint32_t k = t; k < n16; k += 256) reinterpret_cast<uint4 *>(lds)[k] = src[k];
+    __syncthreads();
+    b = reinterpret_cast<const uint8_t *>(lds) + (size_t)t * L.stride;
+  } else {
+    b = body + (size_t)(i0 + (valid ? t : 0u)) * L.stride;
+  }
+  Decoded<DEG> d{};
+  if (valid) {
+    auto get = [&](int k) {
+      if constexpr (STAGED) return (double)__uint_as_float(*reinterpret_cast<const uint32_t *>(b + L.off[k]));
+      return ply_get(b, L.off[k], L.type[k]);
+    };
+    const double alpha = 1.0 / (1.0 + exp(-get(6)));
+    put(d, get(0), get(1), get(2), js_byte(0.5 + kShC0 * get(3)), js_byte(0.5 + kShC0 * get(4)),
+        js_byte(0.5 + kShC0 * get(5)), js_byte(alpha), exp(get(7)), exp(get(8)), exp(get(9)), get(10), get(11),
+        get(12), get(13));
+    constexpr uint32_t per = kShCoeffs[DEG];
+    if constexpr (STAGED) {
+      auto getf = [&](int k) { return __uint_as_float(*reinterpret_cast<const uint32_t *>(b + L.off[k])); };
+#pragma unroll
+      for (uint32_t k = 0; k < per; ++k)
+        put_sh_f32(d, k, getf(kPlyNeeds + k), getf(kPlyNeeds + k + per), getf(kPlyNeeds + k + 2 * per));
+    } else {
+#pragma unroll
+      for (uint32_t k = 0; k < per; ++k)
+        put_sh(d, k, get(kPlyNeeds + k), get(kPlyNeeds + k + per), get(kPlyNeeds + k + 2 * per));
+    }
+  }
+  if constexpr (STAGED) __syncthreads();  // the staged vertices are read; the LDS takes the outputs
+  emit(o, i0, cnt, valid, d, lds);
 }
 
 // parseSpz's loop (src/scene/splats.js:257-301), over the sections of a gunzipped .spz.
+template <int DEG>
 __global__ __launch_bounds__(256) void k_decode_spz(const uint8_t *__restrict__ data, uint32_t n, SpzLayout L,
                                                     DecodeOut o) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= n) return;
-  double pos[3];
-  for (int a = 0; a < 3; ++a) {
-    if (L.version == 1) {
-      const uint8_t *p = data + L.positions + (size_t)i * 6 + a * 2;
-      const uint32_t h = p[0] | (p[1] << 8);
-      const int e = (h >> 10) & 31;
-      const uint32_t m = h & 1023u;
-      const double sign = (h & 0x8000u) ? -1.0 : 1.0;
-      pos[a] = e == 31 ? (m ? NAN : sign * INFINITY)
-                       : sign * (e == 0 ? (double)m * 0x1p-24 : (1.0 + m / 1024.0) * ldexp(1.0, e - 15));
+  extern __shared__ __align__(16) uint32_t lds[];
+  const uint32_t t = threadIdx.x, i0 = blockIdx.x * 256u, cnt = min(256u, n - i0);
+  const bool valid = t < cnt;
+  const uint32_t i = i0 + t;
+  Decoded<DEG> d{};
+  if (valid) {
+    double pos[3];
+    for (int a = 0; a < 3; ++a) {
+      if (L.version == 1) {
+        const uint8_t *p = data + L.positions + (size_t)i * 6 + a * 2;
+        const uint32_t h = p[0] | (p[1] << 8);
+        const int e = (h >> 10) & 31;
+        const uint32_t m = h & 1023u;
+        const double sign = (h & 0x8000u) ? -1.0 : 1.0;
+        pos[a] = e == 31 ? (m ? NAN : sign * INFINITY)
+                         : sign * (e == 0 ? (double)m * 0x1p-24 : (1.0 + m / 1024.0) * ldexp(1.0, e - 15));
+      } else {
+        const uint8_t *p = data + L.positions + (size_t)i * 9 + a * 3;
+        const int32_t fixed = (int32_t)((uint32_t)(p[0] | (p[1] << 8) | (p[2] << 16)) << 8) >> 8;
+        pos[a] = (double)fixed * L.fixed;
+      }
+    }
+    double q[4];
+    const uint8_t *r = data + L.rotations + (size_t)i * L.rotation_bytes;
+    if (L.rotation_bytes == 3) {
+      for (int a = 0; a < 3; ++a) q[a] = r[a] / 127.5 - 1.0;
+      const double s = 1.0 - (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+      q[3] = sqrt(s > 0.0 ? s : 0.0);
     } else {
-      const uint8_t *p = data + L.positions + (size_t)i * 9 + a * 3;
-      const int32_t fixed = (int32_t)((uint32_t)(p[0] | (p[1] << 8) | (p[2] << 16)) << 8) >> 8;
-      pos[a] = (double)fixed * L.fixed;
+      // smallest three: the largest component's index in the top two bits, the
+      // others as a sign and nine bits of magnitude each, last first
+      uint32_t packed = r[0] | (r[1] << 8) | (r[2] << 16) | ((uint32_t)r[3] << 24);
+      const uint32_t largest = packed >> 30;
+      double sum = 0.0;
+      for (int a = 3; a >= 0; --a) {
+        if ((uint32_t)a == largest) continue;
+        const uint32_t magnitude = packed & 511u;
+        const uint32_t negative = (packed >> 9) & 1u;
+        packed >>= 10;
+        q[a] = 0.7071067811865476 * magnitude / 511.0 * (negative ? -1.0 : 1.0);
+        sum += q[a] * q[a];
+      }
+      const double s = 1.0 - sum;
+      q[largest] = sqrt(s > 0.0 ? s : 0.0);
+    }
+    const uint8_t *c = data + L.colours + (size_t)i * 3;
+    const uint8_t *s = data + L.scales + (size_t)i * 3;
+    auto colour = [&](int k) { return js_byte(0.5 + kShC0 * ((c[k] / 255.0 - 0.5) / 0.15)); };
+    auto scale = [&](int k) { return exp(s[k] / 16.0 - 10.0); };
+    // RUB to RDF: a position's y and z negate, and so do a rotation's y and z
+    put(d, pos[0], -pos[1], -pos[2], colour(0), colour(1), colour(2), data[L.alphas + i], scale(0), scale(1),
+        scale(2), q[3], q[0], -q[1], -q[2]);
+    const uint8_t *h = data + L.sh + (size_t)i * L.file_coefficients * 3;
+#pragma unroll
+    for (uint32_t k = 0; k < kShCoeffs[DEG]; ++k) {
+      // (byte - 128) / 128, flipped: exact in f32
+      const float flip = ((kSpzShNegative >> k) & 1u) ? -1.0f : 1.0f;
+      put_sh_f32(d, k, (h[k * 3] - 128) / 128.0f * flip, (h[k * 3 + 1] - 128) / 128.0f * flip,
+                 (h[k * 3 + 2] - 128) / 128.0f * flip);
     }
   }
-  double q[4];
-  const uint8_t *r = data + L.rotations + (size_t)i * L.rotation_bytes;
-  if (L.rotation_bytes == 3) {
-    for (int a = 0; a < 3; ++a) q[a] = r[a] / 127.5 - 1.0;
-    const double s = 1.0 - (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-    q[3] = sqrt(s > 0.0 ? s : 0.0);
-  } else {
-    // smallest three: the largest component's index in the top two bits, the
-    // others as a sign and nine bits of magnitude each, last first
-    uint32_t packed = r[0] | (r[1] << 8) | (r[2] << 16) | ((uint32_t)r[3] << 24);
-    const uint32_t largest = packed >> 30;
-    double sum = 0.0;
-    for (int a = 3; a >= 0; --a) {
-      if ((uint32_t)a == largest) continue;
-      const uint32_t magnitude = packed & 511u;
-      const uint32_t negative = (packed >> 9) & 1u;
-      packed >>= 10;
-      q[a] = 0.7071067811865476 * magnitude / 511.0 * (negative ? -1.0 : 1.0);
-      sum += q[a] * q[a];
-    }
-    const double s = 1.0 - sum;
-    q[largest] = sqrt(s > 0.0 ? s : 0.0);
-  }
-  const uint8_t *c = data + L.colours + (size_t)i * 3;
-  const uint8_t *s = data + L.scales + (size_t)i * 3;
-  auto colour = [&](int k) { return js_byte(0.5 + kShC0 * ((c[k] / 255.0 - 0.5) / 0.15)); };
-  auto scale = [&](int k) { return exp(s[k] / 16.0 - 10.0); };
-  // RUB to RDF: a position's y and z negate, and so do a rotation's y and z
-  put(o, i, pos[0], -pos[1], -pos[2], colour(0), colour(1), colour(2), data[L.alphas + i], scale(0), scale(1),
-      scale(2), q[3], q[0], -q[1], -q[2]);
-  const uint8_t *h = data + L.sh + (size_t)i * L.file_coefficients * 3;
-  for (uint32_t k = 0; k < L.coefficients; ++k) {
-    const double flip = ((kSpzShNegative >> k) & 1u) ? -1.0 : 1.0;
-    put_sh(o, i, k, (h[k * 3] - 128) / 128.0 * flip, (h[k * 3 + 1] - 128) / 128.0 * flip,
-           (h[k * 3 + 2] - 128) / 128.0 * flip);
-  }
+  emit(o, i0, cnt, valid, d, lds);
 }
 
 // readSog's loop (src/scene/splats.js:170-203), over the decoded images.
+template <int DEG>
 __global__ __launch_bounds__(256) void k_decode_sog(SogLayout L, DecodeOut o) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= L.count) return;
-  const uint32_t px = 4 * i;
-  double m[3];
-  for (int a = 0; a < 3; ++a) {
-    // stored as sign(x) ln(|x| + 1) in a 16-bit range
-    const double v = L.means_l[px + a] | (L.means_u[px + a] << 8);
-    const double range = L.maxs[a] - L.mins[a];
-    const double n = L.mins[a] + ((range != 0.0 && range == range) ? range : 1.0) * v / 65535.0;
-    const double e = exp(fabs(n)) - 1.0;
-    m[a] = n < 0 ? -e : e;
-  }
-  double q[4] = {1.0, 0.0, 0.0, 0.0};
-  const uint32_t tag = L.quats[px + 3];
-  if (tag >= 252) {
-    const uint32_t largest = tag - 252;
-    uint32_t k = 0;
-    double sum = 0.0;
-    for (uint32_t cpt = 0; cpt < 4; ++cpt) {
-      if (cpt == largest) continue;
-      q[cpt] = ((L.quats[px + k++] / 255.0) * 2.0 - 1.0) / 1.4142135623730951;
-      sum += q[cpt] * q[cpt];
+  extern __shared__ __align__(16) uint32_t lds[];
+  const uint32_t t = threadIdx.x, i0 = blockIdx.x * 256u, cnt = min(256u, L.count - i0);
+  const bool valid = t < cnt;
+  const uint32_t px = 4 * (i0 + t);
+  Decoded<DEG> d{};
+  if (valid) {
+    double m[3];
+    for (int a = 0; a < 3; ++a) {
+      // stored as sign(x) ln(|x| + 1) in a 16-bit range
+      const double v = L.means_l[px + a] | (L.means_u[px + a] << 8);
+      const double range = L.maxs[a] - L.mins[a];
+      const double nn = L.mins[a] + ((range != 0.0 && range == range) ? range : 1.0) * v / 65535.0;
+      const double e = exp(fabs(nn)) - 1.0;
+      m[a] = nn < 0 ? -e : e;
     }
-    const double s = 1.0 - sum;
-    q[largest] = sqrt(s > 0.0 ? s : 0.0);
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    const uint32_t tag = L.quats[px + 3];
+    if (tag >= 252) {
+      const uint32_t largest = tag - 252;
+      uint32_t k = 0;
+      double sum = 0.0;
+      for (uint32_t cpt = 0; cpt < 4; ++cpt) {
+        if (cpt == largest) continue;
+        q[cpt] = ((L.quats[px + k++] / 255.0) * 2.0 - 1.0) / 1.4142135623730951;
+        sum += q[cpt] * q[cpt];
+      }
+      const double s = 1.0 - sum;
+      q[largest] = sqrt(s > 0.0 ? s : 0.0);
+    }
+    put(d, m[0], m[1], m[2], L.colour_byte[L.sh0[px]], L.colour_byte[L.sh0[px + 1]], L.colour_byte[L.sh0[px + 2]],
+        L.sh0[px + 3], L.scale_exp[L.scales[px]], L.scale_exp[L.scales[px + 1]], L.scale_exp[L.scales[px + 2]],
+        q[0], q[1], q[2], q[3]);
+    if constexpr (DEG > 0) {
+      const uint32_t label = L.labels[px] | (L.labels[px + 1] << 8);
+      if (label < L.sh_count) {
+        constexpr uint32_t per = kShCoeffs[DEG];
+        const uint64_t row = (uint64_t)(label / 64) * L.centroids_width + (label % 64) * per;
+#pragma unroll
+        for (uint32_t k = 0; k < per; ++k) {
+          const uint64_t c = (row + k) * 4;
+          if (c + 2 >= L.centroids_bytes) break;
+          set_half(d, 3 * k, L.sh_half[L.centroids[c]]);
+          set_half(d, 3 * k + 1, L.sh_half[L.centroids[c + 1]]);
+          set_half(d, 3 * k + 2, L.sh_half[L.centroids[c + 2]]);
+        }
+      }
+    }
   }
-  put(o, i, m[0], m[1], m[2], L.colour_byte[L.sh0[px]], L.colour_byte[L.sh0[px + 1]], L.colour_byte[L.sh0[px + 2]],
-      L.sh0[px + 3], L.scale_exp[L.scales[px]], L.scale_exp[L.scales[px + 1]], L.scale_exp[L.scales[px + 2]], q[0],
-      q[1], q[2], q[3]);
-  if (L.per == 0) return;
-  const uint32_t label = L.labels[px] | (L.labels[px + 1] << 8);
-  if (label >= L.sh_count) return;
-  const uint64_t row = (uint64_t)(label / 64) * L.centroids_width + (label % 64) * L.per;
-  for (uint32_t k = 0; k < L.per; ++k) {
-    const uint64_t c = (row + k) * 4;
-    if (c + 2 >= L.centroids_bytes) break;
-    uint16_t *h = reinterpret_cast<uint16_t *>(o.sh) + (size_t)i * o.sh_words * 2 + k * 3;
-    h[0] = L.sh_half[L.centroids[c]];
-    h[1] = L.sh_half[L.centroids[c + 1]];
-    h[2] = L.sh_half[L.centroids[c + 2]];
-  }
+  emit(o, i0, cnt, valid, d, lds);
 }
 
 }  // namespace
 
+template <typename K>
+static K pick(const K (&forms)[4], uint32_t degree) { return forms[degree <= 3 ? degree : 0]; }
+
 int launch_decode_ply(wd_context *ctx, const uint8_t *body, uint32_t n, const PlyLayout &L, const DecodeOut &o) {
-  hipLaunchKernelGGL(k_decode_ply, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, body, n, L, o);
+  using Kernel = void (*)(const uint8_t *, uint32_t, PlyLayout, DecodeOut);
+  static const Kernel kDirect[4] = {k_decode_ply<0, false>, k_decode_ply<1, false>, k_decode_ply<2, false>,
+                                    k_decode_ply<3, false>};
+  static const Kernel kStaged[4] = {k_decode_ply<0, true>, k_decode_ply<1, true>, k_decode_ply<2, true>,
+                                    k_decode_ply<3, true>};
+  const uint32_t degree = L.rest / 3 == 15 ? 3 : L.rest / 3 == 8 ? 2 : L.rest / 3 == 3 ? 1 : 0;
+  bool staged = L.stride % 4 == 0 && L.stride <= kPlyStagedStride;
+  for (uint32_t k = 0; k < (uint32_t)kPlyNeeds + L.rest && staged; ++k)
+    staged = L.type[k] == kPlyF32 && L.off[k] % 4 == 0;
+  const size_t lds = std::max<size_t>(staged ? 256u * L.stride : 0u, 4u * emit_words(3));
+  hipEvent_t ev = nullptr;
+  if (ctx->prof_begin(kKDecode, &ev)) return WD_E_HIP;
+  hipLaunchKernelGGL(pick(staged ? kStaged : kDirect, degree), dim3((n + 255) / 256), dim3(256), lds, ctx->stream,
+                     body, n, L, o);
   WD_HIP(hipGetLastError(), "splats: .ply decode launch");
+  if (ctx->prof_end(kKDecode, ev)) return WD_E_HIP;
   return WD_OK;
 }
 
 int launch_decode_spz(wd_context *ctx, const uint8_t *data, uint32_t n, const SpzLayout &L, const DecodeOut &o) {
-  hipLaunchKernelGGL(k_decode_spz, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, data, n, L, o);
+  using Kernel = void (*)(const uint8_t *, uint32_t, SpzLayout, DecodeOut);
+  static const Kernel kForms[4] = {k_decode_spz<0>, k_decode_spz<1>, k_decode_spz<2>, k_decode_spz<3>};
+  const uint32_t degree = L.coefficients == 15 ? 3 : L.coefficients == 8 ? 2 : L.coefficients == 3 ? 1 : 0;
+  hipEvent_t ev = nullptr;
+  if (ctx->prof_begin(kKDecode, &ev)) return WD_E_HIP;
+  hipLaunchKernelGGL(pick(kForms, degree), dim3((n + 255) / 256), dim3(256), 4u * emit_words(3), ctx->stream, data, n,
+                     L, o);
   WD_HIP(hipGetLastError(), "splats: .spz decode launch");
+  if (ctx->prof_end(kKDecode, ev)) return WD_E_HIP;
   return WD_OK;
 }
 
 int launch_decode_sog(wd_context *ctx, const SogLayout &L, const DecodeOut &o) {
-  hipLaunchKernelGGL(k_decode_sog, dim3((L.count + 255) / 256), dim3(256), 0, ctx->stream, L, o);
+  using Kernel = void (*)(SogLayout, DecodeOut);
+  static const Kernel kForms[4] = {k_decode_sog<0>, k_decode_sog<1>, k_decode_sog<2>, k_decode_sog<3>};
+  const uint32_t degree = L.per == 15 ? 3 : L.per == 8 ? 2 : L.per == 3 ? 1 : 0;
+  hipEvent_t ev = nullptr;
+  if (ctx->prof_begin(kKDecode, &ev)) return WD_E_HIP;
+  hipLaunchKernelGGL(pick(kForms, degree), dim3((L.count + 255) / 256), dim3(256), 4u * emit_words(3), ctx->stream,
+                     L, o);
   WD_HIP(hipGetLastError(), "splats: .sog decode launch");
+  if (ctx->prof_end(kKDecode, ev)) return WD_E_HIP;
   return WD_OK;
 }
 

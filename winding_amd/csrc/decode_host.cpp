@@ -115,13 +115,15 @@ int decode_into(wd_context *ctx, uint64_t n, uint32_t degree, const std::functio
 }
 
 // A host buffer copied to the device for the length of one decode.
+// 16 bytes over the end, zeroed, so that 16-byte loads of the last whole words stay inside it.
 struct DeviceCopy {
   void *p = nullptr;
   ~DeviceCopy() {
     if (p) hipFree(p);
   }
   hipError_t from(const void *src, size_t bytes) {
-    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    hipError_t e = hipMalloc(&p, bytes + 16);
+    if (e == hipSuccess) e = hipMemset((uint8_t *)p + bytes, 0, 16);
     return e == hipSuccess && bytes ? hipMemcpy(p, src, bytes, hipMemcpyHostToDevice) : e;
   }
 };

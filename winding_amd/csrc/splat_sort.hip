@@ -333,7 +333,7 @@ ViewK make_viewk(const wd_view &v) {
 
 const char *const kKernelNames[kKernelCount] = {
     "k_bounds", "k_project", "k_scan", "k_scatter_lo", "k_hist",        "k_scan2",
-    "k_scatter_hi", "k_batch", "k_mlp", "k_bin_table", "k_bin_scatter", "k_bin_sort", "k_particle_keys"};
+    "k_scatter_hi", "k_batch", "k_mlp", "k_bin_table", "k_bin_scatter", "k_bin_sort", "k_particle_keys", "k_decode"};
 
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8) {
   static const uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0, 0xFFFFFFFFu, 0};

@@ -245,7 +245,7 @@ namespace wd {
 
 enum KernelId {
   kKBounds = 0, kKProject, kKScan, kKScatterLo, kKHist, kKScan2, kKScatterHi, kKBatch, kKMlp, kKBinTable, kKBinScatter,
-  kKBinSort, kKParticleKeys, kKernelCount
+  kKBinSort, kKParticleKeys, kKDecode, kKernelCount
 };
 extern const char *const kKernelNames[kKernelCount];
 
