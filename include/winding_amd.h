@@ -151,7 +151,8 @@ int wd_cloud_destroy(wd_cloud *cloud);
  * the harmonics along the direction from the eye (the inverse model-view's
  * translation, as SplatPass.prepare sets it, :585-586), clamped to [0, 1] and
  * packed as unorm bytes, alpha kept; in f32, within one unit of the byte of
- * the FP64 evaluation. Batches and learned runs refuse such a cloud.
+ * the FP64 evaluation. Batches shade it too under the reverse-Z perspective
+ * (another projection is refused there); learned runs refuse it.
  */
 int wd_cloud_sh(wd_context *ctx, wd_cloud *cloud, const uint32_t *words,
                 uint64_t n_words, uint32_t degree);
@@ -163,8 +164,9 @@ int wd_cloud_sh(wd_context *ctx, wd_cloud *cloud, const uint32_t *words,
  * in the fourth) and 6 covariance floats each (Sigma's xx xy xz yy yz zz), host
  * memory. Refuses a non-finite position by index, as put() does
  * (src/scene/splats.js:336-338), and fills the box. Its sorts decide the
- * footprint and the cull in FP64 from the f32 Sigma. Batches and learned runs
- * refuse such a cloud; wd_cloud_records gives null for it.
+ * footprint and the cull in FP64 from the f32 Sigma. Batches take such a
+ * cloud under the reverse-Z perspective; learned runs refuse it;
+ * wd_cloud_records gives null for it.
  */
 int wd_cloud_create_splats(wd_context *ctx, const float *centers,
                            const float *covariances, uint64_t count,

@@ -446,15 +446,36 @@ def test_gpu_sort_of_a_decoded_cloud_is_bit_exact(ctx, cull):
             assert np.array_equal(order2, order) and np.array_equal(cov2.view(np.uint32), cov.view(np.uint32))
         finally:
             host.destroy()
-        # batches take .splat records only
+        # one batch of every form: the decoded cloud (centres, Sigma, harmonics), the same without
+        # harmonics, and .splat records; each job's output equals its single sort's
         plain = wa.Cloud.from_splats(ctx, centers, covs)
-        b = wa.Batch(ctx, 1)
+        recs = wa.Cloud(ctx, wa.workloads.plain("plain-64k", n=30000)[0])
+        jobs = [(cloud, view), (plain, view),
+                (recs, wa.prepare_view(v, p, 1280.0, 720.0, recs.min, recs.max, cull))]
+        states = [wa.State(ctx, c.count) for c, _ in jobs]
+        b = wa.Batch(ctx, len(jobs))
         try:
-            with pytest.raises(wa.WindingError, match="centres and covariances"):
-                b.set([(plain, view, st)])
+            want = []
+            for (c, vw), s in zip(jobs, states):
+                s.sort(c, vw)
+                o, cv, _ = s.read(n_cov=c.count)
+                want.append((o, cv.view(np.uint32).copy()))
+            b.set([(c, vw, s) for (c, vw), s in zip(jobs, states)])
+            for run in range(2):
+                b.run()
+                for ((c, vw), s), (o, cv) in zip(zip(jobs, states), want):
+                    got_o, got_c, _ = s.read(n_cov=c.count)
+                    assert np.array_equal(got_o, o) and np.array_equal(got_c.view(np.uint32), cv), run
+            # the harmonics changed since the jobs were set: refused until they are set again
+            plain.set_sh(sh, degree)
+            with pytest.raises(wa.WindingError, match="harmonics changed"):
+                b.run()
         finally:
             b.destroy()
+            for s in states:
+                s.destroy()
             plain.destroy()
+            recs.destroy()
     finally:
         st.destroy()
         cloud.destroy()

@@ -190,12 +190,21 @@ def test_gpu_known_answer_and_refusals(ctx):
             cloud.set_sh(words, 4)
         with pytest.raises(wa.WindingError, match="take 23 words"):
             cloud.set_sh(words, 3)
+        # a batch shades as the single sort does; under a projection other than the reverse-Z
+        # perspective it refuses such a cloud
         batch = wa.Batch(ctx, 1)
         try:
             v, p = wa.workloads.camera((5, 0, 0), 160, 120)
             view = wa.prepare_view(v, p, 160.0, 120.0, cloud.min, cloud.max, 0.0)
-            with pytest.raises(wa.WindingError, match="harmonics"):
-                batch.set([(cloud, view, st)])
+            batch.set([(cloud, view, st)])
+            batch.run()
+            _, cov, stats = st.read(n_cov=1)
+            assert stats.count == 1 and rgb(cov[0, 3].view(np.uint32)) == seen["plus"]
+            ortho = np.zeros(16, np.float32)
+            ortho[0], ortho[5], ortho[10], ortho[14], ortho[15] = 0.1, 0.1, -0.01, 0.5, 1.0
+            view2 = wa.prepare_view(v, ortho, 160.0, 120.0, cloud.min, cloud.max, 0.0)
+            with pytest.raises(wa.WindingError, match="reverse-Z perspective only"):
+                batch.set([(cloud, view2, st)])
         finally:
             batch.destroy()
     finally:

@@ -52,12 +52,11 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     if (!J.cloud || !b->ctx->live.count(J.cloud)) return fail(WD_E_STATE, "batch: job %u: its splats were unloaded", i);
     if (!J.state || !b->ctx->live.count(J.state)) return fail(WD_E_STATE, "batch: job %u: its sort state was destroyed", i);
     if (J.cloud->ctx != b->ctx || J.state->ctx != b->ctx) return fail(WD_E_STATE, "batch: job %u is of another device", i);
-    if (J.cloud->degree)
-      return fail(WD_E_ARG, "batch: job %u: its splats have harmonics, which only a single sort (wd_sort) shades", i);
     if (J.state->ellipse)
       return fail(WD_E_ARG, "batch: job %u: its state takes the draw's ellipses, which only a single sort (wd_sort) writes", i);
-    if (!J.cloud->records)
-      return fail(WD_E_ARG, "batch: job %u: its splats are centres and covariances, which only a single sort (wd_sort) takes", i);
+    if ((J.cloud->degree || !J.cloud->records) && !wd::is_std(J.view))
+      return fail(WD_E_ARG, "batch: job %u: splats with harmonics, or given as centres and covariances, are batched "
+                  "under the reverse-Z perspective only; sort them with wd_sort", i);
     if (J.cloud->n > J.state->capacity)
       return fail(WD_E_CAPACITY, "batch: job %u: %llu splats are more than its state holds, %llu", i,
                   (unsigned long long)J.cloud->n, (unsigned long long)J.state->capacity);
@@ -67,7 +66,10 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     wd::BatchJob &B = hj[i];
     B.view = wd::make_viewk(J.view);
     B.std_form = wd::is_std(J.view) ? 1u : 0u;
-    B.rec = J.cloud->records;
+    B.rec = J.cloud->records ? (const void *)J.cloud->records : (const void *)J.cloud->centers;
+    B.sig = J.cloud->records ? nullptr : (const uint2 *)J.cloud->covs;
+    B.sh = J.cloud->degree ? J.cloud->sh : nullptr;
+    B.degree = J.cloud->degree;
     B.n = (uint32_t)J.cloud->n;
     wd_state *s = J.state;
     B.keys = s->keys;
@@ -119,6 +121,8 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   b->host_starts.swap(starts);
   b->shape.swap(shape);
   if (b->api_jobs.data() != jobs) b->api_jobs.assign(jobs, jobs + n);
+  b->sh_set.clear();
+  for (uint32_t i = 0; i < n; ++i) b->sh_set.emplace_back(jobs[i].cloud->sh, jobs[i].cloud->degree);
   b->n = n;
   b->total = total;
   return same_total && b->grid ? WD_OK : wd::batch_grid(ctx, total, &b->grid);
@@ -129,8 +133,11 @@ int wd_batch_run(wd_context *ctx, wd_batch *b) {
   if (b->n == 0) return fail(WD_E_STATE, "batch: no jobs set");
   for (const void *p : b->shape)
     if (!ctx->live.count(p)) return fail(WD_E_STATE, "batch: a job's splats or sort state was destroyed; set the jobs again");
-  for (const wd_job &J : b->api_jobs)
-    if (J.cloud->degree) return fail(WD_E_ARG, "batch: a job's splats have harmonics, which only a single sort (wd_sort) shades");
+  for (size_t i = 0; i < b->api_jobs.size(); ++i) {
+    const wd_cloud *c = b->api_jobs[i].cloud;
+    if (c->sh != b->sh_set[i].first || c->degree != b->sh_set[i].second)
+      return fail(WD_E_STATE, "batch: a job's harmonics changed since the jobs were set; set the jobs again");
+  }
   for (const wd_job &J : b->api_jobs)
     if (J.state->ellipse)
       return fail(WD_E_ARG, "batch: a job's state takes the draw's ellipses, which only a single sort (wd_sort) writes");

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <string>
 #include <unordered_set>
+#include <utility>
 #include <vector>
 
 #include "winding_amd.h"
@@ -97,6 +98,10 @@ struct BatchJob {
   WD_GLOBAL uint16_t *lut;          // [4096] 16-key block -> (bin << 9 | the block's offset in its bin) >> 4
   WD_GLOBAL uint32_t *prefix_bin;   // [tiles][512] where a tile's run of each bin starts
   const WD_GLOBAL float *probs;     // [256] predicted bin widths (the predictor's output row)
+  // a cloud given as centres and Sigma: rec is the centres, sig Sigma (null for .splat records)
+  const WD_GLOBAL uint2 *sig;
+  const WD_GLOBAL uint32_t *sh;     // harmonics (degree > 0), the cloud's sh words
+  uint32_t degree;
   uint32_t cap;           // most survivors a bin may hold; more and the view runs the exact path
   uint32_t n, std_form;
   uint32_t tiles;    // projection chunks = scatter tiles, of kTile splats
@@ -213,6 +218,7 @@ struct wd_batch {
   int schedule = WD_SCHEDULE_PHASED;
   uint32_t super_tiles = 1;                     // the scatter work item: tiles a workgroup (1 or 2)
   std::vector<wd_job> api_jobs;                 // the jobs as set, to rebuild the tables on a change
+  std::vector<std::pair<const uint32_t *, uint32_t>> sh_set;  // each job's harmonics (words, degree) as set
   void *stage[2] = {nullptr, nullptr};          // pinned staging for the tables
   hipEvent_t staged[2] = {nullptr, nullptr};    // the copy out of stage[i] has finished
   bool staged_used[2] = {false, false};

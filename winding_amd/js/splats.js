@@ -204,6 +204,8 @@ class Splats {
     this.handle = made.handle;
     this.count = made.count;
     this.degree = degree;
+    /** Given as centres and covariances (decoded, or the reference's parsed data), not .splat records. */
+    this.sigma = data.format !== undefined || data.centers instanceof Float32Array;
     this.min = made.min;
     this.max = made.max;
     this.unloaded = false;
@@ -282,6 +284,12 @@ function cloudUniform(out, view, world, projection, width, height, min, max, cul
   out[35] = (BUCKETS - 1) / Math.max(far - near, 1e-6);
   out[36] = cull;
   return out;
+}
+
+/** The reverse-Z infinite perspective over an affine model-view, the uniform's floats 0-31. */
+function reverseZ(u) {
+  return u[17] === 0 && u[18] === 0 && u[19] === 0 && u[20] === 0 && u[22] === 0 && u[23] === 0 && u[26] === 0
+    && u[27] === -1 && u[28] === 0 && u[29] === 0 && u[31] === 0 && u[3] === 0 && u[7] === 0 && u[11] === 0 && u[15] === 1;
 }
 
 class SplatPass {
@@ -364,9 +372,10 @@ class SplatPass {
   sort() {
     const marked = this._items.filter((i) => i.sort);
     for (const item of marked) item.state.last.set(item.state.next);
-    // Clouds with harmonics, or whose state makes the draw's ellipses, have a sort of their own;
-    // the rest share one batch launch.
-    const own = (i) => i.state.splats.degree > 0 || i.state.ellipses;
+    // A state that makes the draw's ellipses has a sort of its own, and so has a cloud with
+    // harmonics or of centres and covariances under a projection other than the reverse-Z
+    // perspective (the only one a batch shades and takes them under); the rest share one launch.
+    const own = (i) => i.state.ellipses || ((i.state.splats.degree > 0 || i.state.splats.sigma) && !reverseZ(i.state.uniform));
     const plain = marked.filter((i) => !own(i));
     for (const item of marked) {
       if (own(item)) {
