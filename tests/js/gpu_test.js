@@ -277,5 +277,55 @@ test('captures decoded on the GPU: a .ply, a .spz and a .sog through parseSplats
   s.destroy();
 });
 
-gpu.destroy();
-console.log(passed + ' checks passed');
+/** A zip of the given files, stored (method 0), enough for readSplats' unzip. */
+function storedZip(files) {
+  const parts = [], directory = [];
+  let offset = 0;
+  for (const [name, data] of Object.entries(files)) {
+    const n = Buffer.from(name);
+    const local = Buffer.alloc(30 + n.length);
+    local.writeUInt32LE(0x04034b50, 0); local.writeUInt32LE(data.length, 18); local.writeUInt32LE(data.length, 22);
+    local.writeUInt16LE(n.length, 26); n.copy(local, 30);
+    const central = Buffer.alloc(46 + n.length);
+    central.writeUInt32LE(0x02014b50, 0); central.writeUInt32LE(data.length, 20); central.writeUInt32LE(data.length, 24);
+    central.writeUInt16LE(n.length, 28); central.writeUInt32LE(offset, 42); n.copy(central, 46);
+    parts.push(local, Buffer.from(data));
+    directory.push(central);
+    offset += local.length + data.length;
+  }
+  const size = directory.reduce((a, d) => a + d.length, 0);
+  const end = Buffer.alloc(22);
+  end.writeUInt32LE(0x06054b50, 0); end.writeUInt16LE(directory.length, 10); end.writeUInt32LE(size, 12); end.writeUInt32LE(offset, 16);
+  return new Uint8Array(Buffer.concat([...parts, ...directory, end]));
+}
+
+(async () => {
+  // a one-splat .sog without harmonics: its images are their pixels as JSON, which decodeImage reads
+  const image = (rgba) => Buffer.from(JSON.stringify({ width: 1, height: 1, rgba }));
+  const meta = {
+    version: 2, count: 1, means: { mins: [-1, -1, -1], maxs: [1, 1, 1], files: ['l.webp', 'u.webp'] },
+    scales: { codebook: Array.from({ length: 256 }, (_, k) => -6 + k * 0.05), files: ['s.webp'] },
+    quats: { files: ['q.webp'] }, sh0: { codebook: Array.from({ length: 256 }, (_, k) => -2 + k / 64), files: ['c.webp'] },
+  };
+  const bytes = storedZip({
+    'meta.json': Buffer.from(JSON.stringify(meta)),
+    'l.webp': image([0, 0xff, 0x80, 255]), 'u.webp': image([0x80, 0xff, 0x80, 255]),
+    'q.webp': image([128, 128, 128, 0]), 's.webp': image([100, 100, 100, 255]), 'c.webp': image([160, 128, 96, 200]),
+  });
+  const decodeImage = async (file) => JSON.parse(Buffer.from(file).toString());
+  const data = await wd.readSplats(bytes, { decodeImage });
+  const s = new wd.Splats(gpu, data);
+  assert.strictEqual(s.count, 1);
+  assert.strictEqual(s.degree, 0);
+  // x: stored 0x8000 of 65535 between -1 and 1 is just above 0 before exp(|n|) - 1
+  assert.ok(Math.abs(s.min[0] - (Math.exp(-1 + 2 * 32768 / 65535) - 1)) < 1e-6 && s.min[1] > 1.7, Array.from(s.min));
+  s.destroy();
+  await assert.rejects(wd.readSplats(bytes), /a .sog needs decodeImage/);
+  passed++;
+  console.log('  ok  a .sog through readSplats (zip, meta.json, decodeImage) and new Splats');
+  gpu.destroy();
+  console.log(passed + ' checks passed');
+})().catch((error) => {
+  console.error(error);
+  process.exit(1);
+});
