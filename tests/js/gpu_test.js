@@ -277,6 +277,39 @@ test('captures decoded on the GPU: a .ply, a .spz and a .sog through parseSplats
   s.destroy();
 });
 
+test('SplatPass batches a cloud with harmonics with a plain one, as their own sorts would order and shade them', () => {
+  const bytes = gpu.native.synth(0, 3, 5000);
+  const sh = new Uint32Array(wd.shWords(2) * 5000);
+  for (let k = 0; k < sh.length; k++) sh[k] = ((k * 2654435761) >>> 0) & 0x33ff33ff;  // halves below 0.25 in size
+  const shaded = new wd.Splats(gpu, { count: 5000, degree: 2, records: bytes, sh });
+  const plain = new wd.Splats(gpu, wd.parseSplats(gpu.native.synth(0, 4, 7000)));
+  const scene = new wd.SplatScene();
+  const a = scene.addSplats({ splats: shaded });
+  const b = scene.addSplats({ splats: plain, matrix: [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 2, 0, 0, 1] });
+  const cam = new wd.Camera({ fovY: 1.0, near: 0.1 });
+  cam.position.set([0, 2, 12]);
+  cam.update(640 / 480);
+  const pass = new wd.SplatPass(gpu);
+  pass.prepare(scene, cam, null, 640, 480);
+  assert.strictEqual(pass.sort(), 2);
+  const got = [pass.read(a, { cov2d: true }), pass.read(b, { cov2d: true })];
+  // each cloud alone, with its own uniform, through the single sort
+  [[shaded, a], [plain, b]].forEach(([splats, entity], k) => {
+    const state = pass._states.get(scene.splats.get(entity));
+    const alone = gpu.native.stateCreate(gpu.handle, splats.count);
+    gpu.native.sort(gpu.handle, splats.handle, state.uniform, alone);
+    const want = gpu.native.stateRead(gpu.handle, alone, splats.count);
+    gpu.native.destroy(alone);
+    assert.strictEqual(got[k].count, want.count);
+    assert.deepStrictEqual(Array.from(got[k].order), Array.from(want.order));
+    assert.deepStrictEqual(new Uint32Array(got[k].cov2d.buffer, got[k].cov2d.byteOffset, splats.count * 4),
+      new Uint32Array(want.cov2d.buffer, want.cov2d.byteOffset, splats.count * 4));
+  });
+  pass.destroy();
+  shaded.destroy();
+  plain.destroy();
+});
+
 /** A zip of the given files, stored (method 0), enough for readSplats' unzip. */
 function storedZip(files) {
   const parts = [], directory = [];
