@@ -12,6 +12,10 @@ Fixtures (numpy .npz, no pickles):
   edges_4096     those 4,096 with every third splat moved onto a depth-bucket edge (+-1e-4 of a bucket)
                  and every fourth survivor rescaled so its opacity x area sits on the 0.5 cull threshold
   slab_4096      4,096 splats of the slab generator (heavy key collisions), 1920x1080
+  cactus_7070    every 64th splat of the reference's own capture, demo/cactus.splat (452,451 splats, CC0),
+                 placed and seen as its demo shows it: turned upright (world diag(1, -1, -1)), eye
+                 (0, 1.6, 3.2) looking at (0, 0.7, 0), fovY pi/3, near 0.05, 1280x720; written only when
+                 the reference tree is at /root/reference (the committed fixture carries the records)
 """
 import math
 import os
@@ -68,6 +72,15 @@ def cases():
     yield "edges_4096", edges_case(r4, v4), v4
     slab = wa.synth_records(1, 0, 4096)
     yield "slab_4096", slab, wa.workloads.view_for(slab, (0.0, 0.0, 10.0), 1920, 1080)
+    capture = "/root/reference/demo/cactus.splat"
+    if os.path.exists(capture):
+        cactus = np.fromfile(capture, np.uint8).reshape(-1, 32)[::64].copy()
+        v = wa.workloads.look_at((0.0, 1.6, 3.2), (0.0, 0.7, 0.0), (0.0, 1.0, 0.0))
+        p = wa.workloads.perspective(math.pi / 3, 1280 / 720, 0.05)
+        world = np.diag([1.0, -1.0, -1.0, 1.0]).astype(np.float32).ravel()
+        bad, mn, mx = oracle.bounds(cactus)
+        assert bad == -1
+        yield "cactus_7070", cactus, wa.prepare_view(v, p, 1280.0, 720.0, mn, mx, 0.5, world)
 
 
 def main():
