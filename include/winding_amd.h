@@ -288,8 +288,9 @@ int wd_state_read_keys(wd_context *ctx, wd_state *state, uint16_t *keys,
  *          viewport clip.w; the Gaussian's offset there is q e.
  * Both are zero for a splat the shader would not draw (culled, or a minor
  * axis or reach not above zero). 0 frees them. f32, within 1e-5 of the
- * largest term of the FP64 evaluation. Batches and learned runs refuse such
- * a state. wd_state_ellipse_buffer gives the device address of plane 0
+ * largest term of the FP64 evaluation. Batches write them too, under the
+ * reverse-Z perspective only (another projection is refused there); learned
+ * runs refuse such a state. wd_state_ellipse_buffer gives the device address of plane 0
  * (plane 1 follows at capacity float4s; null when off); wd_state_read_ellipses
  * copies the first n splats' 8 floats (clip position, then axes) to the host.
  */

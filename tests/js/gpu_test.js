@@ -198,7 +198,22 @@ test('the draw\'s ellipse set-up beside cov2d: a round splat seen head on (src/r
   pass.sort();
   const got = pass.read(entity, { cov2d: true, ellipses: true });
   pass.destroy();
+  // beside a second cloud the two share one batch launch, and the ellipses come out the same
+  const other = new wd.Splats(gpu, wd.parseSplats(record([0.5, 0, -1], [0.2, 0.3, 0.2], [10, 20, 30, 200], [255, 128, 128, 128])));
+  const scene2 = new wd.SplatScene();
+  const first = scene2.addSplats({ splats });
+  scene2.addSplats({ splats: other });
+  const pass2 = new wd.SplatPass(gpu);
+  pass2.cull = 0;
+  pass2.ellipses = true;
+  pass2.prepare(scene2, cam, null, 160, 120);
+  assert.strictEqual(pass2.sort(), 2);
+  const batched = pass2.read(first, { ellipses: true });
+  pass2.destroy();
+  other.destroy();
   splats.destroy();
+  assert.deepStrictEqual(new Uint32Array(batched.ellipses.buffer, batched.ellipses.byteOffset, 8),
+    new Uint32Array(got.ellipses.buffer, got.ellipses.byteOffset, 8));
   assert.strictEqual(got.count, 1);
   const [xx, xy, yy] = got.cov2d;
   const e = got.ellipses;

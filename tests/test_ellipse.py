@@ -170,10 +170,58 @@ def test_gpu_ellipses_general_projection_and_harmonics(ctx):
         batch = wa.Batch(ctx, 1)
         try:
             cloud.set_sh(None, 0)
-            with pytest.raises(wa.WindingError, match="ellipses"):
-                batch.set([(cloud, view, st)])
+            with pytest.raises(wa.WindingError, match="reverse-Z perspective only"):
+                batch.set([(cloud, v2, st)])
         finally:
             batch.destroy()
     finally:
         st.destroy()
         cloud.destroy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", [wa.WD_SCHEDULE_PHASED, wa.WD_SCHEDULE_PERSISTENT])
+def test_gpu_batch_ellipses_equal_single_sorts(ctx, schedule):
+    # one batch: records with ellipses, centres and Sigma with harmonics and ellipses, records without;
+    # each job's order, cov2d and ellipses equal its own single sort's, bit for bit
+    recs, view, _ = wa.workloads.plain("plain-64k", n=20000, cull=0.0)
+    rng = np.random.default_rng(11)
+    centers = np.frombuffer(recs.tobytes(), np.float32).reshape(-1, 8)[:, :4].copy()
+    centers[:, 3] = np.frombuffer(recs[:, 24:28].tobytes(), np.float32)
+    covs = np.stack([oracle.sigma(r) for r in recs[:6000]]).astype(np.float32)
+    words = rng.integers(0, 1 << 14, size=6000 * oracle.sh_words(2)).astype(np.uint32)
+    a = wa.Cloud(ctx, recs)
+    b = wa.Cloud.from_splats(ctx, centers[:6000], covs)
+    b.set_sh(words, 2)
+    c = wa.Cloud(ctx, recs[:9000])
+    jobs = [(a, view, True), (b, view, True), (c, view, False)]
+    states = [wa.State(ctx, cl.count) for cl, _, _ in jobs]
+    singles = [wa.State(ctx, cl.count) for cl, _, _ in jobs]
+    batch = wa.Batch(ctx, 4)
+    try:
+        for (cl, v, ell), st, one in zip(jobs, states, singles):
+            st.set_ellipses(ell)
+            one.set_ellipses(ell)
+            one.sort(cl, v)
+        batch.schedule(schedule)
+        batch.set([(cl, v, st) for (cl, v, _), st in zip(jobs, states)])
+        for run in range(2):
+            batch.run()
+            for (cl, _, ell), st, one in zip(jobs, states, singles):
+                o1, c1, s1 = st.read(n_cov=cl.count)
+                o2, c2, s2 = one.read(n_cov=cl.count)
+                print("run %d: %d splats, %d drawn, ellipses %s" % (run, cl.count, s1.count, ell), flush=True)
+                assert s1.count == s2.count and np.array_equal(o1, o2)
+                assert np.array_equal(c1.view(np.uint32), c2.view(np.uint32))
+                if ell:
+                    assert np.array_equal(st.read_ellipses(cl.count).view(np.uint32),
+                                          one.read_ellipses(cl.count).view(np.uint32))
+        states[2].set_ellipses(True)
+        with pytest.raises(wa.WindingError, match="turned on or off since the jobs were set"):
+            batch.run()
+    finally:
+        batch.destroy()
+        for st in states + singles:
+            st.destroy()
+        for cl in (a, b, c):
+            cl.destroy()

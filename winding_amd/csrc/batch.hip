@@ -64,32 +64,41 @@ __device__ __forceinline__ void count_done(uint32_t *p) {
 }
 
 // Item bodies shared by both schedules.
-// The projection's form by the job: .splat records or centres and Sigma, and
-// the harmonics' degree (the host takes those two only with the reverse-Z
-// perspective, so the general projection has one form here).
-template <bool SIG, int SHD>
+// The projection's form by the job: .splat records or centres and Sigma, the
+// harmonics' degree, and whether the state takes the draw's ellipses (the host
+// takes the last three only with the reverse-Z perspective, so the general
+// projection has one form here).
+template <bool SIG, int SHD, bool ELL>
 __device__ __forceinline__ void project_form(ProjectSmem &sm, const BatchJob &J, uint32_t t) {
-  project_chunk<true, false, SHD, false, SIG>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov,
-                                              J.tilehist, &J.ctl->recomputed_acc, t, nullptr,
-                                              (const uint32_t *)J.sh, nullptr, nullptr, (const uint2 *)J.sig);
+  float4 *ell = (float4 *)J.ellipse;
+  project_chunk<true, false, SHD, ELL, SIG>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov,
+                                            J.tilehist, &J.ctl->recomputed_acc, t, nullptr, (const uint32_t *)J.sh,
+                                            ell, ELL ? reinterpret_cast<const float *>(ell + 2 * J.capacity) : nullptr,
+                                            (const uint2 *)J.sig, J.capacity);
 }
 
-template <bool SIG>
+template <bool SIG, bool ELL>
 __device__ __forceinline__ void project_degree(ProjectSmem &sm, const BatchJob &J, uint32_t t) {
   switch (J.degree) {
-    case 1: project_form<SIG, 1>(sm, J, t); break;
-    case 2: project_form<SIG, 2>(sm, J, t); break;
-    case 3: project_form<SIG, 3>(sm, J, t); break;
-    default: project_form<SIG, 0>(sm, J, t); break;
+    case 1: project_form<SIG, 1, ELL>(sm, J, t); break;
+    case 2: project_form<SIG, 2, ELL>(sm, J, t); break;
+    case 3: project_form<SIG, 3, ELL>(sm, J, t); break;
+    default: project_form<SIG, 0, ELL>(sm, J, t); break;
   }
 }
 
 __device__ __forceinline__ void item_project(ProjectSmem &sm, const BatchJob &J, uint32_t t) {
   if (J.std_form) {
-    if (J.sig)
-      project_degree<true>(sm, J, t);
-    else
-      project_degree<false>(sm, J, t);
+    if (J.ellipse) {
+      if (J.sig)
+        project_degree<true, true>(sm, J, t);
+      else
+        project_degree<false, true>(sm, J, t);
+    } else if (J.sig) {
+      project_degree<true, false>(sm, J, t);
+    } else {
+      project_degree<false, false>(sm, J, t);
+    }
   } else {
     project_chunk<false>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov, J.tilehist,
                          &J.ctl->recomputed_acc, t);

@@ -52,11 +52,10 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     if (!J.cloud || !b->ctx->live.count(J.cloud)) return fail(WD_E_STATE, "batch: job %u: its splats were unloaded", i);
     if (!J.state || !b->ctx->live.count(J.state)) return fail(WD_E_STATE, "batch: job %u: its sort state was destroyed", i);
     if (J.cloud->ctx != b->ctx || J.state->ctx != b->ctx) return fail(WD_E_STATE, "batch: job %u is of another device", i);
-    if (J.state->ellipse)
-      return fail(WD_E_ARG, "batch: job %u: its state takes the draw's ellipses, which only a single sort (wd_sort) writes", i);
-    if ((J.cloud->degree || !J.cloud->records) && !wd::is_std(J.view))
-      return fail(WD_E_ARG, "batch: job %u: splats with harmonics, or given as centres and covariances, are batched "
-                  "under the reverse-Z perspective only; sort them with wd_sort", i);
+    if ((J.cloud->degree || !J.cloud->records || J.state->ellipse) && !wd::is_std(J.view))
+      return fail(WD_E_ARG, "batch: job %u: splats with harmonics or given as centres and covariances, and states "
+                  "that take the draw's ellipses, are batched under the reverse-Z perspective only; sort them with "
+                  "wd_sort", i);
     if (J.cloud->n > J.state->capacity)
       return fail(WD_E_CAPACITY, "batch: job %u: %llu splats are more than its state holds, %llu", i,
                   (unsigned long long)J.cloud->n, (unsigned long long)J.state->capacity);
@@ -70,6 +69,8 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     B.sig = J.cloud->records ? nullptr : (const uint2 *)J.cloud->covs;
     B.sh = J.cloud->degree ? J.cloud->sh : nullptr;
     B.degree = J.cloud->degree;
+    B.ellipse = J.state->ellipse;
+    B.capacity = J.state->capacity;
     B.n = (uint32_t)J.cloud->n;
     wd_state *s = J.state;
     B.keys = s->keys;
@@ -122,7 +123,11 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   b->shape.swap(shape);
   if (b->api_jobs.data() != jobs) b->api_jobs.assign(jobs, jobs + n);
   b->sh_set.clear();
-  for (uint32_t i = 0; i < n; ++i) b->sh_set.emplace_back(jobs[i].cloud->sh, jobs[i].cloud->degree);
+  b->ellipse_set.clear();
+  for (uint32_t i = 0; i < n; ++i) {
+    b->sh_set.emplace_back(jobs[i].cloud->sh, jobs[i].cloud->degree);
+    b->ellipse_set.push_back(jobs[i].state->ellipse);
+  }
   b->n = n;
   b->total = total;
   return same_total && b->grid ? WD_OK : wd::batch_grid(ctx, total, &b->grid);
@@ -138,9 +143,9 @@ int wd_batch_run(wd_context *ctx, wd_batch *b) {
     if (c->sh != b->sh_set[i].first || c->degree != b->sh_set[i].second)
       return fail(WD_E_STATE, "batch: a job's harmonics changed since the jobs were set; set the jobs again");
   }
-  for (const wd_job &J : b->api_jobs)
-    if (J.state->ellipse)
-      return fail(WD_E_ARG, "batch: a job's state takes the draw's ellipses, which only a single sort (wd_sort) writes");
+  for (size_t i = 0; i < b->api_jobs.size(); ++i)
+    if (b->api_jobs[i].state->ellipse != b->ellipse_set[i])
+      return fail(WD_E_STATE, "batch: a job's ellipses were turned on or off since the jobs were set; set the jobs again");
   WD_HIP(hipSetDevice(ctx->device), "batch: set device");
   if (ctx->profiling) return wd::launch_batch(ctx, b);  // timed launches go straight to the stream
   if (!b->exec) {

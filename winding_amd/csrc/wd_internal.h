@@ -102,6 +102,10 @@ struct BatchJob {
   const WD_GLOBAL uint2 *sig;
   const WD_GLOBAL uint32_t *sh;     // harmonics (degree > 0), the cloud's sh words
   uint32_t degree;
+  // the draw's ellipse set-up, when the state takes it: two planes of capacity float4s, then the
+  // reach per alpha byte (null when off)
+  WD_GLOBAL float *ellipse;
+  uint64_t capacity;
   uint32_t cap;           // most survivors a bin may hold; more and the view runs the exact path
   uint32_t n, std_form;
   uint32_t tiles;    // projection chunks = scatter tiles, of kTile splats
@@ -219,6 +223,7 @@ struct wd_batch {
   uint32_t super_tiles = 1;                     // the scatter work item: tiles a workgroup (1 or 2)
   std::vector<wd_job> api_jobs;                 // the jobs as set, to rebuild the tables on a change
   std::vector<std::pair<const uint32_t *, uint32_t>> sh_set;  // each job's harmonics (words, degree) as set
+  std::vector<const float *> ellipse_set;                      // each job's ellipse buffer as set
   void *stage[2] = {nullptr, nullptr};          // pinned staging for the tables
   hipEvent_t staged[2] = {nullptr, nullptr};    // the copy out of stage[i] has finished
   bool staged_used[2] = {false, false};

@@ -372,10 +372,10 @@ class SplatPass {
   sort() {
     const marked = this._items.filter((i) => i.sort);
     for (const item of marked) item.state.last.set(item.state.next);
-    // A state that makes the draw's ellipses has a sort of its own, and so has a cloud with
-    // harmonics or of centres and covariances under a projection other than the reverse-Z
-    // perspective (the only one a batch shades and takes them under); the rest share one launch.
-    const own = (i) => i.state.ellipses || ((i.state.splats.degree > 0 || i.state.splats.sigma) && !reverseZ(i.state.uniform));
+    // A cloud with harmonics or of centres and covariances, or a state that makes the draw's
+    // ellipses, has a sort of its own under a projection other than the reverse-Z perspective
+    // (the only one a batch shades, takes them and makes ellipses under); the rest share one launch.
+    const own = (i) => (i.state.ellipses || i.state.splats.degree > 0 || i.state.splats.sigma) && !reverseZ(i.state.uniform);
     const plain = marked.filter((i) => !own(i));
     for (const item of marked) {
       if (own(item)) {
