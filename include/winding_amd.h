@@ -385,10 +385,17 @@ typedef struct wd_tuning {
 int wd_batch_tune(wd_context *ctx, wd_batch *batch, uint32_t reps, wd_tuning *out);
 /*
  * Sets the jobs (views may change between runs at no cost; other clouds or
- * states re-capture the graph on the next run).
+ * states re-capture the graph on the next run). A job whose cloud has
+ * harmonics or is given as centres and covariances, or whose state takes the
+ * draw's ellipses, is taken under the reverse-Z perspective only (WD_E_ARG
+ * otherwise; wd_sort takes any projection).
  */
 int wd_batch_set(wd_batch *batch, const wd_job *jobs, uint32_t n);
-/* Enqueues one run of every job; results land in each job's state. */
+/*
+ * Enqueues one run of every job; results land in each job's state. Refused
+ * (WD_E_STATE) when a job's harmonics or ellipses changed since the jobs were
+ * set: set them again.
+ */
 int wd_batch_run(wd_context *ctx, wd_batch *batch);
 /* Waits, then reports whether a bounded wait in the kernel ran out (a bug). */
 int wd_batch_stuck(wd_context *ctx, wd_batch *batch, uint32_t *stuck);
