@@ -105,20 +105,20 @@ __device__ __forceinline__ void item_project(ProjectSmem &sm, const BatchJob &J,
   }
 }
 
-__device__ __forceinline__ void item_scan(const BatchJob &J, uint32_t cg, uint32_t (*s_w)[kScanCols]) {
-  scan_cols(cg * kScanCols, J.tiles, J.tilehist, 512, J.prefix_lo, 256, 256, J.ctl->total, s_w);
+__device__ __forceinline__ void item_scan(const BatchJob &J, uint32_t cg, uint32_t (*s_w)[kBatchScanCols]) {
+  scan_cols<kBatchScanCols>(cg * kBatchScanCols, J.tiles, J.tilehist, 512, J.prefix_lo, 256, 256, J.ctl->total, s_w);
   if (cg == 0 && threadIdx.x == 0) {
     J.ctl->recomputed = J.ctl->recomputed_acc;
     J.ctl->recomputed_acc = 0;
   }
 }
 
-__device__ __forceinline__ void item_scan2(const BatchJob &J, uint32_t cg, uint32_t (*s_w)[kScanCols]) {
+__device__ __forceinline__ void item_scan2(const BatchJob &J, uint32_t cg, uint32_t (*s_w)[kBatchScanCols]) {
   const uint32_t T3 = (J.ctl->count + kTile - 1) / kTile;
-  scan_cols(cg * kScanCols, T3, J.tilehist3, 256, J.prefix_hi, 256, 256, J.ctl->total3, s_w);
+  scan_cols<kBatchScanCols>(cg * kBatchScanCols, T3, J.tilehist3, 256, J.prefix_hi, 256, 256, J.ctl->total3, s_w);
 }
 
-constexpr uint32_t kScanItems = 512 / kScanCols, kScan2Items = 256 / kScanCols;
+constexpr uint32_t kScanItems = 512 / kBatchScanCols, kScan2Items = 256 / kBatchScanCols;
 static_assert(kScanItems % kXcds == 0 && kScan2Items % kXcds == 0, "scan items a multiple of the XCDs");
 
 // gate: when given, the whole launch is skipped unless *gate is nonzero (the
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
     ProjectSmem p;
     HistSmem h;
     ScatterSmem<kSuper> s;
-    uint32_t part[4][kScanCols];
+    uint32_t part[4][kBatchScanCols];
   } sm;
   __shared__ uint32_t s_item;
   uint32_t *proj_done = reinterpret_cast<uint32_t *>(bc + 1);
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void kb_project(const BatchJob *__restrict__ j
 // (kScanItems and kScan2Items are multiples of 8, so workgroup b's XCD is
 // b mod 8 within every job too: the column groups are made XCD-contiguous.)
 __global__ __launch_bounds__(256) void kb_scan(const BatchJob *__restrict__ jobs) {
-  __shared__ uint32_t part[4][kScanCols];
+  __shared__ uint32_t part[4][kBatchScanCols];
   item_scan(jobs[blockIdx.x / kScanItems], xcd_item(blockIdx.x % kScanItems, kScanItems), part);
 }
 
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(256) void kb_hist(const BatchJob *__restrict__ jobs
 }
 
 __global__ __launch_bounds__(256) void kb_scan2(const BatchJob *__restrict__ jobs) {
-  __shared__ uint32_t part[4][kScanCols];
+  __shared__ uint32_t part[4][kBatchScanCols];
   item_scan2(jobs[blockIdx.x / kScan2Items], xcd_item(blockIdx.x % kScan2Items, kScan2Items), part);
 }
 

@@ -110,7 +110,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   WD_HIP(hipEventRecord(b->staged[f], ctx->stream), "batch: staging");
   b->staged_used[f] = true;
   b->flip ^= 1;
-  const uint32_t total = 2 * t[n] + 2 * u[n] + (512 / wd::kScanCols + 256 / wd::kScanCols) * n;
+  const uint32_t total = 2 * t[n] + 2 * u[n] + (512 / wd::kBatchScanCols + 256 / wd::kBatchScanCols) * n;
   const bool same_total = total == b->total;
   if (shape != b->shape || n != b->n || total != b->total) {
     if (b->exec) hipGraphExecDestroy(b->exec);

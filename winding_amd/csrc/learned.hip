@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_project_bins(const BatchJob *__restrict
 __global__ __launch_bounds__(256) void k_scan_bins(const BatchJob *__restrict__ jobs) {
   __shared__ uint32_t s_w[4][kScanCols];
   const BatchJob &J = jobs[blockIdx.y];
-  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, J.tiles, J.tilehist, 512, J.prefix_bin, 512, 512,
+  scan_cols<kScanCols>(xcd_item(blockIdx.x, gridDim.x) * kScanCols, J.tiles, J.tilehist, 512, J.prefix_bin, 512, 512,
             J.ctl->total, s_w);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     J.ctl->recomputed = J.ctl->recomputed_acc;

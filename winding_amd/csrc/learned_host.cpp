@@ -128,7 +128,7 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
     WD_HIP(hipMemcpyAsync(l->starts, st.data(), st.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream),
            "learned: item table");
     WD_HIP(hipStreamSynchronize(ctx->stream), "learned: item table");
-    const uint32_t items = n_views * (2 * tiles + 2 * supers + 512 / wd::kScanCols + 256 / wd::kScanCols);
+    const uint32_t items = n_views * (2 * tiles + 2 * supers + 512 / wd::kBatchScanCols + 256 / wd::kBatchScanCols);
     const int rc = wd::batch_grid(ctx, items, &l->grid);
     if (rc) return rc;
     l->last_views = n_views;

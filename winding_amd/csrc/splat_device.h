@@ -590,58 +590,66 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
 }
 
 // ---------------------------------------------------------------------------
-// Scans between the passes. A workgroup takes kScanCols adjacent columns
-// (digits) of a [T][stride] u16 histogram, one row block a thread: thread t
-// sums rows [t R, (t + 1) R), R = ceil(T / 256), with one 8-byte load a row,
-// the workgroup scans the four column sums (a wave scan and one barrier), and
-// the thread writes its rows' exclusive prefixes, prefix[r * pstride + c],
-// one 16-byte store a row, from the rows it still holds (up to kHeld rows a
-// thread, 1M tiles of 4096; beyond that it reads them again). Only columns
-// below pcols get prefixes; every column's sum goes to total[c]. No
-// workgroup waits on another: a scatter workgroup turns the totals into digit
-// bases itself (digit_bases), so no fence, look-back or last-block step is
-// needed.
+// Scans between the passes. A workgroup takes C adjacent columns (digits) of
+// a [T][stride] u16 histogram, one row block a thread: thread t sums rows
+// [t R, (t + 1) R), R = ceil(T / 256), with one load a row (8 bytes for C = 4,
+// 16 for C = 8), the workgroup scans the C column sums (a wave scan and one
+// barrier), and the thread writes its rows' exclusive prefixes,
+// prefix[r * pstride + c], in 16-byte stores, from the rows it still holds
+// (up to 16 rows a thread for C = 4, 8 for C = 8, so 32 or 64 registers;
+// beyond that it reads them again). Only columns below pcols get prefixes;
+// every column's sum goes to total[c]. No workgroup waits on another: a
+// scatter workgroup turns the totals into digit bases itself (digit_bases), so
+// no fence, look-back or last-block step is needed. A single sort's scans run
+// in one round of workgroups and take C = 4 (kScanCols); a batch of many jobs
+// takes C = 8 (kBatchScanCols), which halves its workgroups and its rounds.
+template <uint32_t C>
 __device__ __forceinline__ void scan_cols(uint32_t c0, uint32_t T, const uint16_t *__restrict__ hist, uint32_t stride,
                                           uint32_t *__restrict__ prefix, uint32_t pstride, uint32_t pcols,
-                                          uint32_t *__restrict__ total, uint32_t (*s_w)[kScanCols]) {
-  static_assert(kScanCols == 4, "one 8-byte load and one 16-byte store a row");
-  constexpr uint32_t kHeld = 16;
+                                          uint32_t *__restrict__ total, uint32_t (*s_w)[C]) {
+  static_assert(C == 4 || C == 8, "one 8- or 16-byte load a row");
+  using Row = typename std::conditional<C == 8, uint4, uint2>::type;
+  constexpr uint32_t kHeld = C == 8 ? 8 : 16;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t R = (T + 255u) >> 8;
   const uint32_t r0 = min(tid * R, T), r1 = min(r0 + R, T);
-  auto row = [&](uint32_t r) { return *reinterpret_cast<const uint2 *>(hist + (size_t)r * stride + c0); };
-  auto add = [](uint32_t (&acc)[4], uint2 x) {
-    acc[0] += x.x & 0xFFFFu;
-    acc[1] += x.x >> 16;
-    acc[2] += x.y & 0xFFFFu;
-    acc[3] += x.y >> 16;
+  auto row = [&](uint32_t r) { return *reinterpret_cast<const Row *>(hist + (size_t)r * stride + c0); };
+  auto add = [](uint32_t (&acc)[C], const Row &x) {
+    const uint32_t *v = reinterpret_cast<const uint32_t *>(&x);
+#pragma unroll
+    for (uint32_t i = 0; i < C / 2; ++i) {
+      acc[2 * i] += v[i] & 0xFFFFu;
+      acc[2 * i + 1] += v[i] >> 16;
+    }
   };
-  uint32_t sum[4] = {0, 0, 0, 0};
-  uint2 held[kHeld];
+  uint32_t sum[C];
+#pragma unroll
+  for (uint32_t c = 0; c < C; ++c) sum[c] = 0;
+  Row held[kHeld];
   const bool hold = R <= kHeld;
   if (hold && T > 0) {
     // every row is loaded before any is added: a thread's loads are in flight together
 #pragma unroll
     for (uint32_t k = 0; k < kHeld; ++k) {
       const bool ok = r0 + k < r1;
-      const uint2 x = row(ok ? r0 + k : 0u);  // row 0 exists when T > 0
-      held[k] = ok ? x : make_uint2(0u, 0u);
+      const Row x = row(ok ? r0 + k : 0u);  // row 0 exists when T > 0
+      held[k] = ok ? x : Row{};
     }
 #pragma unroll
     for (uint32_t k = 0; k < kHeld; ++k) add(sum, held[k]);
   } else {
     for (uint32_t r = r0; r < r1; ++r) add(sum, row(r));
   }
-  uint32_t inc[4];
+  uint32_t inc[C];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) inc[c] = wave_scan_incl(sum[c]);
+  for (uint32_t c = 0; c < C; ++c) inc[c] = wave_scan_incl(sum[c]);
   if (lane == 63u)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) s_w[w][c] = inc[c];
+    for (uint32_t c = 0; c < C; ++c) s_w[w][c] = inc[c];
   __syncthreads();
-  uint32_t run[4];
+  uint32_t run[C];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (uint32_t c = 0; c < C; ++c) {
     uint32_t off = 0, tot = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
@@ -650,11 +658,13 @@ __device__ __forceinline__ void scan_cols(uint32_t c0, uint32_t T, const uint16_
       tot += t;
     }
     run[c] = off + inc[c] - sum[c];
-    if (tid == (uint32_t)c) total[c0 + c] = tot;
+    if (tid == c) total[c0 + c] = tot;
   }
   if (!prefix || c0 >= pcols || T == 0) return;
   auto put = [&](uint32_t r) {
-    *reinterpret_cast<uint4 *>(prefix + (size_t)r * pstride + c0) = make_uint4(run[0], run[1], run[2], run[3]);
+    uint4 *q = reinterpret_cast<uint4 *>(prefix + (size_t)r * pstride + c0);
+#pragma unroll
+    for (uint32_t i = 0; i < C / 4; ++i) q[i] = make_uint4(run[4 * i], run[4 * i + 1], run[4 * i + 2], run[4 * i + 3]);
   };
   if (hold) {
 #pragma unroll

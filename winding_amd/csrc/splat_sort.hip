@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_scan(uint32_t T, const uint16_t *__rest
   __shared__ uint32_t s_w[4][kScanCols];
   // XCD-contiguous column groups: the 4-column pieces of one prefix row's cache
   // lines are written from one L2
-  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T, tilehist, 512, prefix_lo, 256, 256, ctl->total, s_w);
+  scan_cols<kScanCols>(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T, tilehist, 512, prefix_lo, 256, 256, ctl->total, s_w);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->recomputed = ctl->recomputed_acc;
     ctl->recomputed_acc = 0;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_scan2(const uint16_t *__restrict__ tile
                                                Ctl *__restrict__ ctl) {
   __shared__ uint32_t s_w[4][kScanCols];
   const uint32_t T3 = (ctl->count + kTile - 1) / kTile;
-  scan_cols(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T3, tilehist3, 256, prefix_hi, 256, 256, ctl->total3, s_w);
+  scan_cols<kScanCols>(xcd_item(blockIdx.x, gridDim.x) * kScanCols, T3, tilehist3, 256, prefix_hi, 256, 256, ctl->total3, s_w);
 }
 
 // ---------------------------------------------------------------------------

@@ -28,7 +28,8 @@ int hip_fail(hipError_t e, const char *what);
 constexpr int kThreads = 256;        // workgroup size of every kernel here
 constexpr int kTile = 4096;          // splats per scatter tile: 256 threads x 16
 constexpr int kKeysPerThread = 16;
-constexpr uint32_t kScanCols = 4;    // histogram columns a scan workgroup takes
+constexpr uint32_t kScanCols = 4;    // histogram columns a scan workgroup takes (a single sort)
+constexpr uint32_t kBatchScanCols = 8;  // the same in a batch's scans
 
 // Polls of a bounded wait before it gives up and flags the sort as failed.
 constexpr uint32_t kSpinLimit = 1u << 20;
