@@ -198,7 +198,7 @@ def bench_decode(args):
     return 0
 
 
-def bench_batch(args):
+def bench_batch(args, emit=True):
     """64 scenes of log-uniform size (about 32M splats), one random orbit view each: one hipGraph launch
     of the batch (schedule picked by the autotuner) against the same 64 sorts as single-view launches."""
     import winding_amd as wa
@@ -262,11 +262,13 @@ def bench_batch(args):
         cloud.destroy()
         state.destroy()
     ctx.close()
+    if not emit:
+        return line
     print(json.dumps(line), flush=True)
     return 0
 
 
-def bench_learned(args):
+def bench_learned(args, emit=True):
     """4,194,304 splats, a 1024-pose orbit. The predictor is trained first (untimed) on the exact sort's
     quantiles at 4096 random poses of the same family (a disjoint seed). Timed: the whole orbit sorted
     (a) plain, one wd_sort a pose; (b) batched, 8 poses a phased batch launch; (c) batched + learned, 8 poses
@@ -359,8 +361,32 @@ def bench_learned(args):
         s.destroy()
     cloud.destroy()
     ctx.close()
+    if not emit:
+        return line
     print(json.dumps(line), flush=True)
     return 0
+
+
+def second_half(args):
+    """The metric's second half, measured in the same run as the slab-8m line: batch-64 (the batched form
+    against per-scene sorts) and learned-4m (plain, batched exact and batched + learned over the orbit)."""
+    b = bench_batch(args, emit=False)
+    lr = bench_learned(args, emit=False)
+    bb, ll = b["batch"], lr["learned"]
+    batch = {"workload": "batch-64", "scenes": b["config"]["scenes"], "splats": b["config"]["splats"],
+             "survivors": b["config"]["survivors"], "replay_ms": b["ms_per_step"],
+             "per_scene_ms": b["ms_per_step"] / b["config"]["scenes"], "value": b["value"],
+             "plain_ms": bb["plain_ms"], "speedup_vs_plain": bb["speedup_vs_plain"],
+             "schedule": bb["schedule"], "tuning": {k: v for k, v in bb.items() if k not in ("plain_ms", "plain_value", "speedup_vs_plain")},
+             "path_frac_of_peak": b["path"]["frac_of_peak"]}
+    learned = {"workload": "learned-4m", "splats": lr["config"]["splats"], "poses": lr["config"]["poses"],
+               "ms_per_orbit": ll["ms_per_orbit"], "value": lr["value"], "plain_value": ll["plain_value"],
+               "batched_value": ll["batched_value"], "speedup_vs_plain": ll["speedup_vs_plain"],
+               "speedup_batched_exact_vs_plain": ll["speedup_batched_exact_vs_plain"],
+               "fallback_views": ll["fallback_views"], "train_poses": ll["train_poses"],
+               "train_loss": ll["train_loss"], "target_entropy": ll["target_entropy"],
+               "path_frac_of_peak": lr["path"]["frac_of_peak"]}
+    return batch, learned
 
 
 def main():
@@ -372,6 +398,8 @@ def main():
                     choices=["plain-64k", "plain-1m", "slab-8m", "batch-64", "learned-4m", "sh3-1m", "particles-8m", "decode-1m"])
     ap.add_argument("--cull", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-second-half", action="store_true",
+                    help="slab-8m only: leave out the batch-64 and learned-4m objects of the default line")
     ap.add_argument("--calibrate", action="store_true",
                     help="time a plain 512 MiB device copy instead (HBM yardstick for the traffic counters)")
     args = ap.parse_args()
@@ -510,6 +538,8 @@ def main():
     st.destroy()
     cloud.destroy()
     ctx.close()
+    if args.workload == "slab-8m" and not args.no_second_half:
+        line["batch"], line["learned"] = second_half(args)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -397,7 +397,12 @@ int wd_batch_set(wd_batch *batch, const wd_job *jobs, uint32_t n);
  * set: set them again.
  */
 int wd_batch_run(wd_context *ctx, wd_batch *batch);
-/* Waits, then reports whether a bounded wait in the kernel ran out (a bug). */
+/*
+ * Waits, then reports whether a bounded wait of the persistent kernel ran out
+ * in any run since the last call (a bug; each call reports a run once). Such a
+ * run also marks every one of its jobs' states, so wd_state_stats and
+ * wd_state_read fail on them (WD_E_STATE) until the next sort.
+ */
 int wd_batch_stuck(wd_context *ctx, wd_batch *batch, uint32_t *stuck);
 int wd_batch_destroy(wd_batch *batch);
 
@@ -492,6 +497,13 @@ int wd_debug_selftest(wd_context *ctx, const double *args, const double *expect,
  * and the yardstick the FETCH_SIZE / WRITE_SIZE counters are scaled by.
  */
 int wd_debug_copy(wd_context *ctx, uint64_t bytes, int reps, double *ms_mean);
+/*
+ * Sets the batch's live "a wait ran out" flag from the host, as a wait that
+ * ran out would, so that tests can check the reporting above without a hang:
+ * the next persistent run then gives up any wait longer than 1024 polls and
+ * marks its jobs' states as not valid.
+ */
+int wd_debug_batch_force_stuck(wd_context *ctx, wd_batch *batch);
 
 #ifdef __cplusplus
 }

@@ -171,6 +171,8 @@ int or_splat(const uint8_t *rec, const or_view *v, uint16_t *key, double cov[3],
   double f[3];
   double sg[6];
   or_sigma(rec, sg);
+  /* Sigma as put() stores it: each term rounded to f32 (a Float32Array, src/scene/splats.js:318,361-367) */
+  for (int k = 0; k < 6; k++) sg[k] = (double)(float)sg[k];
   or_footprint(v, view, sg, f);
   if (cov) memcpy(cov, f, sizeof f);
   double cull = v->cull;

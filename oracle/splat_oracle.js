@@ -104,7 +104,10 @@ function splat(dv, i, view, cov) {
   if (v2 >= 0 || c2 > w || Math.abs(c0) > C13 * w || Math.abs(c1) > C13 * w) return -1;
   const a = dv.getUint8(o + 27);
   if (a <= 1) return -1;
-  const f = footprint(view, v0, v1, v2, sigma(dv, o, SG), cov || FP);
+  // Sigma as put() stores it: each term rounded to f32 (a Float32Array, src/scene/splats.js:318,361-367)
+  const sg = sigma(dv, o, SG);
+  for (let k = 0; k < 6; k++) sg[k] = Math.fround(sg[k]);
+  const f = footprint(view, v0, v1, v2, sg, cov || FP);
   if (view.cull > 0) {
     const det = (f[0] + C03) * (f[2] + C03) - f[1] * f[1];
     const area = C2PI * Math.sqrt(det > 0 ? det : 0);

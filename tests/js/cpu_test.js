@@ -48,6 +48,92 @@ test('parseSplats refuses what is not .splat records, by what is wrong with it',
   assert.strictEqual(ok.degree, 0);
 });
 
+// A zip as a .sog packs it, written here from the format (local headers, data, central directory, end
+// record): each entry stored or deflated, an optional trailing comment, CRC-32 computed bit by bit.
+function zipOf(files, options) {
+  const zlib = require('zlib');
+  const opt = options || {};
+  const crc = (data) => {
+    let c = 0xffffffff;
+    for (const b of data) {
+      c ^= b;
+      for (let k = 0; k < 8; k++) c = (c >>> 1) ^ (0xedb88320 & -(c & 1));
+    }
+    return (c ^ 0xffffffff) >>> 0;
+  };
+  const parts = [], directory = [];
+  let offset = 0;
+  for (const [name, raw] of Object.entries(files)) {
+    const data = Buffer.from(raw);
+    const packed = opt.deflate ? zlib.deflateRawSync(data) : data;
+    const n = Buffer.from(name);
+    const local = Buffer.alloc(30 + n.length);
+    local.writeUInt32LE(0x04034b50, 0); local.writeUInt16LE(opt.deflate ? 8 : 0, 8); local.writeUInt32LE(crc(data), 14);
+    local.writeUInt32LE(packed.length, 18); local.writeUInt32LE(data.length, 22); local.writeUInt16LE(n.length, 26);
+    n.copy(local, 30);
+    const central = Buffer.alloc(46 + n.length);
+    central.writeUInt32LE(0x02014b50, 0); central.writeUInt16LE(opt.method !== undefined ? opt.method : (opt.deflate ? 8 : 0), 10);
+    central.writeUInt32LE((crc(data) ^ (opt.badCrc ? 1 : 0)) >>> 0, 16); central.writeUInt32LE(opt.zip64 ? 0xffffffff : packed.length, 20);
+    central.writeUInt32LE(data.length, 24); central.writeUInt16LE(n.length, 28); central.writeUInt32LE(offset, 42);
+    n.copy(central, 46);
+    parts.push(local, packed);
+    directory.push(central);
+    offset += local.length + packed.length;
+  }
+  const size = directory.reduce((a, d) => a + d.length, 0);
+  const comment = Buffer.from(opt.comment || '');
+  const end = Buffer.alloc(22);
+  end.writeUInt32LE(0x06054b50, 0); end.writeUInt16LE(directory.length, 8); end.writeUInt16LE(directory.length, 10);
+  end.writeUInt32LE(size, 12); end.writeUInt32LE(offset, 16); end.writeUInt16LE(comment.length, 20);
+  return new Uint8Array(Buffer.concat([...parts, ...directory, end, comment]));
+}
+
+function sogFiles(meta) {
+  const px = (rgba) => Buffer.from(JSON.stringify({ width: 1, height: 1, rgba }));
+  return {
+    'meta.json': JSON.stringify(meta), 'l.webp': px([0, 0, 0, 255]), 'u.webp': px([0, 0, 0, 255]),
+    'q.webp': px([128, 128, 128, 0]), 's.webp': px([100, 100, 100, 255]), 'c.webp': px([1, 2, 3, 4]),
+  };
+}
+
+const SOG_META = {
+  version: 2, count: 1, means: { mins: [-1, -1, -1], maxs: [1, 1, 1], files: ['l.webp', 'u.webp'] },
+  scales: { codebook: [0, 1], files: ['s.webp'] }, quats: { files: ['q.webp'] }, sh0: { codebook: [0, 1], files: ['c.webp'] },
+};
+
+async function zipChecks() {
+  const decodeImage = async (file) => JSON.parse(Buffer.from(file).toString());
+  const read = (bytes) => wd.readSplats(bytes, { decodeImage });
+  // stored and deflated entries read alike, behind a trailing comment too
+  for (const opt of [{}, { deflate: true }, { deflate: true, comment: 'made by a test' }]) {
+    const got = await read(zipOf(sogFiles(SOG_META), opt));
+    assert.strictEqual(got.format, 'sog');
+    assert.strictEqual(got.sog.count, 1);
+    assert.deepStrictEqual(Array.from(got.sog.images.sh0.rgba), [1, 2, 3, 4]);
+    assert.deepStrictEqual(Array.from(got.sog.mins), [-1, -1, -1]);
+  }
+  const good = zipOf(sogFiles(SOG_META));
+  await assert.rejects(read(good.subarray(0, good.length - 30)), /a zip with no end record: cut short\?/);
+  const damaged = Uint8Array.from(good);
+  const dir = new DataView(damaged.buffer).getUint32(damaged.length - 6, true);
+  damaged[dir] ^= 0xff;
+  await assert.rejects(read(damaged), /a zip whose directory is damaged/);
+  await assert.rejects(read(zipOf(sogFiles(SOG_META), { zip64: true })), /a zip64 \.sog is not read/);
+  await assert.rejects(read(zipOf(sogFiles(SOG_META), { method: 12 })), /meta\.json is compressed with method 12; stored and deflate are read/);
+  await assert.rejects(read(zipOf(sogFiles(SOG_META), { badCrc: true })), /meta\.json in the zip is damaged/);
+  const cut = zipOf({ 'meta.json': JSON.stringify(SOG_META) });
+  const cv = new DataView(cut.buffer);
+  cv.setUint32(cut.length - 22 - 46 - 9 + 20, 1 << 20, true);  // the directory claims more bytes than the file holds
+  await assert.rejects(read(cut), /the zip is cut short in meta\.json/);
+  const files = sogFiles(SOG_META);
+  delete files['q.webp'];
+  await assert.rejects(read(zipOf(files)), /the \.sog has no q\.webp/);
+  await assert.rejects(read(zipOf(sogFiles(Object.assign({}, SOG_META, { version: undefined })))), /a \.sog of version 1; version 2 is read/);
+  await assert.rejects(read(zipOf(sogFiles(Object.assign({}, SOG_META, { count: 0 })))), /a \.sog of 0 splats/);
+  await assert.rejects(read(zipOf(sogFiles(Object.assign({}, SOG_META, { count: 2 })))), /l\.webp has 1 pixels, for 2 splats/);
+  await assert.rejects(wd.readSplats(good), /a \.sog needs decodeImage/);
+}
+
 test('lookAt puts the target on -Z; reverse-Z maps near to 1', () => {
   const view = wd.mat4LookAt(new Float32Array(16), [0, 0, 5], [0, 0, 0], [0, 1, 0]);
   assert.ok(Math.abs(view[14] + 5) < 1e-6);
@@ -174,4 +260,11 @@ test('refusals: a negative cull, unloaded splats, a cloud wholly behind is skipp
   assert.throws(() => scene.addSplats({ splats: {} }), /what new Splats returned/);
 });
 
-console.log(passed + ' checks passed');
+zipChecks().then(() => {
+  passed++;
+  console.log('  ok  a .sog\'s zip: stored, deflated, a comment; every refusal by its message');
+  console.log(passed + ' checks passed');
+}).catch((error) => {
+  console.error(error);
+  process.exit(1);
+});

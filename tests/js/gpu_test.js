@@ -326,17 +326,29 @@ test('SplatPass batches a cloud with harmonics with a plain one, as their own so
 });
 
 /** A zip of the given files, stored (method 0), enough for readSplats' unzip. */
+// CRC-32 bit by bit (no table), for the zips these tests write
+function crc32(data) {
+  let c = 0xffffffff;
+  for (const b of data) {
+    c ^= b;
+    for (let k = 0; k < 8; k++) c = (c >>> 1) ^ (0xedb88320 & -(c & 1));
+  }
+  return (c ^ 0xffffffff) >>> 0;
+}
+
 function storedZip(files) {
   const parts = [], directory = [];
   let offset = 0;
   for (const [name, data] of Object.entries(files)) {
     const n = Buffer.from(name);
+    const crc = crc32(data);
     const local = Buffer.alloc(30 + n.length);
-    local.writeUInt32LE(0x04034b50, 0); local.writeUInt32LE(data.length, 18); local.writeUInt32LE(data.length, 22);
-    local.writeUInt16LE(n.length, 26); n.copy(local, 30);
+    local.writeUInt32LE(0x04034b50, 0); local.writeUInt32LE(crc, 14); local.writeUInt32LE(data.length, 18);
+    local.writeUInt32LE(data.length, 22); local.writeUInt16LE(n.length, 26); n.copy(local, 30);
     const central = Buffer.alloc(46 + n.length);
-    central.writeUInt32LE(0x02014b50, 0); central.writeUInt32LE(data.length, 20); central.writeUInt32LE(data.length, 24);
-    central.writeUInt16LE(n.length, 28); central.writeUInt32LE(offset, 42); n.copy(central, 46);
+    central.writeUInt32LE(0x02014b50, 0); central.writeUInt32LE(crc, 16); central.writeUInt32LE(data.length, 20);
+    central.writeUInt32LE(data.length, 24); central.writeUInt16LE(n.length, 28); central.writeUInt32LE(offset, 42);
+    n.copy(central, 46);
     parts.push(local, Buffer.from(data));
     directory.push(central);
     offset += local.length + data.length;

@@ -92,3 +92,81 @@ def test_batch_64_scenes_32m(ctx):
         for cloud, _, state in jobs:
             cloud.destroy()
             state.destroy()
+
+
+def _small_jobs(ctx, sizes, seed):
+    st = wl.Stream(seed)
+    jobs, data = [], []
+    for j, n in enumerate(sizes):
+        recs = wa.synth_records(0, 500 + j, n)
+        view = wl.view_for(recs, wl.orbit_pose(st), 1280, 720) or wl.view_for(recs, (0.0, 0.0, 10.0), 1280, 720)
+        jobs.append((wa.Cloud(ctx, recs), view, wa.State(ctx, n)))
+        data.append(recs)
+    return jobs, data
+
+
+def test_batch_refuses_a_destroyed_state_even_at_a_reused_address(ctx):
+    """A job whose state was destroyed is refused at run time, also when a new state was given the
+    same address (the batch compares each object's serial, not its address)."""
+    jobs, data = _small_jobs(ctx, [5000, 70001], 21)
+    batch = wa.Batch(ctx, 2)
+    made = []
+    try:
+        batch.set(jobs)
+        batch.run()
+        ctx.sync()
+        cloud, view, state = jobs[1]
+        old = state.ptr.value
+        state.destroy()
+        for _ in range(64):  # the allocator most often hands the address out again at once
+            made.append(wa.State(ctx, len(data[1])))
+            if made[-1].ptr.value == old:
+                break
+        print("address reused: %s" % (made[-1].ptr.value == old), flush=True)
+        with pytest.raises(wa.WindingError, match="destroyed; set the jobs again") as e:
+            batch.run()
+        assert e.value.code == -5
+        jobs[1] = (cloud, view, made[-1])
+        batch.set(jobs)  # set again: runs, and the results are the new state's
+        batch.run()
+        ctx.sync()
+        for (c, v, s), recs in zip(jobs, data):
+            check_job(s, recs, v, "after set again n=%d" % len(recs))
+    finally:
+        batch.destroy()
+        for s in made[:-1]:
+            s.destroy()
+        for cloud, _, state in jobs:
+            cloud.destroy()
+            state.destroy()
+
+
+def test_a_stuck_persistent_run_fails_its_states_once(ctx):
+    """The persistent kernel's "a wait ran out" flag, raised from the host (no hang): the run's states
+    then fail to read, the batch reports it once, and the next run is whole again."""
+    jobs, data = _small_jobs(ctx, [4096, 100000, 300000], 33)
+    batch = wa.Batch(ctx, 3)
+    try:
+        batch.schedule(wa.WD_SCHEDULE_PERSISTENT, 1)
+        batch.set(jobs)
+        batch.run()
+        ctx.sync()
+        assert batch.stuck() == 0
+        batch.force_stuck()
+        batch.run()
+        ctx.sync()
+        for _, _, state in jobs:
+            with pytest.raises(wa.WindingError, match="bounded wait ran out"):
+                state.stats()
+        assert batch.stuck() == 1
+        assert batch.stuck() == 0  # reported once
+        batch.run()
+        ctx.sync()
+        assert batch.stuck() == 0
+        for (c, v, s), recs in zip(jobs, data):
+            check_job(s, recs, v, "after a stuck run n=%d" % len(recs))
+    finally:
+        batch.destroy()
+        for cloud, _, state in jobs:
+            cloud.destroy()
+            state.destroy()

@@ -1,10 +1,12 @@
 """The learned sort: the predictor trained on the exact sort's own quantiles, then up to 8 views of one
-cloud binned by its depth CDF. Every view's order must equal the exact sort's, bit for bit, whether the
-fast path held or the check sent the view back to the exact path; and the trained predictor must keep
-views on the fast path where even key-width bins (the method without it) do not."""
+cloud binned by its depth CDF. Every view's keys, survivors and order must equal the oracle's (and the
+exact sort's), bit for bit, whether the fast path held or the check sent the view back to the exact path;
+and the trained predictor must keep views on the fast path where even key-width bins (the method without
+it) do not."""
 import numpy as np
 import pytest
 
+import oracle
 import winding_amd as wa
 from winding_amd import workloads as wl
 
@@ -14,10 +16,14 @@ N = 262144
 W, H = 1280, 720
 
 
+RECS = {}
+
+
 @pytest.fixture(scope="module")
 def scene(ctx):
     recs = wa.synth_records(0, 7, N)
     cloud = wa.Cloud(ctx, recs)
+    RECS[id(cloud)] = recs
     exact = wa.State(ctx, N)
     states = [wa.State(ctx, N) for _ in range(8)]
     yield cloud, exact, states
@@ -48,11 +54,29 @@ def orbit_views(cloud, first, count, step=37):
     return [wl.view_at(cloud, eyes[(first + k * step) % 1024], W, H) for k in range(count)]
 
 
-def check_views(ctx, cloud, exact, states, views, label, n=N):
-    """Each view's learned result against wd_sort's; returns how many views fell back."""
+def check_oracle(st, recs, view, order, cov, label):
+    """One view's learned result against the oracle itself: survivors, keys and order bit for bit,
+    cov2d within 1e-5 of the FP64 footprint."""
+    o_order, o_keys, o_keep, o_cov = oracle.run(recs, view)
+    keys, keep = st.read_keys(len(recs))
+    assert np.array_equal(keep, o_keep), "%s: survivors differ from the oracle's" % label
+    assert np.array_equal(keys[keep], o_keys[o_keep]), "%s: keys differ from the oracle's" % label
+    assert np.array_equal(order, o_order), "%s: order differs from the oracle's" % label
+    if keep.any():
+        g, w = cov[keep, :3].astype(np.float64), o_cov[keep]
+        err = float(np.max(np.abs(g - w).max(axis=1) / np.maximum(np.abs(w[:, 0]), np.abs(w[:, 2]))))
+        assert err <= 1e-5, "%s: cov2d error %g against the oracle" % (label, err)
+
+
+def check_views(ctx, cloud, exact, states, views, label, n=N, recs=None):
+    """Each view's learned result against the oracle (when the records are known) and against wd_sort's;
+    returns how many views fell back."""
     fell = 0
+    recs = RECS.get(id(cloud)) if recs is None else recs
     for k, (view, st) in enumerate(zip(views, states)):
         order, cov, stats = st.read(n_cov=n)
+        if recs is not None:
+            check_oracle(st, recs, view, order, cov, "%s view %d" % (label, k))
         exact.sort(cloud, view)
         e_order, e_cov, e_stats = exact.read(n_cov=n)
         assert stats.count == e_stats.count, "%s view %d: V %d vs %d" % (label, k, stats.count, e_stats.count)
@@ -146,7 +170,7 @@ def test_ragged_sizes_and_a_view_with_no_survivors(ctx, n):
         away.model_view[14] += 1000.0  # everything behind the camera
         views.append(away)
         learned.run(flat, cloud, views, states)
-        assert check_views(ctx, cloud, exact, states, views, "n=%d" % n, n=n) == 0
+        assert check_views(ctx, cloud, exact, states, views, "n=%d" % n, n=n, recs=recs) == 0
         assert states[2].stats().count == 0
     finally:
         learned.destroy()

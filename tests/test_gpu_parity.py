@@ -247,3 +247,92 @@ def test_refusals(ctx):
     st2.destroy()
     with pytest.raises(wa.WindingError, match="more than one sort holds"):
         wa.State(ctx, wa.WD_MAX_SORT + 1)
+
+
+def _box_bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("n", [1, 63, 4097, 70001])
+def test_cloud_box_equals_put_box(ctx, n):
+    """The box wd_cloud_create reports (k_bounds on the GPU; depthMin and depthScale come from it in the JS
+    API) equals put()'s min/max (src/scene/splats.js:343-348) bit for bit: negative coordinates, ragged
+    sizes, one splat, and zeros of both signs at the ends (put's strict compares keep the first zero)."""
+    recs = wa.synth_records(0, 40 + n, n)
+    pos = recs.view(np.float32).reshape(-1, 8)
+    pos[:, :3] -= 7.5  # mostly negative
+    rng = np.random.default_rng(n)
+    if n > 1:
+        # a zero of each sign at the minimum of x and the maximum of y, the first one positive or negative
+        pos[:, 0] = np.abs(pos[:, 0]) + 1.0
+        pos[:, 1] = -np.abs(pos[:, 1]) - 1.0
+        i, j = sorted(rng.choice(n, 2, replace=False))
+        first = rng.choice([0.0, -0.0])
+        pos[i, 0], pos[j, 0] = first, -first
+        pos[i, 1], pos[j, 1] = -first, first
+    cases = [recs]
+    if n == 4097:
+        cases.append(wa.workloads.plain("slab-8m")[0])
+    for r in cases:
+        cloud = wa.Cloud(ctx, r)
+        try:
+            bad, mn, mx = oracle.bounds(r)
+            assert bad < 0 or bad >= len(r)
+            print("n=%d box %s %s" % (len(r), mn, mx), flush=True)
+            assert np.array_equal(_box_bits(cloud.min), _box_bits(mn)), (cloud.min, mn)
+            assert np.array_equal(_box_bits(cloud.max), _box_bits(mx)), (cloud.max, mx)
+        finally:
+            cloud.destroy()
+
+
+def test_off_axis_projection_takes_the_clamp_branch(ctx):
+    """An off-axis (jittered) projection, p[8] and p[9] nonzero: many survivors lie past footprint()'s
+    1.3 margin (src/render/splats.js:111-114) and take its clamp; keys, order and cov2d still match."""
+    recs = wa.synth_records(0, 77, 65536)
+    pos = recs.view(np.float32).reshape(-1, 8)
+    pos[:, 0] *= 2.5  # wide in x: many splats beyond the margin
+    v = oracle.look_at((0.0, 0.0, 10.0))
+    p = oracle.perspective(math.pi / 3, 16 / 9, 0.1).copy()
+    p[8], p[9] = 0.6, -0.4
+    _, mn, mx = oracle.bounds(recs)
+    view = wa.View.from_buffer_copy(bytes(oracle.prepare(v, p, 1280.0, 720.0, mn, mx, 0.5)))
+    mv = np.array(view.model_view, np.float64).reshape(4, 4).T
+    vp = pos[:, :3].astype(np.float64) @ mv[:3, :3].T + mv[:3, 3]
+    _, _, keep, _ = oracle.run(recs, view)
+    clamped = keep & (np.abs(vp[:, 0] / -vp[:, 2]) > (1.3 / p[0]))
+    print("survivors %d, clamped %d" % (keep.sum(), clamped.sum()), flush=True)
+    assert clamped.sum() > 1000
+    assert_parity(ctx, recs, view, "off-axis")
+
+
+@pytest.mark.parametrize("which", ["edges_4096", "plain-64k"])
+def test_record_form_equals_splats_data_form_on_gpu(ctx, which):
+    """One capture as .splat records and as the reference's Splats data (centres, Sigma rounded to f32 as
+    put() stores it): the same survivors, keys and order on the GPU, and both equal the oracle's."""
+    if which == "plain-64k":
+        recs, view, _ = wa.workloads.plain("plain-64k")
+    else:
+        with np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", which + ".npz"),
+                     allow_pickle=False) as z:
+            recs = z["records"]
+            view = wa.View.from_buffer_copy(z["view"].astype(np.float32).tobytes())
+    centers = np.zeros((len(recs), 4), np.float32)
+    centers[:, :3] = recs.view(np.float32).reshape(-1, 8)[:, :3]
+    centers.view(np.uint32)[:, 3] = recs[:, 24:28].copy().view(np.uint32).ravel()
+    covs = np.array([oracle.sigma(r) for r in recs], np.float64).astype(np.float32)
+    o_order, o_keys, o_keep, _ = oracle.run(recs, view)
+    a = wa.Cloud(ctx, recs)
+    b = wa.Cloud.from_splats(ctx, centers, covs)
+    st = wa.State(ctx, len(recs))
+    try:
+        for cloud in (a, b):
+            st.sort(cloud, view)
+            order, _, stats = st.read()
+            keys, keep = st.read_keys(len(recs))
+            assert np.array_equal(keep, o_keep)
+            assert np.array_equal(keys[keep], o_keys[o_keep])
+            assert np.array_equal(order, o_order)
+    finally:
+        st.destroy()
+        a.destroy()
+        b.destroy()

@@ -78,7 +78,9 @@ def test_gpu_gradients_match_oracle(ctx):
         errs = tensor_errors(g, w_g)
         print("loss gpu %.6f oracle %.6f; per-tensor max rel err %s" % (loss, w_loss, errs), flush=True)
         assert abs(loss - w_loss) <= 1e-3 * abs(w_loss)
-        assert max(errs.values()) <= 2e-2, errs
+        # per tensor, the largest difference relative to the tensor's largest entry: fp32 sums in another
+        # order than the oracle's float64 ones, with the same bf16 rounding points
+        assert max(errs.values()) <= 1e-4, errs
         cos = float(g.astype(np.float64) @ w_g / (np.linalg.norm(g) * np.linalg.norm(w_g)))
         assert cos > 0.9999
         # the same inputs give the same gradients, bit for bit (the loss is summed by atomics: to rounding)

@@ -237,3 +237,89 @@ def test_prepare_depth_range():
     assert out.depth_min == 9 and out.depth_scale == np.float32(65535 / 2)
     behind = oracle.look_at((0, 0, 10), (0, 0, 20))
     assert oracle.prepare(behind, p, 1280, 720, [-1, -1, -1], [1, 1, 1], 0.5) is None
+
+
+def test_bounds_keep_the_first_of_equal_values():
+    # put() updates min/max with strict compares in splat order (src/scene/splats.js:343-348), so of
+    # equal values the first stays: a zero end takes the sign of the first zero met.
+    def box(xs, ys):
+        recs = cloud(*[record([x, y, 1.0], [1, 1, 1], [0, 0, 0, 0], [255, 128, 128, 128]) for x, y in zip(xs, ys)])
+        bad, mn, mx = oracle.bounds(recs)
+        assert bad == -1
+        return mn.view(np.uint32), mx.view(np.uint32)
+    mn, mx = box([0.0, -0.0, 3.0], [-3.0, -0.0, 0.0])
+    assert mn[0] == 0x00000000 and mx[1] == 0x80000000
+    mn, mx = box([-0.0, 0.0, 3.0], [-3.0, 0.0, -0.0])
+    assert mn[0] == 0x80000000 and mx[1] == 0x00000000
+    mn, mx = box([1.0, -0.0, 0.0, -2.0], [5.0, 0.0, -0.0, 1.0])
+    assert np.float32(mn.view(np.float32)[0]) == -2.0 and mx.view(np.float32)[1] == 5.0
+
+
+def test_cov_clamp_branch_is_the_margin_ray():
+    # footprint() clamps x/d to +-1.3/p00 (src/render/splats.js:111-114): a splat past that margin, still
+    # inside the frustum because the projection is off-axis (p[8] != 0, as a jittered projection has), has
+    # the footprint of the same splat moved onto the margin ray at the same depth. An unclamped Jacobian
+    # (central differences of the pixel map) gives a different one, so the check tells the clamp apart.
+    v = oracle.look_at((0, 0, 10))
+    p = oracle.perspective(math.pi / 3, 16 / 9, 0.1).copy()
+    p[8] = 0.6
+    p00 = float(p[0])
+    lx = float(np.float64(np.float32(1.3)) / p00)
+    d = 11.0
+    recs, moved = [], []
+    for sgn, ratio, yv in [(1, 1.6, 0.4), (1, 1.85, -0.7), (-1, 1.35, 0.0)]:
+        if sgn < 0:
+            ratio = 0.6  # on the left the frustum ends first at (0.6 - 1.3) / p00: stay inside the margin
+        x = sgn * ratio / p00 * d
+        q = [160, 100, 140, 120]
+        recs.append(record([x, yv, 10.0 - d], [0.05, 0.02, 0.08], [200, 100, 50, 220], q))
+        xm = np.float32(sgn * min(ratio / p00, lx) * d)
+        moved.append(record([xm, yv, 10.0 - d], [0.05, 0.02, 0.08], [200, 100, 50, 220], q))
+    recs = cloud(*recs, *moved)
+    view = oracle.prepare(v, p, 1280, 720, [-30, -3, -3], [30, 3, 3], 0.0)
+    k, keep, cov, _ = oracle.keys(recs, view)
+    assert keep.all()
+    n = len(recs) // 2
+    for i in range(n):
+        np.testing.assert_allclose(cov[i], cov[n + i], rtol=2e-6, atol=2e-6 * max(cov[i][0], cov[i][2]))
+    # the unclamped Jacobian at the first splat is far from the clamped footprint
+    mv = np.array(view.model_view, np.float64).reshape(4, 4).T
+    P = np.array(view.projection, np.float64)
+    fx, fy = P[0] * 1280 * 0.5, P[5] * 720 * 0.5
+    pos = recs[0].view(np.float32)[:3].astype(np.float64)
+
+    def pix(x):
+        vv = mv[:3, :3] @ x + mv[:3, 3]
+        return np.array([fx * vv[0] / -vv[2], fy * vv[1] / -vv[2]])
+    J = np.stack([(pix(pos + 1e-5 * e) - pix(pos - 1e-5 * e)) / 2e-5 for e in np.eye(3)], axis=1)
+    s = np.array([float(np.float32(t)) for t in oracle.sigma(recs[0])])
+    S = np.array([[s[0], s[1], s[2]], [s[1], s[3], s[4]], [s[2], s[4], s[5]]])
+    free = J @ S @ J.T
+    assert abs(free[0, 0] - cov[0][0]) > 0.05 * cov[0][0]
+
+
+def test_record_form_equals_splats_data_form():
+    # The same capture given as .splat records and as the reference's Splats data (centres with the colour
+    # bits, Sigma as put() stores it in f32) is culled, keyed and sorted alike: the record form reads Sigma
+    # at put()'s f32 rounding. Checked on every committed fixture, including edges_4096, whose survivors sit
+    # on the cull threshold, at cull 0.5 and 2.0.
+    import glob
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for path in sorted(glob.glob(os.path.join(here, "*.npz"))):
+        with np.load(path, allow_pickle=False) as z:
+            recs = z["records"]
+            vf = z["view"].astype(np.float32)
+        centers = np.zeros((len(recs), 4), np.float32)
+        centers[:, :3] = recs.view(np.float32).reshape(-1, 8)[:, :3]
+        centers.view(np.uint32)[:, 3] = recs[:, 24:28].copy().view(np.uint32).ravel()
+        covs = np.array([oracle.sigma(r) for r in recs], np.float64).astype(np.float32)
+        for cull in (float(vf[36]), 2.0):
+            vf2 = vf.copy()
+            vf2[36] = cull
+            view = wa.View.from_buffer_copy(vf2.tobytes())
+            o1, k1, keep1, c1 = oracle.run(recs, view)
+            o2, k2, keep2, c2 = oracle.run_sigma(centers, covs, view)
+            assert np.array_equal(keep1, keep2), os.path.basename(path)
+            assert np.array_equal(k1[keep1], k2[keep2]) and np.array_equal(o1, o2)
+            assert np.array_equal(c1[keep1], c2[keep2])

@@ -37,7 +37,7 @@ EXPORTS = [
     "wd_mlp_create", "wd_mlp_destroy", "wd_mlp_set", "wd_mlp_get", "wd_mlp_save", "wd_mlp_load",
     "wd_mlp_grad", "wd_mlp_train", "wd_mlp_predict", "wd_pose_features",
     "wd_learned_create", "wd_learned_run", "wd_learned_destroy", "wd_state_quantiles",
-    "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest", "wd_debug_copy",
+    "wd_synth_records", "wd_debug_canonical", "wd_debug_selftest", "wd_debug_copy", "wd_debug_batch_force_stuck",
 ]
 
 
@@ -172,6 +172,7 @@ def lib():
             "wd_debug_canonical": (I, [P, P, C.POINTER(View), P]),
             "wd_debug_selftest": (I, [P, P, P, U32, P]),
             "wd_debug_copy": (I, [P, U64, I, C.POINTER(C.c_double)]),
+            "wd_debug_batch_force_stuck": (I, [P, P]),
         }
         missing = sorted(set(EXPORTS) - set(sig))
         if missing:
@@ -603,6 +604,10 @@ class Batch:
         s = C.c_uint32()
         check(lib().wd_batch_stuck(self.ctx.ptr, self.ptr, C.byref(s)))
         return s.value
+
+    def force_stuck(self):
+        """Raises the live "a wait ran out" flag from the host (a test of the reporting)."""
+        check(lib().wd_debug_batch_force_stuck(self.ctx.ptr, self.ptr))
 
     def destroy(self):
         if self.ptr:

@@ -28,9 +28,10 @@ ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
+    "-Wall", "-Werror", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-D__HIP_PLATFORM_AMD__"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Werror", "-Wno-unknown-pragmas",
+              "-D__HIP_PLATFORM_AMD__"]
 
 HIP_SOURCES = ["splat_sort.hip", "batch.hip", "mlp.hip", "learned.hip", "decode.hip"]
 HOST_SOURCES = ["api.cpp", "synth.cpp", "batch_host.cpp", "mlp_host.cpp", "learned_host.cpp", "decode_host.cpp"]
@@ -76,7 +77,7 @@ def build_addon():
         raise RuntimeError("build failed: Node's N-API headers are not at %s" % NODE_INCLUDE)
     src = os.path.join(PKG, "napi", "addon.c")
     tmp = ADDON + ".tmp"
-    _run(["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-I", NODE_INCLUDE, "-I", os.path.join(ROOT, "include"),
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-Werror", "-I", NODE_INCLUDE, "-I", os.path.join(ROOT, "include"),
           "-DNODE_GYP_MODULE_NAME=winding_amd", src, "-o", tmp, "-L", PKG, "-lwinding_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, ADDON)
     return ADDON
@@ -86,7 +87,7 @@ def build_oracle():
     os.makedirs(os.path.dirname(ORACLE_LIB), exist_ok=True)
     srcs = [os.path.join(ROOT, "oracle", f) for f in ("splat_oracle.c", "decode_oracle.c")]
     tmp = ORACLE_LIB + ".tmp"
-    _run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall"] + srcs + ["-o", tmp, "-lm"])
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Werror"] + srcs + ["-o", tmp, "-lm"])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB
 

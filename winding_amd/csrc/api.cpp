@@ -26,6 +26,22 @@ int fail(int code, const char *fmt, ...) {
 
 int hip_fail(hipError_t e, const char *what) { return fail(WD_E_HIP, "%s: %s", what, hipGetErrorString(e)); }
 
+void box_from_ordered(const uint32_t lo[3], const uint32_t hi[3], const uint32_t zero_index[3],
+                      float (*value_at)(const void *, uint32_t, int), const void *arg, float mn[3], float mx[3]) {
+  for (int k = 0; k < 3; ++k) {
+    uint32_t a = lo[k], b = hi[k];
+    a = (a & 0x80000000u) ? (a & 0x7FFFFFFFu) : ~a;
+    b = (b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b;
+    std::memcpy(&mn[k], &a, 4);
+    std::memcpy(&mx[k], &b, 4);
+    if ((mn[k] == 0.0f || mx[k] == 0.0f) && zero_index[k] != 0xFFFFFFFFu) {
+      const float z = value_at(arg, zero_index[k], k);
+      if (mn[k] == 0.0f) mn[k] = z;
+      if (mx[k] == 0.0f) mx[k] = z;
+    }
+  }
+}
+
 int launch_selftest_kernel(wd_context *ctx, const double *args, const double *expect, uint32_t count,
                            uint32_t *mismatch3);
 
@@ -110,15 +126,15 @@ int wd_open(int device, wd_context **out) {
 
 int wd_close(wd_context *ctx) {
   if (!ctx || ctx->closed) return fail(WD_E_STATE, "close: no context, or closed already");
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   for (auto &p : ctx->pending) {
-    hipEventDestroy(p.a);
-    hipEventDestroy(p.b);
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
   }
-  for (auto e : ctx->spare) hipEventDestroy(e);
-  if (ctx->particle_edges) hipFree(ctx->particle_edges);
-  hipStreamDestroy(ctx->stream);
+  for (auto e : ctx->spare) (void)hipEventDestroy(e);
+  if (ctx->particle_edges) (void)hipFree(ctx->particle_edges);
+  (void)hipStreamDestroy(ctx->stream);
   ctx->closed = true;
   delete ctx;
   return WD_OK;
@@ -158,15 +174,15 @@ static int cloud_from(wd_context *ctx, const void *src, uint64_t bytes, bool dev
   c->ctx = ctx;
   c->n = n;
   uint32_t *scratch = nullptr;
-  uint32_t res[8];
+  uint32_t res[12];
   hipError_t e = hipMalloc(&c->records, bytes);
   if (e == hipSuccess)
     e = hipMemcpyAsync(c->records, src, bytes, device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                        ctx->stream);
-  if (e == hipSuccess) e = hipMalloc(&scratch, 8 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&scratch, 12 * sizeof(uint32_t));
   if (e != hipSuccess) {
-    if (scratch) hipFree(scratch);
-    hipFree(c->records);
+    if (scratch) (void)hipFree(scratch);
+    (void)hipFree(c->records);
     delete c;
     return wd::hip_fail(e, "splats: upload");
   }
@@ -176,23 +192,24 @@ static int cloud_from(wd_context *ctx, const void *src, uint64_t bytes, bool dev
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = wd::hip_fail(e, "splats: check");
   }
-  hipFree(scratch);
+  (void)hipFree(scratch);
   if (rc == WD_OK && res[6] != 0xFFFFFFFFu) rc = fail(WD_E_DATA, "splats: splat %u has a position that is not a number", res[6]);
   if (rc != WD_OK) {
-    hipFree(c->records);
+    (void)hipFree(c->records);
     delete c;
     return rc;
   }
-  for (int k = 0; k < 3; ++k) {
-    uint32_t lo = res[k], hi = res[3 + k];
-    lo = (lo & 0x80000000u) ? (lo & 0x7FFFFFFFu) : ~lo;
-    hi = (hi & 0x80000000u) ? (hi & 0x7FFFFFFFu) : ~hi;
-    std::memcpy(&c->min[k], &lo, 4);
-    std::memcpy(&c->max[k], &hi, 4);
-  }
+  // a zero end of the box takes its sign from the first zero, read from the records as uploaded
+  auto at = [](const void *recs, uint32_t i, int k) -> float {
+    float v = 0.f;
+    if (hipMemcpy(&v, (const uint8_t *)recs + (size_t)i * WD_RECORD_BYTES + 4 * k, 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return 0.f;
+    return v;
+  };
+  wd::box_from_ordered(res, res + 3, res + 7, at, c->records, c->min, c->max);
   if (min_out) std::memcpy(min_out, c->min, sizeof c->min);
   if (max_out) std::memcpy(max_out, c->max, sizeof c->max);
-  ctx->live.insert(c);
+  ctx->adopt(c);
   *out = c;
   return WD_OK;
 }
@@ -234,8 +251,8 @@ int wd_cloud_create_splats(wd_context *ctx, const float *centers, const float *c
   if (e == hipSuccess) e = hipMemcpy(c->centers, centers, count * 16, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->covs, covariances, count * 24, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    if (c->centers) hipFree(c->centers);
-    if (c->covs) hipFree(c->covs);
+    if (c->centers) (void)hipFree(c->centers);
+    if (c->covs) (void)hipFree(c->covs);
     delete c;
     return wd::hip_fail(e, "splats: upload");
   }
@@ -243,7 +260,7 @@ int wd_cloud_create_splats(wd_context *ctx, const float *centers, const float *c
   std::memcpy(c->max, mx, sizeof mx);
   if (min_out) std::memcpy(min_out, mn, sizeof mn);
   if (max_out) std::memcpy(max_out, mx, sizeof mx);
-  ctx->live.insert(c);
+  ctx->adopt(c);
   *out = c;
   return WD_OK;
 }
@@ -270,7 +287,7 @@ int wd_cloud_sh(wd_context *ctx, wd_cloud *cloud, const uint32_t *words, uint64_
                 (unsigned long long)cloud->n, degree);
   WD_HIP(hipSetDevice(ctx->device), "splats: set device");
   WD_HIP(hipStreamSynchronize(ctx->stream), "splats: harmonics");  // no sort in flight reads the old ones
-  if (cloud->sh) hipFree(cloud->sh);
+  if (cloud->sh) (void)hipFree(cloud->sh);
   cloud->sh = nullptr;
   cloud->degree = 0;
   if (!degree) return WD_OK;
@@ -291,12 +308,12 @@ int wd_cloud_records(const wd_cloud *cloud, const void **device_records) {
 
 int wd_cloud_destroy(wd_cloud *cloud) {
   if (!cloud || cloud->magic != 0x53504c54) return fail(WD_E_STATE, "splats: destroyed twice");
-  hipSetDevice(cloud->ctx->device);
-  hipStreamSynchronize(cloud->ctx->stream);
-  if (cloud->records) hipFree(cloud->records);
-  if (cloud->centers) hipFree(cloud->centers);
-  if (cloud->covs) hipFree(cloud->covs);
-  if (cloud->sh) hipFree(cloud->sh);
+  (void)hipSetDevice(cloud->ctx->device);
+  (void)hipStreamSynchronize(cloud->ctx->stream);
+  if (cloud->records) (void)hipFree(cloud->records);
+  if (cloud->centers) (void)hipFree(cloud->centers);
+  if (cloud->covs) (void)hipFree(cloud->covs);
+  if (cloud->sh) (void)hipFree(cloud->sh);
   cloud->ctx->live.erase(cloud);
   cloud->magic = 0;
   delete cloud;
@@ -369,19 +386,19 @@ int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
     wd_state_destroy(s);
     return wd::hip_fail(e, "state: allocation");
   }
-  ctx->live.insert(s);
+  ctx->adopt(s);
   *out = s;
   return WD_OK;
 }
 
 int wd_state_destroy(wd_state *s) {
   if (!s || s->magic != 0x53544154) return fail(WD_E_STATE, "state: destroyed twice");
-  hipSetDevice(s->ctx->device);
-  hipStreamSynchronize(s->ctx->stream);
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
   for (void *p : {(void *)s->ctl, (void *)s->keys, (void *)s->mask, (void *)s->packed, (void *)s->order,
                   (void *)s->cov, (void *)s->tilehist, (void *)s->tilehist3, (void *)s->prefix_lo,
                   (void *)s->prefix_hi, (void *)s->lut, (void *)s->prefix_bin, (void *)s->ellipse})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   s->ctx->live.erase(s);
   s->magic = 0;
   delete s;
@@ -411,11 +428,11 @@ int wd_pool_create(wd_context *ctx, const void *particles, uint64_t count, wd_po
   hipError_t e = hipMalloc(&p->particles, count * WD_PARTICLE_BYTES);
   if (e == hipSuccess) e = hipMemcpy(p->particles, particles, count * WD_PARTICLE_BYTES, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    if (p->particles) hipFree(p->particles);
+    if (p->particles) (void)hipFree(p->particles);
     delete p;
     return wd::hip_fail(e, "particles: upload");
   }
-  ctx->live.insert(p);
+  ctx->adopt(p);
   *out = p;
   return WD_OK;
 }
@@ -428,9 +445,9 @@ int wd_pool_device(const wd_pool *pool, void **device_particles) {
 
 int wd_pool_destroy(wd_pool *pool) {
   if (!pool || pool->magic != 0x504f4f4c) return fail(WD_E_STATE, "particles: destroyed twice");
-  hipSetDevice(pool->ctx->device);
-  hipStreamSynchronize(pool->ctx->stream);
-  hipFree(pool->particles);
+  (void)hipSetDevice(pool->ctx->device);
+  (void)hipStreamSynchronize(pool->ctx->stream);
+  (void)hipFree(pool->particles);
   pool->ctx->live.erase(pool);
   pool->magic = 0;
   delete pool;
@@ -479,7 +496,7 @@ int wd_state_ellipses(wd_context *ctx, wd_state *s, int on) {
            "ellipses: reach table");
   } else if (!on && s->ellipse) {
     WD_HIP(hipStreamSynchronize(ctx->stream), "ellipses: sync");  // no sort in flight writes them
-    hipFree(s->ellipse);
+    (void)hipFree(s->ellipse);
     s->ellipse = nullptr;
   }
   return WD_OK;
@@ -535,7 +552,7 @@ int wd_state_stats(wd_context *ctx, wd_state *s, wd_stats *stats) {
   WD_HIP(hipStreamSynchronize(ctx->stream), "stats: sync");
   if (head.stuck) {
     const uint32_t zero = 0;
-    hipMemcpy(&s->ctl->stuck, &zero, sizeof zero, hipMemcpyHostToDevice);
+    (void)hipMemcpy(&s->ctl->stuck, &zero, sizeof zero, hipMemcpyHostToDevice);
     return fail(WD_E_STATE, "sort: a bounded wait ran out; the result is not valid");
   }
   stats->count = head.count;
@@ -621,7 +638,7 @@ int wd_time_sorts(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, w
     ms[r] = t;
   }
   for (hipEvent_t x : ev)
-    if (x) hipEventDestroy(x);
+    if (x) (void)hipEventDestroy(x);
   if (rc) return rc;
   if (e != hipSuccess) return wd::hip_fail(e, "time sorts");
   return WD_OK;
@@ -641,7 +658,7 @@ int wd_debug_canonical(wd_context *ctx, const wd_cloud *cloud, const wd_view *vi
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = wd::hip_fail(e, "canonical debug: read");
   }
-  hipFree(dev);
+  (void)hipFree(dev);
   return rc;
 }
 
@@ -665,10 +682,10 @@ int wd_debug_copy(wd_context *ctx, uint64_t bytes, int reps, double *ms_mean) {
     if (e != hipSuccess && rc == WD_OK) rc = wd::hip_fail(e, "copy: timing");
     *ms_mean = ms / reps;
   }
-  if (a) hipEventDestroy(a);
-  if (b) hipEventDestroy(b);
-  hipFree(src);
-  hipFree(dst);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  (void)hipFree(src);
+  (void)hipFree(dst);
   return rc;
 }
 
@@ -689,9 +706,9 @@ int wd_debug_selftest(wd_context *ctx, const double *args, const double *expect,
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = wd::hip_fail(e, "selftest: read");
   }
-  hipFree(da);
-  hipFree(de);
-  hipFree(dm);
+  (void)hipFree(da);
+  (void)hipFree(de);
+  (void)hipFree(dm);
   return rc;
 }
 

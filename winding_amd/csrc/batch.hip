@@ -189,11 +189,19 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
                                   J.prefix_hi, J.ctl->total + 256, J.ctl, J.order);
     }
   }
-  // The last workgroup out resets the queue for the next run.
+  // The last workgroup out resets the queue for the next run. A run whose wait
+  // ran out marks every job's sort as not valid (the state's stats then fail on
+  // the host), keeps the fact for wd_batch_stuck, and clears the live flag, so
+  // the next run waits in full again.
   if (tid == 0) {
     __threadfence();
     if (atomicAdd(&bc->exited, 1u) == gridDim.x - 1) {
       __threadfence();
+      if (__hip_atomic_load(&bc->stuck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        for (uint32_t j = 0; j < nj; ++j) jobs[j].ctl->stuck = 1u;
+        bc->stuck_seen = 1u;
+        bc->stuck = 0u;
+      }
       for (uint32_t c = 0; c < (uint32_t)kBatchCounters; ++c)
         for (uint32_t j = 0; j < nj; ++j) proj_done[c * max_jobs + j] = 0;
       bc->ticket = 0;

@@ -46,7 +46,7 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   if (!jobs || n == 0 || n > b->max_jobs) return fail(WD_E_ARG, "batch: %u jobs, this batch holds 1 to %u", n, b->max_jobs);
   std::vector<wd::BatchJob> hj(n);
   std::vector<uint32_t> t(n + 1, 0), u(n + 1, 0);  // item starts: tiles, super-tiles
-  std::vector<const void *> shape;
+  std::vector<std::pair<const void *, uint64_t>> shape;
   for (uint32_t i = 0; i < n; ++i) {
     const wd_job &J = jobs[i];
     if (!J.cloud || !b->ctx->live.count(J.cloud)) return fail(WD_E_STATE, "batch: job %u: its splats were unloaded", i);
@@ -88,8 +88,8 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     const uint32_t S = b->schedule == WD_SCHEDULE_PERSISTENT ? (uint32_t)wd::kSuper : b->super_tiles;
     u[i + 1] = u[i] + (B.tiles + S - 1) / S;
     s->last_n = B.n;
-    shape.push_back(J.cloud);
-    shape.push_back(J.state);
+    shape.emplace_back(J.cloud, b->ctx->serial(J.cloud));
+    shape.emplace_back(J.state, b->ctx->serial(J.state));
   }
   std::vector<uint32_t> starts(2 * (n + 1), 0);
   std::copy(t.begin(), t.end(), starts.begin());
@@ -113,8 +113,8 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   const uint32_t total = 2 * t[n] + 2 * u[n] + (512 / wd::kBatchScanCols + 256 / wd::kBatchScanCols) * n;
   const bool same_total = total == b->total;
   if (shape != b->shape || n != b->n || total != b->total) {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
   }
@@ -136,8 +136,10 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
 int wd_batch_run(wd_context *ctx, wd_batch *b) {
   if (!ctx || !b || b->magic != 0x42415443 || b->ctx != ctx) return fail(WD_E_STATE, "batch: destroyed or of another device");
   if (b->n == 0) return fail(WD_E_STATE, "batch: no jobs set");
-  for (const void *p : b->shape)
-    if (!ctx->live.count(p)) return fail(WD_E_STATE, "batch: a job's splats or sort state was destroyed; set the jobs again");
+  // by serial, not address: an object made after a destroy may sit at the same address
+  for (const auto &p : b->shape)
+    if (ctx->serial(p.first) != p.second)
+      return fail(WD_E_STATE, "batch: a job's splats or sort state was destroyed; set the jobs again");
   for (size_t i = 0; i < b->api_jobs.size(); ++i) {
     const wd_cloud *c = b->api_jobs[i].cloud;
     if (c->sh != b->sh_set[i].first || c->degree != b->sh_set[i].second)
@@ -154,7 +156,7 @@ int wd_batch_run(wd_context *ctx, wd_batch *b) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       return rc;
     }
     WD_HIP(e, "batch: end capture");
@@ -171,8 +173,8 @@ int wd_batch_schedule(wd_batch *b, int schedule, uint32_t super_tiles) {
     return fail(WD_E_ARG, "batch: schedule %d is neither phased (0) nor persistent (1)", schedule);
   if (super_tiles != 1 && super_tiles != 2) return fail(WD_E_ARG, "batch: super-tiles of %u; 1 or 2", super_tiles);
   if (schedule == b->schedule && super_tiles == b->super_tiles) return WD_OK;
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   b->schedule = schedule;
@@ -206,8 +208,8 @@ int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, wd_tuning *out) {
     if (!rc && e == hipSuccess) e = hipEventElapsedTime(&t, a, z);
     ms[c] = (double)t / reps;
   }
-  hipEventDestroy(a);
-  if (z) hipEventDestroy(z);
+  (void)hipEventDestroy(a);
+  if (z) (void)hipEventDestroy(z);
   if (rc) return rc;
   if (e != hipSuccess) return wd::hip_fail(e, "batch: tune");
   int best = 0;
@@ -227,22 +229,38 @@ int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, wd_tuning *out) {
 int wd_batch_stuck(wd_context *ctx, wd_batch *b, uint32_t *stuck) {
   if (!ctx || !b || b->magic != 0x42415443 || !stuck) return fail(WD_E_ARG, "batch: null argument or destroyed");
   WD_HIP(hipStreamSynchronize(ctx->stream), "batch: sync");
-  WD_HIP(hipMemcpy(stuck, &b->ctl->stuck, sizeof(uint32_t), hipMemcpyDeviceToHost), "batch: read");
+  uint32_t flags[2] = {0, 0};  // stuck, stuck_seen
+  WD_HIP(hipMemcpy(flags, &b->ctl->stuck, sizeof flags, hipMemcpyDeviceToHost), "batch: read");
+  *stuck = (flags[0] | flags[1]) ? 1u : 0u;
+  if (flags[1]) {
+    // reported once: the next call tells of runs after this one
+    const uint32_t zero = 0;
+    WD_HIP(hipMemcpy(&b->ctl->stuck_seen, &zero, sizeof zero, hipMemcpyHostToDevice), "batch: clear");
+  }
+  return WD_OK;
+}
+
+int wd_debug_batch_force_stuck(wd_context *ctx, wd_batch *b) {
+  if (!ctx || !b || b->magic != 0x42415443 || b->ctx != ctx) return fail(WD_E_ARG, "batch: null argument or destroyed");
+  WD_HIP(hipSetDevice(ctx->device), "batch: set device");
+  static const uint32_t one = 1;
+  WD_HIP(hipMemcpyAsync(&b->ctl->stuck, &one, sizeof one, hipMemcpyHostToDevice, ctx->stream), "batch: force");
+  WD_HIP(hipStreamSynchronize(ctx->stream), "batch: force");
   return WD_OK;
 }
 
 int wd_batch_destroy(wd_batch *b) {
   if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed twice");
-  hipSetDevice(b->ctx->device);
-  hipStreamSynchronize(b->ctx->stream);
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
-  if (b->jobs) hipFree(b->jobs);
-  if (b->starts) hipFree(b->starts);
-  if (b->ctl) hipFree(b->ctl);
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->jobs) (void)hipFree(b->jobs);
+  if (b->starts) (void)hipFree(b->starts);
+  if (b->ctl) (void)hipFree(b->ctl);
   for (int i = 0; i < 2; ++i) {
-    if (b->stage[i]) hipHostFree(b->stage[i]);
-    if (b->staged[i]) hipEventDestroy(b->staged[i]);
+    if (b->stage[i]) (void)hipHostFree(b->stage[i]);
+    if (b->staged[i]) (void)hipEventDestroy(b->staged[i]);
   }
   b->magic = 0;
   delete b;

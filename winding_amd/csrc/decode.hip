@@ -173,17 +173,18 @@ template <int DEG>
 __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t cnt, bool valid, const Decoded<DEG> &d,
                                      uint32_t *lds) {
   constexpr uint32_t W = DEG ? sh_words(DEG) : 0;
-  __shared__ uint32_t box[6];
+  __shared__ uint32_t box[9];  // ordered min xyz, ordered max xyz, first zero per axis
   const uint32_t t = threadIdx.x;
-  if (t < 6) box[t] = t < 3 ? 0xFFFFFFFFu : 0u;
+  if (t < 9) box[t] = (t < 3 || t >= 6) ? 0xFFFFFFFFu : 0u;
   if (valid && !d.finite) atomicMin(o.bad, i0 + t);
   const bool in_box = valid && d.finite;
   const float p[3] = {d.centre.x, d.centre.y, d.centre.z};
-  uint32_t lo[3], hi[3];
+  uint32_t lo[3], hi[3], zi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     lo[k] = wave_min(in_box ? ordered(p[k]) : 0xFFFFFFFFu);
     hi[k] = wave_max(in_box ? ordered(p[k]) : 0u);
+    zi[k] = wave_min(in_box && p[k] == 0.0f ? i0 + t : 0xFFFFFFFFu);
   }
   uint32_t *centres = lds, *covs = lds + 256 * 4, *sh = lds + 256 * 10;
   if (valid) {
@@ -200,6 +201,7 @@ __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t c
     for (int k = 0; k < 3; ++k) {
       atomicMin(&box[k], lo[k]);
       atomicMax(&box[3 + k], hi[k]);
+      atomicMin(&box[6 + k], zi[k]);
     }
   }
   uint32_t *gc = reinterpret_cast<uint32_t *>(o.centers) + (size_t)i0 * 4;
@@ -213,6 +215,7 @@ __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t c
   __syncthreads();
   if (t < 3 && box[t] != 0xFFFFFFFFu) atomicMin(&o.box[t], box[t]);
   if (t >= 3 && t < 6 && box[t] != 0u) atomicMax(&o.box[t], box[t]);
+  if (t >= 6 && t < 9 && box[t] != 0xFFFFFFFFu) atomicMin(&o.zero[t - 6], box[t]);
 }
 
 constexpr uint32_t emit_words(int DEG) { return 256u * (10u + (DEG ? sh_words(DEG) : 0u)); }

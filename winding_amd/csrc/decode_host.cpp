@@ -68,25 +68,27 @@ int decode_into(wd_context *ctx, uint64_t n, uint32_t degree, const std::functio
   const uint32_t W = degree ? wd::sh_words(degree) : 0;
   uint32_t *scratch = nullptr;
   auto undo = [&]() {
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     for (void *p : {(void *)c->centers, (void *)c->covs, (void *)c->sh, (void *)scratch})
-      if (p) hipFree(p);
+      if (p) (void)hipFree(p);
     delete c;
   };
   hipError_t e = hipMalloc(&c->centers, n * 16);
   if (e == hipSuccess) e = hipMalloc(&c->covs, n * 24);
   if (e == hipSuccess && W) e = hipMalloc(&c->sh, (n * W + 1) * 4);
   if (e == hipSuccess && W) e = hipMemsetAsync(c->sh, 0, (n * W + 1) * 4, ctx->stream);
-  if (e == hipSuccess) e = hipMalloc(&scratch, 8 * sizeof(uint32_t));
-  const uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u, 0u};
+  if (e == hipSuccess) e = hipMalloc(&scratch, 12 * sizeof(uint32_t));
+  // bad index, ordered min xyz, ordered max xyz, first zero per axis
+  const uint32_t init[12] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u,
+                             0u,          0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u};
   if (e == hipSuccess) e = hipMemcpyAsync(scratch, init, sizeof init, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) {
     undo();
     return wd::hip_fail(e, "splats: decode buffers");
   }
-  wd::DecodeOut o{c->centers, c->covs, c->sh, W, scratch, scratch + 1};
+  wd::DecodeOut o{c->centers, c->covs, c->sh, W, scratch, scratch + 1, scratch + 7};
   int rc = launch(o);
-  uint32_t res[8];
+  uint32_t res[12];
   if (rc == WD_OK) {
     e = hipMemcpyAsync(res, scratch, sizeof res, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -98,18 +100,19 @@ int decode_into(wd_context *ctx, uint64_t n, uint32_t degree, const std::functio
     undo();
     return rc;
   }
-  hipFree(scratch);
-  for (int k = 0; k < 3; ++k) {
-    uint32_t lo = res[1 + k], hi = res[4 + k];
-    lo = (lo & 0x80000000u) ? (lo & 0x7FFFFFFFu) : ~lo;
-    hi = (hi & 0x80000000u) ? (hi & 0x7FFFFFFFu) : ~hi;
-    std::memcpy(&c->min[k], &lo, 4);
-    std::memcpy(&c->max[k], &hi, 4);
-  }
+  (void)hipFree(scratch);
+  scratch = nullptr;
+  // a zero end of the box takes its sign from the first zero (put()'s strict compares), read back
+  auto at = [](const void *centers, uint32_t i, int k) -> float {
+    float v = 0.f;
+    if (hipMemcpy(&v, (const float *)centers + (size_t)i * 4 + k, 4, hipMemcpyDeviceToHost) != hipSuccess) return 0.f;
+    return v;
+  };
+  wd::box_from_ordered(res + 1, res + 4, res + 7, at, c->centers, c->min, c->max);
   c->degree = degree;
   if (min_out) std::memcpy(min_out, c->min, sizeof c->min);
   if (max_out) std::memcpy(max_out, c->max, sizeof c->max);
-  ctx->live.insert(c);
+  ctx->adopt(c);
   *out = c;
   return WD_OK;
 }
@@ -119,7 +122,7 @@ int decode_into(wd_context *ctx, uint64_t n, uint32_t degree, const std::functio
 struct DeviceCopy {
   void *p = nullptr;
   ~DeviceCopy() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
   hipError_t from(const void *src, size_t bytes) {
     hipError_t e = hipMalloc(&p, bytes + 16);
@@ -169,7 +172,8 @@ int wd_decode_ply(wd_context *ctx, const void *bytes, uint64_t len, wd_cloud **o
   if (!ctx || !bytes || !out) return fail(WD_E_ARG, "splats: null argument");
   *out = nullptr;
   const uint8_t *data = (const uint8_t *)bytes;
-  // The header is text, up to and including "end_header" and its newline (src/scene/splats.js:400-410).
+  // The header ends at the first "end_header" line; the vertex bytes start after its line break
+  // (src/scene/splats.js:400-410).
   const char *marker = "end_header";
   const int64_t scan = (int64_t)std::min<uint64_t>(len, 65536) - 10;
   int64_t end = -1;
@@ -240,7 +244,7 @@ int wd_decode_ply(wd_context *ctx, const void *bytes, uint64_t len, wd_cloud **o
   if ((uint64_t)end + n * stride > len)
     return fail(WD_E_DATA, "splats: the .ply is cut short: %llu vertices need %llu bytes, it has %lld",
                 (unsigned long long)n, (unsigned long long)(n * stride), (long long)len - (long long)end);
-  // The higher harmonics: as many f_rest_ as there are, which says the degree.
+  // Count the consecutive f_rest_k properties: 9, 24 or 45 of them give degree 1, 2 or 3.
   uint32_t rest = 0;
   while (rest < 46 && at.count("f_rest_" + std::to_string(rest))) ++rest;
   int degree = -1;

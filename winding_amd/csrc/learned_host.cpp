@@ -184,15 +184,15 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
 
 int wd_learned_destroy(wd_learned *l) {
   if (!alive(l)) return fail(WD_E_STATE, "learned: destroyed twice");
-  hipSetDevice(l->ctx->device);
-  hipStreamSynchronize(l->ctx->stream);
+  (void)hipSetDevice(l->ctx->device);
+  (void)hipStreamSynchronize(l->ctx->stream);
   for (int i = 0; i < 2; ++i) {
-    if (l->host_jobs[i]) hipHostFree(l->host_jobs[i]);
-    if (l->host_poses[i]) hipHostFree(l->host_poses[i]);
-    if (l->staged[i]) hipEventDestroy(l->staged[i]);
+    if (l->host_jobs[i]) (void)hipHostFree(l->host_jobs[i]);
+    if (l->host_poses[i]) (void)hipHostFree(l->host_poses[i]);
+    if (l->staged[i]) (void)hipEventDestroy(l->staged[i]);
   }
   for (void *p : {(void *)l->jobs, (void *)l->poses, (void *)l->gate, (void *)l->starts, (void *)l->bctl})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   l->magic = 0;
   delete l;
   return WD_OK;
@@ -206,7 +206,7 @@ int wd_state_quantiles(wd_context *ctx, wd_state *s, uint32_t edges[257]) {
   int rc = wd::launch_quantiles(ctx, s, dev);
   hipError_t e = rc ? hipSuccess : hipStreamSynchronize(ctx->stream);
   if (!rc && e == hipSuccess) e = hipMemcpy(edges, dev, 257 * sizeof(uint32_t), hipMemcpyDeviceToHost);
-  hipFree(dev);
+  (void)hipFree(dev);
   if (rc) return rc;
   if (e != hipSuccess) return wd::hip_fail(e, "quantiles: read");
   return WD_OK;

@@ -51,7 +51,7 @@ int ensure_rows(wd_mlp *m, uint32_t rows) {
   if (rows <= m->cap) return WD_OK;
   void *ptrs[] = {m->X, m->H1, m->H2, m->Z, m->P, m->T, m->dZ, m->dH1, m->dH2, m->H1b, m->H2b};
   for (void *p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   const size_t r = rows;
   hipError_t e = hipMalloc(&m->X, r * kIn * 4);
   if (e == hipSuccess) e = hipMalloc(&m->H1, r * kH * 4);
@@ -181,12 +181,12 @@ int wd_mlp_create(wd_context *ctx, uint64_t seed, wd_mlp **out) {
 
 int wd_mlp_destroy(wd_mlp *m) {
   if (!alive(m)) return fail(WD_E_STATE, "mlp: destroyed twice");
-  hipSetDevice(m->ctx->device);
-  hipStreamSynchronize(m->ctx->stream);
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
   void *ptrs[] = {m->w, m->wb, m->g, m->m, m->v, m->loss, m->X, m->H1, m->H2, m->Z, m->P,
                   m->T, m->dZ, m->dH1, m->dH2, m->H1b, m->H2b};
   for (void *p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   m->magic = 0;
   delete m;
   return WD_OK;

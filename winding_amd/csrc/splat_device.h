@@ -159,10 +159,11 @@ __device__ __forceinline__ double canonical_value(const ViewK &v, uint4 a, uint4
   const double m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
   const double m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
   const double m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
-  const double S[3][3] = {
-      {m00 * m00 + m01 * m01 + m02 * m02, m00 * m10 + m01 * m11 + m02 * m12, m00 * m20 + m01 * m21 + m02 * m22},
-      {m00 * m10 + m01 * m11 + m02 * m12, m10 * m10 + m11 * m11 + m12 * m12, m10 * m20 + m11 * m21 + m12 * m22},
-      {m00 * m20 + m01 * m21 + m02 * m22, m10 * m20 + m11 * m21 + m12 * m22, m20 * m20 + m21 * m21 + m22 * m22}};
+  // Sigma as put() stores it: each term rounded to f32 (src/scene/splats.js:318,361-367).
+  const double sxx = (float)(m00 * m00 + m01 * m01 + m02 * m02), sxy = (float)(m00 * m10 + m01 * m11 + m02 * m12);
+  const double sxz = (float)(m00 * m20 + m01 * m21 + m02 * m22), syy = (float)(m10 * m10 + m11 * m11 + m12 * m12);
+  const double syz = (float)(m10 * m20 + m11 * m21 + m12 * m22), szz = (float)(m20 * m20 + m21 * m21 + m22 * m22);
+  const double S[3][3] = {{sxx, sxy, sxz}, {sxy, syy, syz}, {sxz, syz, szz}};
   double xx, xy, yy;
   footprint64(v, S, vx, vy, vz, xx, xy, yy);
   return cull_value64(xx, xy, yy, b.z >> 24);
@@ -252,7 +253,9 @@ __device__ __forceinline__ Projected project_front(const ViewK &v, double x, dou
   o.cz = (float)cz;
   o.cw = (float)cw;
   o.xx = o.xy = o.yy = 0.f;
-  o.keep = !((vz >= 0) | (cz > cw) | (fabs(cx) > kC13 * cw) | (fabs(cy) > kC13 * cw) | (abyte <= 1u));
+  // bitwise on purpose: every test is evaluated, no branch
+  o.keep = !((int)(vz >= 0) | (int)(cz > cw) | (int)(fabs(cx) > kC13 * cw) | (int)(fabs(cy) > kC13 * cw) |
+             (int)(abyte <= 1u));
   return o;
 }
 
@@ -317,7 +320,9 @@ __device__ __forceinline__ Projected project_with(const ViewK &v, uint4 a, uint4
     if (v.cull > 0.f) {
       // The f32 value and its bound against the FP64 one. With n_r the row
       // norms of E (|W_r| + l_r |W_2|, l_r >= |x/d|), S2 = |s|^2 and u = 2^-24,
-      // |xx - xx64| <= 52 u (j0 n0)^2 S2, likewise yy and xy; 64 is used.
+      // |xx - xx64| <= 52 u (j0 n0)^2 S2, likewise yy and xy; Sigma's rounding
+      // to f32 (|dSigma_ij| <= u |Sigma_ij| <= u S2) adds at most u (j0 n0)^2
+      // S2 more; 64 is used.
       const float a0 = j0 * (l0 * bw2 + bn0w0), a1 = j1 * (l1 * bw2 + bn1w1);
       const float c = (64.f * kU) * (sx * sx + sy * sy + sz * sz);
       const float exx = c * a0 * a0, eyy = c * a1 * a1, exy = c * a0 * a1;
@@ -700,7 +705,7 @@ struct HistSmem {
 __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32_t *__restrict__ packed,
                                           const Ctl *__restrict__ ctl, uint16_t *__restrict__ tilehist3) {
   constexpr int KPT = kKeysPerThread;
-  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t tid = threadIdx.x, w = tid >> 6;
   const uint32_t limit = ctl->count, tile0 = t * (uint32_t)kTile;
   if (tile0 >= limit) return;
   // Counting needs no order: thread tid takes 16 consecutive elements, in four 16-byte loads.

@@ -178,9 +178,12 @@ __device__ __forceinline__ uint32_t ordered_bits(float f) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-__global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, uint32_t n, uint32_t *out8) {
-  // out8: [0..2] ordered min, [3..5] ordered max, [6] first bad index
+__global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, uint32_t n, uint32_t *out12) {
+  // out12: [0..2] ordered min, [3..5] ordered max, [6] first bad index, [7..9] first index whose
+  // coordinate k is a zero of either sign (put()'s strict compares keep the first of equal values, so
+  // a zero minimum or maximum takes that zero's sign)
   uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0, 0, 0}, bad = 0xFFFFFFFFu;
+  uint32_t zero[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
     const uint4 a = rec[2 * (size_t)i];
     const float p[3] = {f32_of(a.x), f32_of(a.y), f32_of(a.z)};
@@ -196,14 +199,16 @@ __global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, u
       const uint32_t o = ordered_bits(p[k]);
       lo[k] = min(lo[k], o);
       hi[k] = max(hi[k], o);
+      if (p[k] == 0.0f) zero[k] = min(zero[k], i);
     }
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    atomicMin(&out8[k], lo[k]);
-    atomicMax(&out8[3 + k], hi[k]);
+    atomicMin(&out12[k], lo[k]);
+    atomicMax(&out12[3 + k], hi[k]);
+    if (zero[k] != 0xFFFFFFFFu) atomicMin(&out12[7 + k], zero[k]);
   }
-  if (bad != 0xFFFFFFFFu) atomicMin(&out8[6], bad);
+  if (bad != 0xFFFFFFFFu) atomicMin(&out12[6], bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -335,13 +340,14 @@ const char *const kKernelNames[kKernelCount] = {
     "k_bounds", "k_project", "k_scan", "k_scatter_lo", "k_hist",        "k_scan2",
     "k_scatter_hi", "k_batch", "k_mlp", "k_bin_table", "k_bin_scatter", "k_bin_sort", "k_particle_keys", "k_decode"};
 
-int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8) {
-  static const uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0, 0xFFFFFFFFu, 0};
-  WD_HIP(hipMemcpyAsync(scratch8, init, sizeof init, hipMemcpyHostToDevice, ctx->stream), "cloud check: upload");
+int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch12) {
+  static const uint32_t init[12] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0,           0, 0,
+                                    0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0};
+  WD_HIP(hipMemcpyAsync(scratch12, init, sizeof init, hipMemcpyHostToDevice, ctx->stream), "cloud check: upload");
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->cus * 4);
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKBounds, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)records, (uint32_t)n, scratch8);
+  hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)records, (uint32_t)n, scratch12);
   WD_HIP(hipGetLastError(), "cloud check: launch");
   return ctx->prof_end(kKBounds, ev);
 }

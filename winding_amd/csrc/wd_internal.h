@@ -6,7 +6,7 @@
 
 #include <cstdint>
 #include <string>
-#include <unordered_set>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -150,7 +150,8 @@ struct BatchCtl {
   uint32_t ticket;
   uint32_t exited;
   uint32_t runs;
-  uint32_t stuck;
+  uint32_t stuck;       // a bounded wait of this run ran out; the run's waits give up early from then on
+  uint32_t stuck_seen;  // some run since the host last asked was stuck (cleared by wd_batch_stuck)
   // followed by five counters a job: projection chunks done, scan column
   // groups done, low-byte super-tiles done, histogram tiles done, second-scan
   // column groups done
@@ -166,9 +167,17 @@ struct wd_context {
   uint64_t mem_total = 0;
   uint64_t mem_free_at_open = 0;
   bool closed = false;
-  // Clouds and states alive on this context: a batch checks its jobs against
-  // it before a run, so a destroyed job is an error, never a dangling pointer.
-  std::unordered_set<const void *> live;
+  // Clouds, states and pools alive on this context, each with its serial (one
+  // per object ever made here, never reused): a batch keeps the serials of its
+  // jobs' objects and checks them before a run, so a destroyed object is an
+  // error even when a new one was given the same address.
+  std::unordered_map<const void *, uint64_t> live;
+  uint64_t next_serial = 1;
+  void adopt(const void *p) { live[p] = next_serial++; }
+  uint64_t serial(const void *p) const {
+    const auto it = live.find(p);
+    return it == live.end() ? 0 : it->second;
+  }
   // Per-kernel event timing.
   bool profiling = false;
   struct Pending { int kernel; hipEvent_t a, b; };
@@ -216,7 +225,7 @@ struct wd_batch {
   wd::BatchCtl *ctl = nullptr;    // device
   std::vector<wd::BatchJob> host_jobs;
   std::vector<uint32_t> host_starts;
-  std::vector<const void *> shape;  // clouds and states of the captured graph
+  std::vector<std::pair<const void *, uint64_t>> shape;  // clouds and states of the set jobs, with their serials
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t grid = 0, total = 0;
@@ -262,14 +271,19 @@ enum KernelId {
 extern const char *const kKernelNames[kKernelCount];
 
 // Kernel launchers (splat_sort.hip). Enqueue on ctx->stream.
-int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch8);
+int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch12);
+// The box put() reports, from the kernels' reduction: ordered bits of the least and greatest value
+// per axis, made floats; a zero end takes the sign of the axis's first zero (zero_index, its value
+// read by value_at), since put()'s strict compares keep the first of equal values.
+void box_from_ordered(const uint32_t lo[3], const uint32_t hi[3], const uint32_t zero_index[3],
+                      float (*value_at)(const void *, uint32_t, int), const void *arg, float mn[3], float mx[3]);
 int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_state *st);
 int launch_batch(wd_context *ctx, wd_batch *b);
 // ---- capture decoders (decode.hip, decode_host.cpp) ----
 // Where a decoder writes: the cloud's centres (x, y, z, rgba bits), Sigma (six
 // floats), harmonics (sh_words words a splat, zeroed before), the least index
-// of a non-finite position (UINT32_MAX when none) and the box as ordered bits
-// (min xyz, then max xyz).
+// of a non-finite position (UINT32_MAX when none), the box as ordered bits
+// (min xyz, then max xyz) and each axis's first zero.
 struct DecodeOut {
   float *centers;
   float *covs;
@@ -277,6 +291,7 @@ struct DecodeOut {
   uint32_t sh_words;
   uint32_t *bad;
   uint32_t *box;
+  uint32_t *zero;  // per axis, the least index whose coordinate is a zero of either sign
 };
 // .ply property types, as PLY_TYPES names them (src/scene/splats.js:392-397).
 enum PlyType : uint8_t { kPlyI8, kPlyU8, kPlyI16, kPlyU16, kPlyI32, kPlyU32, kPlyF32, kPlyF64 };

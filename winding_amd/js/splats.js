@@ -85,33 +85,100 @@ function parseSplats(bytes) {
   return { count: data.length / SPLAT_BYTES, degree: 0, records: data, min: null, max: null };
 }
 
-/** A zip's files, by name, each a function that gives its bytes: stored and deflated entries, not zip64. */
-function unzip(data) {
-  const zlib = require('zlib');
-  const view = new DataView(data.buffer, data.byteOffset, data.byteLength);
-  let end = -1;
-  for (let i = data.length - 22; i >= Math.max(0, data.length - 22 - 65535); i--) {
-    if (view.getUint32(i, true) === 0x06054b50) { end = i; break; }
+// ---- zip, as a .sog packs its meta.json and images -------------------------------------------
+// Written from the format's specification (PKWARE's APPNOTE.TXT, sections 4.3.7, 4.3.12 and 4.3.16):
+// the end-of-central-directory record is found by scanning back over at most a 65,535-byte comment,
+// the central directory is walked entry by entry, and each entry's data is found behind its local
+// header. Entries stored (method 0) or deflated (method 8) are read, with their size and CRC-32 checked;
+// zip64 archives are not. The refusals carry the reference's messages (src/scene/splats.js:97-125).
+
+/** Little-endian fields of the three records read: name -> [offset in the record, bytes]. */
+const ZIP_RECORDS = {
+  end: { magic: 0x06054b50, bytes: 22, fields: { entries: [10, 2], dirOffset: [16, 4] } },
+  central: {
+    magic: 0x02014b50, bytes: 46,
+    fields: { method: [10, 2], crc: [16, 4], packed: [20, 4], size: [24, 4], nameBytes: [28, 2],
+      extraBytes: [30, 2], commentBytes: [32, 2], localOffset: [42, 4] },
+  },
+  local: { magic: 0x04034b50, bytes: 30, fields: { nameBytes: [26, 2], extraBytes: [28, 2] } },
+};
+
+let crcTable = null;
+/** CRC-32 (the IEEE polynomial, reflected: 0xEDB88320), byte at a time through a 256-entry table. */
+function crc32(bytes) {
+  if (crcTable === null) {
+    crcTable = new Int32Array(256);
+    for (let b = 0; b < 256; b++) {
+      let r = b;
+      for (let k = 0; k < 8; k++) r = (r & 1) ? (0xedb88320 ^ (r >>> 1)) : (r >>> 1);
+      crcTable[b] = r;
+    }
   }
-  if (end < 0) throw new Error('splats: a zip with no end record: cut short?');
-  const files = new Map();
-  let at = view.getUint32(end + 16, true);
-  for (let n = view.getUint16(end + 10, true); n > 0; n--) {
-    if (at + 46 > data.length || view.getUint32(at, true) !== 0x02014b50) throw new Error('splats: a zip whose directory is damaged');
-    const method = view.getUint16(at + 10, true);
-    const size = view.getUint32(at + 20, true);
-    const nameLength = view.getUint16(at + 28, true);
-    const local = view.getUint32(at + 42, true);
-    const name = Buffer.from(data.subarray(at + 46, at + 46 + nameLength)).toString('utf8');
-    if (size === 0xffffffff || local === 0xffffffff) throw new Error('splats: a zip64 .sog is not read');
-    at += 46 + nameLength + view.getUint16(at + 30, true) + view.getUint16(at + 32, true);
-    const start = local + 30 + view.getUint16(local + 26, true) + view.getUint16(local + 28, true);
-    if (start + size > data.length) throw new Error('splats: the zip is cut short in ' + name);
-    const stored = data.subarray(start, start + size);
-    if (method !== 0 && method !== 8) throw new Error('splats: ' + name + ' is compressed with method ' + method + '; stored and deflate are read');
-    files.set(name, () => (method === 0 ? stored : new Uint8Array(zlib.inflateRawSync(stored))));
+  let c = -1;
+  for (let i = 0; i < bytes.length; i++) c = crcTable[(c ^ bytes[i]) & 0xff] ^ (c >>> 8);
+  return (c ^ -1) >>> 0;
+}
+
+class ZipArchive {
+  constructor(bytes) {
+    this.bytes = bytes;
+    this.view = new DataView(bytes.buffer, bytes.byteOffset, bytes.byteLength);
+    this.entries = new Map();
+    const end = this._findEnd();
+    if (end === null) throw new Error('splats: a zip with no end record: cut short?');
+    let at = end.dirOffset;
+    for (let k = 0; k < end.entries; k++) {
+      const e = this._record('central', at);
+      if (e === null) throw new Error('splats: a zip whose directory is damaged');
+      const name = Buffer.from(bytes.subarray(at + 46, at + 46 + e.nameBytes)).toString('utf8');
+      if (e.packed === 0xffffffff || e.size === 0xffffffff || e.localOffset === 0xffffffff) {
+        throw new Error('splats: a zip64 .sog is not read');
+      }
+      this.entries.set(name, e);
+      at += 46 + e.nameBytes + e.extraBytes + e.commentBytes;
+    }
   }
-  return files;
+
+  /** The record of `kind` at `at`, its fields read, or null when its magic is not there. */
+  _record(kind, at) {
+    const r = ZIP_RECORDS[kind];
+    if (at < 0 || at + r.bytes > this.bytes.length || this.view.getUint32(at, true) !== r.magic) return null;
+    const out = {};
+    for (const name of Object.keys(r.fields)) {
+      const [offset, size] = r.fields[name];
+      out[name] = size === 4 ? this.view.getUint32(at + offset, true) : this.view.getUint16(at + offset, true);
+    }
+    return out;
+  }
+
+  /** The end record: the last 22 bytes, or earlier by the length of a trailing comment. */
+  _findEnd() {
+    const last = this.bytes.length - ZIP_RECORDS.end.bytes;
+    for (let at = last; at >= 0 && at >= last - 0xffff; at--) {
+      const r = this._record('end', at);
+      if (r !== null) return r;
+    }
+    return null;
+  }
+
+  has(name) {
+    return this.entries.has(name);
+  }
+
+  /** An entry's bytes, inflated when deflated, checked against the directory's size and CRC-32. */
+  read(name) {
+    const e = this.entries.get(name);
+    const local = this._record('local', e.localOffset);
+    const start = local === null ? -1 : e.localOffset + 30 + local.nameBytes + local.extraBytes;
+    if (start < 0 || start + e.packed > this.bytes.length) throw new Error('splats: the zip is cut short in ' + name);
+    if (e.method !== 0 && e.method !== 8) {
+      throw new Error('splats: ' + name + ' is compressed with method ' + e.method + '; stored and deflate are read');
+    }
+    const packed = this.bytes.subarray(start, start + e.packed);
+    const data = e.method === 0 ? packed : new Uint8Array(require('zlib').inflateRawSync(packed));
+    if (data.length !== e.size || crc32(data) !== e.crc) throw new Error('splats: ' + name + ' in the zip is damaged');
+    return data;
+  }
 }
 
 /**
@@ -127,43 +194,50 @@ async function readSplats(bytes, options) {
   return parseSplats(data);
 }
 
-/** readSog's host part (src/scene/splats.js:128-160): meta.json and the decoded images. */
+/**
+ * A .sog's host part: its meta.json and its images, decoded by the caller's decodeImage(bytes) ->
+ * { width, height, rgba }, in the shape the GPU decoder takes (wd_sog). What meta.json fixes and what
+ * is refused follow the reference's readSog (src/scene/splats.js:128-160), with its messages; the
+ * splats themselves are decoded on the GPU by new Splats.
+ */
 async function readSog(data, decodeImage) {
   if (typeof decodeImage !== 'function') throw new Error('splats: a .sog needs decodeImage, for its WebP images');
-  const files = unzip(data);
-  const file = (name) => {
-    const get = files.get(name);
-    if (get === undefined) throw new Error('splats: the .sog has no ' + name);
-    return get();
+  const zip = new ZipArchive(data);
+  const bytesOf = (name) => {
+    if (!zip.has(name)) throw new Error('splats: the .sog has no ' + name);
+    return zip.read(name);
   };
-  const meta = JSON.parse(Buffer.from(file('meta.json')).toString('utf8'));
+  const meta = JSON.parse(Buffer.from(bytesOf('meta.json')).toString('utf8'));
   if (meta.version !== 2) {
-    throw new Error('splats: a .sog of version ' + (meta.version === undefined ? 1 : meta.version) + '; version 2 is read (re-save it with splat-transform)');
+    const v = meta.version === undefined ? 1 : meta.version;
+    throw new Error('splats: a .sog of version ' + v + '; version 2 is read (re-save it with splat-transform)');
   }
   const count = meta.count;
   if (!(Number.isInteger(count) && count > 0)) throw new Error('splats: a .sog of ' + count + ' splats');
-  const image = async (name, checkCount) => {
-    const decoded = await decodeImage(file(name));
-    if (checkCount !== false && decoded.width * decoded.height < count) {
-      throw new Error('splats: ' + name + ' has ' + (decoded.width * decoded.height) + ' pixels, for ' + count + ' splats');
-    }
-    return { width: decoded.width, height: decoded.height, rgba: Uint8Array.from(decoded.rgba) };
-  };
-  const [meansL, meansU, quats, scales, sh0] = await Promise.all([
-    image(meta.means.files[0]), image(meta.means.files[1]), image(meta.quats.files[0]),
-    image(meta.scales.files[0]), image(meta.sh0.files[0]),
-  ]);
   const bands = meta.shN ? meta.shN.bands : 0;
   const degree = SH_COEFFICIENTS[bands] > 0 ? bands : 0;
+  // [wd_sog image, file in the zip, a pixel a splat?]; the harmonic palette is as wide as its degree makes it
+  const wanted = [
+    ['means_l', meta.means.files[0], true], ['means_u', meta.means.files[1], true], ['quats', meta.quats.files[0], true],
+    ['scales', meta.scales.files[0], true], ['sh0', meta.sh0.files[0], true],
+  ];
+  if (degree > 0) wanted.push(['centroids', meta.shN.files[0], false], ['labels', meta.shN.files[1], true]);
+  const decoded = await Promise.all(wanted.map(async ([, file, perSplat]) => {
+    const img = await decodeImage(bytesOf(file));
+    const pixels = img.width * img.height;
+    if (perSplat && pixels < count) throw new Error('splats: ' + file + ' has ' + pixels + ' pixels, for ' + count + ' splats');
+    return { width: img.width, height: img.height, rgba: Uint8Array.from(img.rgba) };
+  }));
+  const images = {};
+  wanted.forEach(([key], k) => { images[key] = decoded[k]; });
   const sog = {
-    count, bands: degree, mins: Float64Array.from(meta.means.mins), maxs: Float64Array.from(meta.means.maxs),
+    count, bands: degree, images,
+    mins: Float64Array.from(meta.means.mins), maxs: Float64Array.from(meta.means.maxs),
     scaleCodebook: Float64Array.from(meta.scales.codebook), colourCodebook: Float64Array.from(meta.sh0.codebook),
-    images: { means_l: meansL, means_u: meansU, quats, scales, sh0 },
   };
   if (degree > 0) {
     sog.shCount = meta.shN.count === undefined ? 0xffffffff : meta.shN.count;
     sog.shCodebook = Float64Array.from(meta.shN.codebook);
-    [sog.images.centroids, sog.images.labels] = await Promise.all([image(meta.shN.files[0], false), image(meta.shN.files[1])]);
   }
   return { format: 'sog', sog, count: null, degree: null, min: null, max: null };
 }
