@@ -57,6 +57,8 @@ extern "C" {
 #define WD_COARSE_BINS 256
 /* Pose features the predictor reads: the model-view's top three rows. */
 #define WD_POSE_FLOATS 12
+/* Most splats wd_sort_noalloc takes (its intermediates live in a fixed array of about 14 MiB). */
+#define WD_NOALLOC_MAX_SPLATS (1u << 21)
 /* Bytes in one particle: position, age, velocity, lifetime. src/render/particles.js:51-56 */
 #define WD_PARTICLE_BYTES 32
 /* The particle sort's log-depth buckets. src/render/particles.js:45 */
@@ -259,6 +261,25 @@ int wd_state_buffers(wd_state *state, void **order, void **count, void **cov2d);
  */
 int wd_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view *view,
             wd_state *state);
+
+/*
+ * wd_sort for callers that cannot allocate device memory (code under HIP graph
+ * capture, a serving loop that must not allocate): no counterpart in the
+ * reference. It allocates nothing and frees nothing: the sort's intermediates
+ * live in one fixed array per device (a few MiB, held until the process ends),
+ * and the caller passes device buffers for the input and the outputs:
+ *   records  n .splat records (32 B each), taken as they are (the finite check
+ *            of wd_cloud_create is not made here)
+ *   order    u32 x n: splat indices, back to front; the first *count are valid
+ *   count    one u32: survivors V
+ *   cov2d    f32x4 x n, as wd_state_buffers' cov2d (zeros for culled splats)
+ * 1 <= n <= WD_NOALLOC_MAX_SPLATS, or WD_E_CAPACITY. Enqueued on the context's
+ * stream. Every call on one device shares that array and leaves its contents
+ * behind: calls that overlap on one device, from any context, race, so issue
+ * them on one stream or wait between them. Results equal wd_sort's bit for bit.
+ */
+int wd_sort_noalloc(wd_context *ctx, const void *records, uint64_t n, const wd_view *view, uint32_t *order,
+                    uint32_t *count, float *cov2d);
 
 /* Waits, then copies the results to host memory. Any pointer may be null. */
 int wd_state_read(wd_context *ctx, wd_state *state, uint32_t *order,

@@ -17,6 +17,7 @@ WD_RECORD_BYTES = 32
 WD_BUCKETS = 65536
 WD_SPLAT_CULL = 0.5
 WD_MAX_SORT = 1 << 24
+WD_NOALLOC_MAX_SPLATS = 1 << 21
 WD_COARSE_BINS = 256
 WD_POSE_FLOATS = 12
 WD_SH_COEFFICIENTS = (0, 3, 8, 15)
@@ -28,7 +29,7 @@ EXPORTS = [
     "wd_last_error", "wd_open", "wd_close", "wd_sync", "wd_stream", "wd_capacity",
     "wd_cloud_create", "wd_cloud_create_device", "wd_cloud_info", "wd_cloud_records", "wd_cloud_destroy",
     "wd_cloud_sh", "wd_cloud_create_splats", "wd_cloud_read", "wd_decode_ply", "wd_decode_spz", "wd_decode_sog",
-    "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort",
+    "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort", "wd_sort_noalloc",
     "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_state_ellipses", "wd_state_ellipse_buffer",
     "wd_state_read_ellipses", "wd_pool_create", "wd_pool_device", "wd_pool_destroy", "wd_particles_sort",
     "wd_profile", "wd_profile_read", "wd_time_sorts",
@@ -133,6 +134,7 @@ def lib():
             "wd_state_destroy": (I, [P]),
             "wd_state_buffers": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
             "wd_sort": (I, [P, P, C.POINTER(View), P]),
+            "wd_sort_noalloc": (I, [P, P, U64, C.POINTER(View), P, P, P]),
             "wd_state_read": (I, [P, P, P, U64, P, U64, C.POINTER(Stats)]),
             "wd_state_stats": (I, [P, P, C.POINTER(Stats)]),
             "wd_state_read_keys": (I, [P, P, P, P, U64]),
@@ -193,6 +195,12 @@ def check(rc):
 
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def sort_noalloc(ctx, records, n, view, order, count, cov2d):
+    """wd_sort_noalloc: device addresses (ints) of the records (n x 32 bytes), order (u32 x n), count (u32) and
+    cov2d (f32x4 x n); allocates nothing, enqueued on the context's stream."""
+    check(lib().wd_sort_noalloc(ctx.ptr, records, n, C.byref(view), order, count, cov2d))
 
 
 def synth_records(kind, seed, n):

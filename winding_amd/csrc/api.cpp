@@ -120,6 +120,15 @@ int wd_open(int device, wd_context **out) {
   }
   ctx->mem_total = total_b;
   ctx->mem_free_at_open = free_b;
+  // the no-allocation sort's array: its address on this device, taken once here
+  void *scratch = nullptr;
+  const int rc = wd::noalloc_scratch(&scratch);
+  if (rc != WD_OK) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return rc;
+  }
+  ctx->noalloc = (uint8_t *)scratch;
   *out = ctx;
   return WD_OK;
 }
@@ -542,6 +551,29 @@ int wd_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_stat
   if (rc) return rc;
   WD_HIP(hipSetDevice(ctx->device), "sort: set device");
   return wd::launch_sort(ctx, cloud, *view, s);
+}
+
+int wd_sort_noalloc(wd_context *ctx, const void *records, uint64_t n, const wd_view *view, uint32_t *order,
+                    uint32_t *count, float *cov2d) {
+  if (!ctx || !records || !view || !order || !count || !cov2d) return fail(WD_E_ARG, "sort: null argument");
+  if (n == 0 || n > WD_NOALLOC_MAX_SPLATS)
+    return fail(WD_E_CAPACITY, "sort: %llu splats; a sort without allocation takes 1 to %u", (unsigned long long)n,
+                WD_NOALLOC_MAX_SPLATS);
+  if (!(view->cull >= 0 && std::isfinite(view->cull)))
+    return fail(WD_E_ARG, "splatCull must be 0 or more, got %g", (double)view->cull);
+  if (!ctx->noalloc) return fail(WD_E_STATE, "sort: the context has no scratch for a sort without allocation");
+  WD_HIP(hipSetDevice(ctx->device), "sort: set device");
+  constexpr wd::NoallocLayout L = wd::noalloc_layout();
+  uint8_t *base = ctx->noalloc;
+  const wd::SortInput in{records, nullptr, nullptr, 0, (uint32_t)n};
+  const wd::SortBuffers b{(wd::Ctl *)(base + L.ctl),        (uint16_t *)(base + L.keys),
+                          (uint64_t *)(base + L.mask),      (uint32_t *)(base + L.packed),
+                          order,                            cov2d,
+                          (uint16_t *)(base + L.tilehist),  (uint16_t *)(base + L.tilehist3),
+                          (uint32_t *)(base + L.prefix_lo), (uint32_t *)(base + L.prefix_hi),
+                          nullptr,                          n,
+                          count,                            false};
+  return wd::launch_sort_into(ctx, in, *view, b);
 }
 
 int wd_state_stats(wd_context *ctx, wd_state *s, wd_stats *stats) {

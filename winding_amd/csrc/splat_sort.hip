@@ -144,16 +144,20 @@ __global__ __launch_bounds__(256) void k_scan(uint32_t T, const uint16_t *__rest
   }
 }
 
-// Pass 2 and 3: a workgroup a super-tile.
+// Pass 2 and 3: a workgroup a super-tile. count_out (the low-byte pass, when
+// given) gets V as well as the control block does.
 template <bool HI>
 __global__ __launch_bounds__(256) void k_scatter(const uint16_t *__restrict__ keys, const uint64_t *__restrict__ mask,
                                                  const uint32_t *__restrict__ in_packed, uint32_t n,
                                                  const uint16_t *__restrict__ hist, uint32_t hstride,
                                                  const uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl,
-                                                 uint32_t *__restrict__ out, uint32_t out_base) {
+                                                 uint32_t *__restrict__ out, uint32_t out_base,
+                                                 uint32_t *__restrict__ count_out) {
   __shared__ ScatterSmem<kSuper> sm;
-  scatter_super<HI, kSuper>(sm, xcd_item(blockIdx.x, gridDim.x), keys, mask, in_packed, n, hist, hstride, prefix,
+  const uint32_t st = xcd_item(blockIdx.x, gridDim.x);
+  scatter_super<HI, kSuper>(sm, st, keys, mask, in_packed, n, hist, hstride, prefix,
                             HI ? ctl->total + 256 : ctl->total, ctl, out, out_base);
+  if (!HI && count_out && st == 0 && threadIdx.x == 0) *count_out = ctl->count;  // this thread wrote it
 }
 
 // Upsweep of pass 3: a workgroup a tile of pass 2's output.
@@ -352,21 +356,19 @@ int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t 
   return ctx->prof_end(kKBounds, ev);
 }
 
-int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_state *st) {
-  const uint32_t n = (uint32_t)cloud->n;
+int launch_sort_into(wd_context *ctx, const SortInput &in, const wd_view &view, const SortBuffers &b) {
+  const uint32_t n = in.n;
   const ViewK vk = make_viewk(view);
   const bool std_form = is_std(view);
-  st->last_n = n;
   const uint32_t tiles = (n + kTile - 1) / kTile;
   const uint32_t supers = (tiles + kSuper - 1) / kSuper;
-  const uint4 *rec = (const uint4 *)cloud->records;
   hipEvent_t ev = nullptr;
-#define WD_LAUNCH(id, what, ...)                  \
-  do {                                            \
-    if (ctx->prof_begin(id, &ev)) return WD_E_HIP; \
-    hipLaunchKernelGGL(__VA_ARGS__);              \
-    WD_HIP(hipGetLastError(), what);              \
-    if (ctx->prof_end(id, ev)) return WD_E_HIP;   \
+#define WD_LAUNCH(id, what, ...)                                   \
+  do {                                                             \
+    if (b.profile && ctx->prof_begin(id, &ev)) return WD_E_HIP;    \
+    hipLaunchKernelGGL(__VA_ARGS__);                               \
+    WD_HIP(hipGetLastError(), what);                               \
+    if (b.profile && ctx->prof_end(id, ev)) return WD_E_HIP;       \
   } while (0)
   // the projection's form: the fast reverse-Z one or the general one, times the
   // harmonics' degree, times whether the state takes the draw's ellipses
@@ -380,25 +382,49 @@ int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_
       {WD_FORMS(false, 0), WD_FORMS(false, 1), WD_FORMS(false, 2), WD_FORMS(false, 3)},
       {WD_FORMS(true, 0), WD_FORMS(true, 1), WD_FORMS(true, 2), WD_FORMS(true, 3)}};
 #undef WD_FORMS
-  const bool sig = cloud->covs != nullptr;
+  const bool sig = in.covs != nullptr;
   const ProjectKernel project =
-      kForms[std_form ? 1 : 0][cloud->degree <= 3 ? cloud->degree : 0][st->ellipse ? 1 : 0][sig ? 1 : 0];
-  const uint4 *first = sig ? (const uint4 *)cloud->centers : rec;
-  WD_LAUNCH(kKProject, "sort: project launch", project, dim3(tiles), dim3(256), 0, ctx->stream, first, n, vk, st->keys,
-            st->mask, (float4 *)st->cov, st->tilehist, st->ctl, (const uint32_t *)cloud->sh, (float4 *)st->ellipse,
-            st->capacity, (const uint2 *)cloud->covs);
-  WD_LAUNCH(kKScan, "sort: scan launch", k_scan, dim3(512 / kScanCols), dim3(256), 0, ctx->stream, tiles,
-            st->tilehist, st->prefix_lo, st->ctl);
+      kForms[std_form ? 1 : 0][in.degree <= 3 ? in.degree : 0][b.ellipse ? 1 : 0][sig ? 1 : 0];
+  WD_LAUNCH(kKProject, "sort: project launch", project, dim3(tiles), dim3(256), 0, ctx->stream, (const uint4 *)in.first,
+            n, vk, b.keys, b.mask, (float4 *)b.cov, b.tilehist, b.ctl, (const uint32_t *)in.sh, (float4 *)b.ellipse,
+            b.capacity, (const uint2 *)in.covs);
+  WD_LAUNCH(kKScan, "sort: scan launch", k_scan, dim3(512 / kScanCols), dim3(256), 0, ctx->stream, tiles, b.tilehist,
+            b.prefix_lo, b.ctl);
   WD_LAUNCH(kKScatterLo, "sort: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0, ctx->stream,
-            st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->ctl, st->packed, 0u);
-  WD_LAUNCH(kKHist, "sort: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, st->packed, st->ctl,
-            st->tilehist3);
+            b.keys, b.mask, (const uint32_t *)nullptr, n, b.tilehist, 512u, b.prefix_lo, b.ctl, b.packed, 0u,
+            b.count_out);
+  WD_LAUNCH(kKHist, "sort: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, b.packed, b.ctl,
+            b.tilehist3);
   WD_LAUNCH(kKScan2, "sort: second scan launch", k_scan2, dim3(256 / kScanCols), dim3(256), 0, ctx->stream,
-            st->tilehist3, st->prefix_hi, st->ctl);
+            b.tilehist3, b.prefix_hi, b.ctl);
   WD_LAUNCH(kKScatterHi, "sort: high-byte pass launch", k_scatter<true>, dim3(supers), dim3(256), 0, ctx->stream,
-            (const uint16_t *)nullptr, (const uint64_t *)nullptr, st->packed, n, st->tilehist3, 256u, st->prefix_hi,
-            st->ctl, st->order, 0u);
+            (const uint16_t *)nullptr, (const uint64_t *)nullptr, b.packed, n, b.tilehist3, 256u, b.prefix_hi, b.ctl,
+            b.order, 0u, (uint32_t *)nullptr);
 #undef WD_LAUNCH
+  return WD_OK;
+}
+
+int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_state *st) {
+  st->last_n = cloud->n;
+  const bool sig = cloud->covs != nullptr;
+  const SortInput in{sig ? (const uint4 *)cloud->centers : (const uint4 *)cloud->records, (const uint2 *)cloud->covs,
+                     (const uint32_t *)cloud->sh, cloud->degree, (uint32_t)cloud->n};
+  const SortBuffers b{st->ctl,       st->keys,      st->mask,      st->packed, st->order, st->cov, st->tilehist,
+                      st->tilehist3, st->prefix_lo, st->prefix_hi, st->ellipse, st->capacity, nullptr, true};
+  return launch_sort_into(ctx, in, view, b);
+}
+
+// ---------------------------------------------------------------------------
+// The sort's intermediates for callers that cannot allocate (wd_sort_noalloc):
+// one array per device, of the size kNoallocBytes gives for up to
+// WD_NOALLOC_MAX_SPLATS splats, laid out as noalloc_layout cuts it. Host code
+// takes its device address from hipGetSymbolAddress (noalloc_scratch), never
+// from the name, which on the host only stands in for the symbol.
+static __device__ __align__(256) uint8_t scratch[kNoallocBytes];
+
+int noalloc_scratch(void **ptr) {
+  *ptr = nullptr;
+  WD_HIP(hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)), "open: the no-allocation sort's scratch");
   return WD_OK;
 }
 
@@ -429,14 +455,14 @@ int launch_particle_sort(wd_context *ctx, const uint8_t *pool, uint64_t first, u
             st->tilehist, st->prefix_lo, st->ctl);
   WD_LAUNCH(kKScatterLo, "particles: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0,
             ctx->stream, st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->ctl,
-            st->packed, 0u);
+            st->packed, 0u, (uint32_t *)nullptr);
   WD_LAUNCH(kKHist, "particles: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, st->packed,
             st->ctl, st->tilehist3);
   WD_LAUNCH(kKScan2, "particles: second scan launch", k_scan2, dim3(256 / kScanCols), dim3(256), 0, ctx->stream,
             st->tilehist3, st->prefix_hi, st->ctl);
   WD_LAUNCH(kKScatterHi, "particles: high-byte pass launch", k_scatter<true>, dim3(supers), dim3(256), 0,
             ctx->stream, (const uint16_t *)nullptr, (const uint64_t *)nullptr, st->packed, n, st->tilehist3, 256u,
-            st->prefix_hi, st->ctl, st->order, (uint32_t)first);
+            st->prefix_hi, st->ctl, st->order, (uint32_t)first, (uint32_t *)nullptr);
 #undef WD_LAUNCH
   return WD_OK;
 }

@@ -31,6 +31,14 @@ constexpr int kKeysPerThread = 16;
 constexpr uint32_t kScanCols = 4;    // histogram columns a scan workgroup takes (a single sort)
 constexpr uint32_t kBatchScanCols = 8;  // the same in a batch's scans
 
+// The no-allocation sort's scratch (wd_sort_noalloc): the intermediates of one sort of up to
+// WD_NOALLOC_MAX_SPLATS splats, each region 256-byte aligned, in one array per device.
+constexpr uint64_t align256(uint64_t x) { return (x + 255u) & ~uint64_t(255); }
+struct NoallocLayout {
+  uint64_t ctl, keys, mask, packed, tilehist, tilehist3, prefix_lo, prefix_hi, bytes;
+};
+constexpr uint64_t kNoallocTiles = (WD_NOALLOC_MAX_SPLATS + kTile - 1) / kTile;
+
 // Polls of a bounded wait before it gives up and flags the sort as failed.
 constexpr uint32_t kSpinLimit = 1u << 20;
 
@@ -48,6 +56,31 @@ struct Ctl {
   uint32_t pad[58];
 };
 static_assert(sizeof(Ctl) % 256 == 0, "Ctl keeps 256-byte alignment");
+
+constexpr NoallocLayout noalloc_layout() {
+  NoallocLayout L{};
+  const uint64_t N = WD_NOALLOC_MAX_SPLATS, T = kNoallocTiles;
+  uint64_t at = 0;
+  L.ctl = at;
+  at = align256(at + sizeof(Ctl));
+  L.keys = at;
+  at = align256(at + N * 2);
+  L.mask = at;
+  at = align256(at + (N + 63) / 64 * 8);
+  L.packed = at;
+  at = align256(at + N * 4);
+  L.tilehist = at;
+  at = align256(at + T * 512 * 2);
+  L.tilehist3 = at;
+  at = align256(at + T * 256 * 2);
+  L.prefix_lo = at;
+  at = align256(at + T * 256 * 4);
+  L.prefix_hi = at;
+  at = align256(at + T * 256 * 4);
+  L.bytes = at;
+  return L;
+}
+constexpr uint64_t kNoallocBytes = noalloc_layout().bytes;
 
 // The per-view constants a kernel takes by value: the uniform plus what the
 // f32 fast path and its error bound precompute once per view.
@@ -186,6 +219,7 @@ struct wd_context {
   double prof_ms[16] = {};
   uint64_t prof_launches[16] = {};
   double *particle_edges = nullptr;  // device: the particle sort's 4096 bucket edges, made at first use
+  uint8_t *noalloc = nullptr;        // device: the no-allocation sort's scratch on this device
   int prof_begin(int kernel, hipEvent_t *a);
   int prof_end(int kernel, hipEvent_t a);
   int prof_collect();
@@ -278,6 +312,33 @@ int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t 
 void box_from_ordered(const uint32_t lo[3], const uint32_t hi[3], const uint32_t zero_index[3],
                       float (*value_at)(const void *, uint32_t, int), const void *arg, float mn[3], float mx[3]);
 int launch_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view &view, wd_state *st);
+// What one sort reads: .splat records (first) or centres (first) with Sigma (covs), the
+// harmonics (sh words, degree 0 when none), n splats.
+struct SortInput {
+  const void *first;
+  const void *covs;
+  const void *sh;
+  uint32_t degree;
+  uint32_t n;
+};
+// Where one sort writes: its intermediates and outputs. count_out, when given, gets V too;
+// profile times each launch with HIP events (the no-allocation path never does).
+struct SortBuffers {
+  Ctl *ctl;
+  uint16_t *keys;
+  uint64_t *mask;
+  uint32_t *packed, *order;
+  float *cov;
+  uint16_t *tilehist, *tilehist3;
+  uint32_t *prefix_lo, *prefix_hi;
+  float *ellipse;
+  uint64_t capacity;
+  uint32_t *count_out;
+  bool profile;
+};
+int launch_sort_into(wd_context *ctx, const SortInput &in, const wd_view &view, const SortBuffers &b);
+// The no-allocation sort's scratch on the current device (hipGetSymbolAddress of the array).
+int noalloc_scratch(void **ptr);
 int launch_batch(wd_context *ctx, wd_batch *b);
 // ---- capture decoders (decode.hip, decode_host.cpp) ----
 // Where a decoder writes: the cloud's centres (x, y, z, rgba bits), Sigma (six
