@@ -41,9 +41,10 @@ bool alive(const wd_learned *l) { return l && l->magic == 0x4c524e44; }
 
 // Learned-sort buffers of a state, made on its first learned run.
 int ensure_learned(wd_context *ctx, wd_state *s) {
-  if (s->lut) return WD_OK;
-  hipError_t e = hipMalloc(&s->lut, wd::kBinBlocks * sizeof(uint16_t));
-  if (e == hipSuccess) e = hipMalloc(&s->prefix_bin, s->tiles * 512 * sizeof(uint32_t));
+  // each buffer made once; a failed allocation leaves the state to try again on its next run
+  hipError_t e = hipSuccess;
+  if (!s->lut) e = hipMalloc(&s->lut, wd::kBinBlocks * sizeof(uint16_t));
+  if (e == hipSuccess && !s->prefix_bin) e = hipMalloc(&s->prefix_bin, s->tiles * 512 * sizeof(uint32_t));
   if (e != hipSuccess) return wd::hip_fail(e, "learned: state buffers");
   return WD_OK;
 }
