@@ -380,18 +380,22 @@ typedef struct wd_job {
  * _encodeSort (src/render/splats.js:676-697): every job's projection and both
  * scatter passes run from one hipGraph launch, on one of two schedules:
  *   WD_SCHEDULE_PERSISTENT  ONE persistent kernel taking every job's work
- *                           tile by tile from a ticketed queue;
+ *                           run by run (a few tiles) from a ticketed queue;
  *   WD_SCHEDULE_PHASED      six kernels, each over every job's tiles.
  * A new batch runs phased with super-tiles of 1; wd_batch_tune times the
- * configurations and keeps the fastest.
+ * configurations and keeps the fastest. On MI355X the phased schedule wins:
+ * the persistent one must make each item's writes visible to the other XCDs'
+ * L2s before counting it done, and that write-back costs more than the five
+ * launch boundaries it replaces.
  */
 #define WD_SCHEDULE_PHASED 0
 #define WD_SCHEDULE_PERSISTENT 1
 int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out);
 /*
- * Picks the schedule and the scatter passes' work item: super_tiles tiles of
- * 4096 a workgroup, 1 or 2 (the persistent schedule always takes 1). The
- * graph is captured again on the next run.
+ * Picks the schedule and its work item. Phased: super_tiles tiles of 4096 a
+ * scatter workgroup takes at once, 1 or 2. Persistent: super_tiles tiles an
+ * item takes one after the other, 1 to 64 (one release and one count a run).
+ * The graph is captured again on the next run.
  */
 int wd_batch_schedule(wd_batch *batch, int schedule, uint32_t super_tiles);
 /* The autotuner's report: the configuration it kept, and each one's mean ms a run. */
@@ -400,7 +404,7 @@ typedef struct wd_tuning {
   uint32_t super_tiles;
   double ms_phased_1;     /* phased, super-tiles of 1 */
   double ms_phased_2;     /* phased, super-tiles of 2 */
-  double ms_persistent;   /* persistent (super-tiles of 1) */
+  double ms_persistent;   /* persistent, runs of 8 tiles */
 } wd_tuning;
 /* Runs the set jobs `reps` times on each configuration and keeps the fastest. */
 int wd_batch_tune(wd_context *ctx, wd_batch *batch, uint32_t reps, wd_tuning *out);

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(ROOT, "libwinding_amd.so")
+# WINDING_AMD_LIB: another build of the library (say, one made with other keys a thread)
+LIB_PATH = os.environ.get("WINDING_AMD_LIB") or os.path.join(ROOT, "libwinding_amd.so")
 
 WD_OK = 0
 WD_RECORD_BYTES = 32
@@ -599,7 +600,9 @@ class Batch:
     def run(self):
         check(lib().wd_batch_run(self.ctx.ptr, self.ptr))
 
-    def schedule(self, which, super_tiles=2):
+    def schedule(self, which, super_tiles=1):
+        """Phased: super_tiles (1 or 2) tiles a scatter workgroup takes at once. Persistent: tiles an
+        item takes one after the other (1 to 64)."""
         check(lib().wd_batch_schedule(self.ptr, which, super_tiles))
 
     def tune(self, reps=5):

@@ -4,15 +4,19 @@
 // every job in each:
 //   [projection chunks][scan column groups][low-byte super-tiles]
 //   [high-byte histograms][second-scan column groups][high-byte super-tiles]
-// A workgroup takes the next ticket, does that item, and takes another,
-// until the queue is empty. An item only ever waits on items with smaller
+// An item is a run of a few tiles of one job (a scan item: a group of
+// columns). A workgroup takes the next ticket, does that item, and takes
+// another, until the queue is empty. An item only ever waits on items with smaller
 // tickets -- a job's scan on its chunks, its low-byte pass on its scan, its
 // histograms on its low-byte pass, its second scan on its histograms, its
 // high-byte pass on its second scan -- and smaller tickets were taken by
 // workgroups already running, so every wait ends, whatever the grid size.
-// Jobs of 10k and 2M splats share the machine tile by tile instead of launch
-// by launch. The price is a release fence after every item: on a chip whose
-// L2 is one per XCD, each is a write-back of this XCD's L2.
+// Jobs of 10k and 2M splats share the machine run by run instead of launch
+// by launch. The price is a release after every item and an acquire before
+// every wait: on a chip whose L2 is one per XCD, a write-back and an
+// invalidation of this XCD's L2. Runs of tiles and one wave's fence an item
+// bring that down, but it still costs more than the phased schedule's five
+// launch boundaries.
 //
 // Phased (six kernels): each of the six runs is one launch over every job's
 // items; the launch boundaries are the only barriers, so no fence runs.
@@ -56,11 +60,15 @@ __device__ __forceinline__ void wait_for(const uint32_t *p, uint32_t want, uint3
 }
 
 // This workgroup's item is done: every thread's writes are released before
-// the count is.
+// the count is. The barrier orders the workgroup's writes before thread 0's
+// fence, and a release is cumulative, so one wave's fence (one write-back of
+// this XCD's L2) releases them all.
 __device__ __forceinline__ void count_done(uint32_t *p) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(p, 1u);
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Item bodies shared by both schedules.
@@ -124,9 +132,11 @@ static_assert(kScanItems % kXcds == 0 && kScan2Items % kXcds == 0, "scan items a
 // gate: when given, the whole launch is skipped unless *gate is nonzero (the
 // learned sort's fallback: it runs, for every job, when any view failed its
 // check).
+// run: tiles (or super-tiles) an item takes, one after the other, so that a
+// run of them pays one release fence and one count instead of one each.
 __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs, const uint32_t *__restrict__ starts,
                                                uint32_t nj, uint32_t max_jobs, BatchCtl *__restrict__ bc,
-                                               const uint32_t *__restrict__ gate) {
+                                               const uint32_t *__restrict__ gate, uint32_t run) {
   if (gate && __hip_atomic_load(gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
   __shared__ union {
     ProjectSmem p;
@@ -140,7 +150,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
   uint32_t *lo_done = scan_done + max_jobs;
   uint32_t *hist_done = lo_done + max_jobs;
   uint32_t *scan2_done = hist_done + max_jobs;
-  // starts: per job item starts of tiles and super-tiles, nj + 1 each
+  // starts: per job item starts of runs of tiles and of super-tiles, nj + 1 each
   const uint32_t *t_start = starts, *u_start = starts + (nj + 1);
   const uint32_t nt = t_start[nj], nu = u_start[nj];
   const uint32_t e0 = nt, e1 = e0 + kScanItems * nj, e2 = e1 + nu, e3 = e2 + nt, e4 = e3 + kScan2Items * nj,
@@ -154,39 +164,58 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
     if (item >= total) break;
     if (item < e0) {
       const uint32_t j = find_job(t_start, nj, item);
-      item_project(sm.p, jobs[j], item - t_start[j]);
+      const BatchJob &J = jobs[j];
+      const uint32_t t0 = (item - t_start[j]) * run, t1 = min(t0 + run, J.tiles);
+      for (uint32_t t = t0; t < t1; ++t) {
+        if (t != t0) __syncthreads();  // the tile before is done with the shared memory
+        item_project(sm.p, J, t);
+      }
       count_done(&proj_done[j]);
     } else if (item < e1) {
       const uint32_t j = (item - e0) / kScanItems;
       const BatchJob &J = jobs[j];
-      wait_for(&proj_done[j], J.tiles, &bc->stuck);
+      wait_for(&proj_done[j], t_start[j + 1] - t_start[j], &bc->stuck);
       item_scan(J, (item - e0) % kScanItems, sm.part);
       count_done(&scan_done[j]);
     } else if (item < e2) {
       const uint32_t it = item - e1, j = find_job(u_start, nj, it);
       const BatchJob &J = jobs[j];
       wait_for(&scan_done[j], kScanItems, &bc->stuck);
-      scatter_super<false, kSuper>(sm.s, it - u_start[j], J.keys, J.mask, nullptr, J.n, J.tilehist, 512, J.prefix_lo,
-                                   J.ctl->total, J.ctl, J.packed);
+      const uint32_t supers = (J.tiles + kSuper - 1) / kSuper;
+      const uint32_t s0 = (it - u_start[j]) * run, s1 = min(s0 + run, supers);
+      for (uint32_t st = s0; st < s1; ++st) {
+        if (st != s0) __syncthreads();
+        scatter_super<false, kSuper>(sm.s, st, J.keys, J.mask, nullptr, J.n, J.tilehist, 512, J.prefix_lo,
+                                     J.ctl->total, J.ctl, J.packed);
+      }
       count_done(&lo_done[j]);
     } else if (item < e3) {
       const uint32_t it = item - e2, j = find_job(t_start, nj, it);
       const BatchJob &J = jobs[j];
       wait_for(&lo_done[j], u_start[j + 1] - u_start[j], &bc->stuck);
-      hist_tile(sm.h, it - t_start[j], J.packed, J.ctl, J.tilehist3);
+      const uint32_t t0 = (it - t_start[j]) * run, t1 = min(t0 + run, J.tiles);
+      for (uint32_t t = t0; t < t1; ++t) {
+        if (t != t0) __syncthreads();
+        hist_tile(sm.h, t, J.packed, J.ctl, J.tilehist3);
+      }
       count_done(&hist_done[j]);
     } else if (item < e4) {
       const uint32_t j = (item - e3) / kScan2Items;
       const BatchJob &J = jobs[j];
-      wait_for(&hist_done[j], J.tiles, &bc->stuck);
+      wait_for(&hist_done[j], t_start[j + 1] - t_start[j], &bc->stuck);
       item_scan2(J, (item - e3) % kScan2Items, sm.part);
       count_done(&scan2_done[j]);
     } else {
       const uint32_t it = item - e4, j = find_job(u_start, nj, it);
       const BatchJob &J = jobs[j];
       wait_for(&scan2_done[j], kScan2Items, &bc->stuck);
-      scatter_super<true, kSuper>(sm.s, it - u_start[j], nullptr, nullptr, J.packed, J.n, J.tilehist3, 256,
-                                  J.prefix_hi, J.ctl->total + 256, J.ctl, J.order);
+      const uint32_t supers = (J.tiles + kSuper - 1) / kSuper;
+      const uint32_t s0 = (it - u_start[j]) * run, s1 = min(s0 + run, supers);
+      for (uint32_t st = s0; st < s1; ++st) {
+        if (st != s0) __syncthreads();
+        scatter_super<true, kSuper>(sm.s, st, nullptr, nullptr, J.packed, J.n, J.tilehist3, 256, J.prefix_hi,
+                                    J.ctl->total + 256, J.ctl, J.order);
+      }
     }
   }
   // The last workgroup out resets the queue for the next run. A run whose wait
@@ -265,10 +294,10 @@ int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid) {
 }
 
 int launch_persistent(wd_context *ctx, const BatchJob *jobs, const uint32_t *starts, uint32_t nj, uint32_t max_jobs,
-                      BatchCtl *bctl, uint32_t grid, const uint32_t *gate) {
+                      BatchCtl *bctl, uint32_t grid, const uint32_t *gate, uint32_t run) {
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKBatch, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), 0, ctx->stream, jobs, starts, nj, max_jobs, bctl, gate);
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), 0, ctx->stream, jobs, starts, nj, max_jobs, bctl, gate, run);
   WD_HIP(hipGetLastError(), "batch: launch");
   return ctx->prof_end(kKBatch, ev);
 }
@@ -276,7 +305,7 @@ int launch_persistent(wd_context *ctx, const BatchJob *jobs, const uint32_t *sta
 int launch_batch(wd_context *ctx, wd_batch *b) {
   const uint32_t nj = b->n, nt = b->host_starts[nj], nu = b->host_starts[2 * nj + 1];
   if (b->schedule == WD_SCHEDULE_PERSISTENT)
-    return launch_persistent(ctx, b->jobs, b->starts, nj, b->max_jobs, b->ctl, b->grid, nullptr);
+    return launch_persistent(ctx, b->jobs, b->starts, nj, b->max_jobs, b->ctl, b->grid, nullptr, b->super_tiles);
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKBatch, &ev)) return WD_E_HIP;
   hipLaunchKernelGGL(kb_project, dim3(nt), dim3(256), 0, ctx->stream, b->jobs, b->starts, nj);

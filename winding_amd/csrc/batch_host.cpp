@@ -84,9 +84,15 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     B.prefix_lo = s->prefix_lo;
     B.prefix_hi = s->prefix_hi;
     B.tiles = (B.n + wd::kTile - 1) / wd::kTile;
-    t[i + 1] = t[i] + B.tiles;
-    const uint32_t S = b->schedule == WD_SCHEDULE_PERSISTENT ? (uint32_t)wd::kSuper : b->super_tiles;
-    u[i + 1] = u[i] + (B.tiles + S - 1) / S;
+    if (b->schedule == WD_SCHEDULE_PERSISTENT) {
+      // a persistent item is a run of super_tiles tiles (scatters: super-tiles of kSuper)
+      const uint32_t R = b->super_tiles, supers = (B.tiles + wd::kSuper - 1) / wd::kSuper;
+      t[i + 1] = t[i] + (B.tiles + R - 1) / R;
+      u[i + 1] = u[i] + (supers + R - 1) / R;
+    } else {
+      t[i + 1] = t[i] + B.tiles;
+      u[i + 1] = u[i] + (B.tiles + b->super_tiles - 1) / b->super_tiles;
+    }
     s->last_n = B.n;
     shape.emplace_back(J.cloud, b->ctx->serial(J.cloud));
     shape.emplace_back(J.state, b->ctx->serial(J.state));
@@ -171,7 +177,10 @@ int wd_batch_schedule(wd_batch *b, int schedule, uint32_t super_tiles) {
   if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
   if (schedule != WD_SCHEDULE_PHASED && schedule != WD_SCHEDULE_PERSISTENT)
     return fail(WD_E_ARG, "batch: schedule %d is neither phased (0) nor persistent (1)", schedule);
-  if (super_tiles != 1 && super_tiles != 2) return fail(WD_E_ARG, "batch: super-tiles of %u; 1 or 2", super_tiles);
+  if (schedule == WD_SCHEDULE_PHASED && super_tiles != 1 && super_tiles != 2)
+    return fail(WD_E_ARG, "batch: phased super-tiles of %u; 1 or 2", super_tiles);
+  if (schedule == WD_SCHEDULE_PERSISTENT && (super_tiles < 1 || super_tiles > wd::kMaxRun))
+    return fail(WD_E_ARG, "batch: persistent runs of %u tiles; 1 to %u", super_tiles, wd::kMaxRun);
   if (schedule == b->schedule && super_tiles == b->super_tiles) return WD_OK;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -191,7 +200,7 @@ int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, wd_tuning *out) {
   if (ctx->profiling) return fail(WD_E_STATE, "batch: tuning times whole runs; switch kernel profiling off first");
   reps = std::max(reps, 1u);
   const int sched[3] = {WD_SCHEDULE_PHASED, WD_SCHEDULE_PHASED, WD_SCHEDULE_PERSISTENT};
-  const uint32_t sup[3] = {1, 2, (uint32_t)wd::kSuper};
+  const uint32_t sup[3] = {1, 2, wd::kPersistentRun};
   double ms[3] = {0, 0, 0};
   hipEvent_t a = nullptr, z = nullptr;
   WD_HIP(hipEventCreate(&a), "batch: tune events");

@@ -122,14 +122,15 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
   if (n_views != l->last_views || tiles != l->last_tiles) {
     // the fallback's item table: every view has the same tiles
     std::vector<uint32_t> st(2 * (n_views + 1));
+    const uint32_t R = wd::kPersistentRun, runs = (tiles + R - 1) / R, sruns = (supers + R - 1) / R;
     for (uint32_t v = 0; v <= n_views; ++v) {
-      st[v] = v * tiles;
-      st[n_views + 1 + v] = v * supers;
+      st[v] = v * runs;
+      st[n_views + 1 + v] = v * sruns;
     }
     WD_HIP(hipMemcpyAsync(l->starts, st.data(), st.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream),
            "learned: item table");
     WD_HIP(hipStreamSynchronize(ctx->stream), "learned: item table");
-    const uint32_t items = n_views * (2 * tiles + 2 * supers + 512 / wd::kBatchScanCols + 256 / wd::kBatchScanCols);
+    const uint32_t items = n_views * (2 * runs + 2 * sruns + 512 / wd::kBatchScanCols + 256 / wd::kBatchScanCols);
     const int rc = wd::batch_grid(ctx, items, &l->grid);
     if (rc) return rc;
     l->last_views = n_views;
@@ -180,7 +181,7 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
   l->staged_used[f] = true;
   l->flip ^= 1;
   if ((rc = wd::launch_learned(ctx, l->jobs, n_views, n, all_std, l->gate))) return rc;
-  return wd::launch_persistent(ctx, l->jobs, l->starts, n_views, l->max_views, l->bctl, l->grid, l->gate);
+  return wd::launch_persistent(ctx, l->jobs, l->starts, n_views, l->max_views, l->bctl, l->grid, l->gate, wd::kPersistentRun);
 }
 
 int wd_learned_destroy(wd_learned *l) {

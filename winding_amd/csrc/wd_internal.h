@@ -26,8 +26,13 @@ int hip_fail(hipError_t e, const char *what);
   } while (0)
 
 constexpr int kThreads = 256;        // workgroup size of every kernel here
-constexpr int kTile = 4096;          // splats per scatter tile: 256 threads x 16
-constexpr int kKeysPerThread = 16;
+// Keys a thread ranks in a scatter tile (a build option: 8, 16 or 32).
+#ifndef WD_KEYS_PER_THREAD
+#define WD_KEYS_PER_THREAD 16
+#endif
+constexpr int kKeysPerThread = WD_KEYS_PER_THREAD;
+static_assert(kKeysPerThread == 8 || kKeysPerThread == 16 || kKeysPerThread == 32, "8, 16 or 32 keys a thread");
+constexpr int kTile = 256 * kKeysPerThread;  // splats per projection chunk and scatter tile
 constexpr uint32_t kScanCols = 4;    // histogram columns a scan workgroup takes (a single sort)
 constexpr uint32_t kBatchScanCols = 8;  // the same in a batch's scans
 
@@ -148,6 +153,8 @@ struct BatchJob {
 // persistent paths: 1 measured faster than 2 at 1M and 8M splats (more
 // workgroups in flight); the phased batch lets the autotuner pick 1 or 2.
 constexpr int kSuper = 1;
+constexpr uint32_t kMaxRun = 64;        // longest run of tiles a persistent batch item takes
+constexpr uint32_t kPersistentRun = 8;  // the run the tuner and the learned sort's fallback use
 // The learned sort: bins a view's key line is cut into (256 predicted, the
 // wide ones split), keys a bin may span (its offsets fit 9 bits), views one
 // projection launch takes, and the largest cloud (indices packed in 23 bits).
@@ -392,7 +399,7 @@ int launch_particle_sort(wd_context *ctx, const uint8_t *pool, uint64_t first, u
                          const double *edges, wd_state *st);
 // The persistent batch kernel over a job table; skipped unless *gate != 0 when gate is given.
 int launch_persistent(wd_context *ctx, const BatchJob *jobs, const uint32_t *starts, uint32_t nj, uint32_t max_jobs,
-                      BatchCtl *bctl, uint32_t grid, const uint32_t *gate);
+                      BatchCtl *bctl, uint32_t grid, const uint32_t *gate, uint32_t run);
 // The 257 equal-mass edges of a sorted state's keys (learned.hip).
 int launch_quantiles(wd_context *ctx, const wd_state *s, uint32_t *edges_dev);
 // The learned sort's kernels (learned.hip) over nv views of one n-splat cloud.
