@@ -250,9 +250,7 @@ def bench_batch(args, emit=True):
         "config": {"workload": "batch-64", "scenes": len(sizes), "splats": total, "survivors": v,
                    "width": 1280, "height": 720, "cull": args.cull, "size_seed": seed,
                    "parallelism": "single GPU"},
-        "batch": {"schedule": ["phased", "persistent"][tuning.schedule], "super_tiles": tuning.super_tiles,
-                  "tuned_ms": {"phased_1": tuning.ms_phased_1, "phased_2": tuning.ms_phased_2,
-                               "persistent": tuning.ms_persistent}, "plain_ms": ms_plain,
+        "batch": {**tuning.report(), "plain_ms": ms_plain,
                   "plain_value": total / (ms_plain * 1e-3) / 1e6, "speedup_vs_plain": ms_plain / ms},
         "path": {"bytes_per_step": path_bytes, "achieved_gbs": path_bytes / (ms * 1e-3) / 1e9,
                  "frac_of_peak": path_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
@@ -377,7 +375,8 @@ def second_half(args):
              "survivors": b["config"]["survivors"], "replay_ms": b["ms_per_step"],
              "per_scene_ms": b["ms_per_step"] / b["config"]["scenes"], "value": b["value"],
              "plain_ms": bb["plain_ms"], "speedup_vs_plain": bb["speedup_vs_plain"],
-             "schedule": bb["schedule"], "tuning": {k: v for k, v in bb.items() if k not in ("plain_ms", "plain_value", "speedup_vs_plain")},
+             "schedule": bb["schedule"],
+             "tuning": {k: v for k, v in bb.items() if k not in ("plain_ms", "plain_value", "speedup_vs_plain")},
              "path_frac_of_peak": b["path"]["frac_of_peak"]}
     learned = {"workload": "learned-4m", "splats": lr["config"]["splats"], "poses": lr["config"]["poses"],
                "ms_per_orbit": ll["ms_per_orbit"], "value": lr["value"], "plain_value": ll["plain_value"],

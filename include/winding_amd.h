@@ -392,19 +392,39 @@ typedef struct wd_job {
 #define WD_SCHEDULE_PERSISTENT 1
 int wd_batch_create(wd_context *ctx, uint32_t max_jobs, wd_batch **out);
 /*
- * Picks the schedule and its work item. Phased: super_tiles tiles of 4096 a
+ * Picks the schedule, its work item and the tile. Phased: super_tiles tiles a
  * scatter workgroup takes at once, 1 or 2. Persistent: super_tiles tiles an
  * item takes one after the other, 1 to 64 (one release and one count a run).
- * The graph is captured again on the next run.
+ * keys_per_thread: 8 or 16, tiles of 2048 or 4096 splats (256 threads a
+ * workgroup). The graph is captured again on the next run.
  */
+int wd_batch_configure(wd_batch *batch, int schedule, uint32_t super_tiles, uint32_t keys_per_thread);
+/* wd_batch_configure with the batch's keys a thread unchanged (16 on a new batch). */
 int wd_batch_schedule(wd_batch *batch, int schedule, uint32_t super_tiles);
-/* The autotuner's report: the configuration it kept, and each one's mean ms a run. */
+/* One configuration the autotuner timed, and its mean ms a run. */
+typedef struct wd_tuned {
+  int schedule;
+  uint32_t super_tiles;
+  uint32_t keys_per_thread;
+  double ms;
+} wd_tuned;
+#define WD_TUNE_CONFIGS 6
+/*
+ * The autotuner's report: the configuration it kept, and every one it timed,
+ * in this order: phased 1 and 2 super-tiles at 16 keys a thread, the same at
+ * 8, persistent runs of 8 tiles at 16 and at 8. The digits are fixed at two
+ * passes of 8 bits: any two-pass split of a 16-bit key ranks 16 bits a key,
+ * and 8 + 8 makes the smallest tile histograms (2^a + 2^b counters) and
+ * scans; three passes would add a full read and write of the keys.
+ */
 typedef struct wd_tuning {
   int schedule;
   uint32_t super_tiles;
-  double ms_phased_1;     /* phased, super-tiles of 1 */
-  double ms_phased_2;     /* phased, super-tiles of 2 */
-  double ms_persistent;   /* persistent, runs of 8 tiles */
+  uint32_t keys_per_thread;
+  uint32_t tile;        /* splats a tile: 256 x keys_per_thread */
+  uint32_t radix_bits;  /* digit width of each pass (8) */
+  uint32_t passes;      /* scatter passes (2) */
+  wd_tuned timed[WD_TUNE_CONFIGS];
 } wd_tuning;
 /* Runs the set jobs `reps` times on each configuration and keeps the fastest. */
 int wd_batch_tune(wd_context *ctx, wd_batch *batch, uint32_t reps, wd_tuning *out);

@@ -21,9 +21,11 @@ def check_job(state, recs, view, label):
         assert err <= 1e-5, "%s: cov2d error %g" % (label, err)
 
 
-@pytest.mark.parametrize("schedule,super_tiles", [(wa.WD_SCHEDULE_PHASED, 2), (wa.WD_SCHEDULE_PHASED, 1),
-                                                  (wa.WD_SCHEDULE_PERSISTENT, 2)])
-def test_batch_uneven_jobs_and_replays(ctx, schedule, super_tiles):
+@pytest.mark.parametrize("schedule,super_tiles,kpt", [(wa.WD_SCHEDULE_PHASED, 2, 16), (wa.WD_SCHEDULE_PHASED, 1, 16),
+                                                      (wa.WD_SCHEDULE_PERSISTENT, 2, 16),
+                                                      (wa.WD_SCHEDULE_PHASED, 2, 8),
+                                                      (wa.WD_SCHEDULE_PERSISTENT, 5, 8)])
+def test_batch_uneven_jobs_and_replays(ctx, schedule, super_tiles, kpt):
     sizes = [1, 100, 4096, 5000, 70001, 300000, 12289, 65536]
     st = wl.Stream(11)
     jobs, data = [], []
@@ -38,7 +40,7 @@ def test_batch_uneven_jobs_and_replays(ctx, schedule, super_tiles):
         data.append(recs)
     batch = wa.Batch(ctx, len(jobs))
     try:
-        batch.schedule(schedule, super_tiles)
+        batch.configure(schedule, super_tiles, kpt)
         batch.set(jobs)
         for replay in range(3):
             if replay == 2:  # new views, same clouds and states: the captured graph is replayed
@@ -76,17 +78,24 @@ def test_batch_64_scenes_32m(ctx):
         batch = wa.Batch(ctx, 64)
         batch.set(jobs)
         t = batch.tune(reps=3)
-        print("64 scenes, %d splats: phased/1 %.3f ms, phased/2 %.3f ms, persistent %.3f ms, kept %d/%d"
-              % (sum(sizes), t.ms_phased_1, t.ms_phased_2, t.ms_persistent, t.schedule, t.super_tiles), flush=True)
-        for which, sup in ((wa.WD_SCHEDULE_PERSISTENT, 2), (wa.WD_SCHEDULE_PHASED, 1), (wa.WD_SCHEDULE_PHASED, 2)):
-            batch.schedule(which, sup)
+        rep = t.report()
+        print("64 scenes, %d splats:" % sum(sizes), rep, flush=True)
+        assert len(rep["timed_ms"]) == wa.WD_TUNE_CONFIGS and min(rep["timed_ms"].values()) > 0
+        assert t.tile == 256 * t.keys_per_thread and (t.radix_bits, t.passes) == (8, 2)
+        kept = [x for x in t.timed if (x.schedule, x.super_tiles, x.keys_per_thread)
+                == (t.schedule, t.super_tiles, t.keys_per_thread)]
+        assert kept and kept[0].ms == min(x.ms for x in t.timed)
+        for which, sup, kpt in ((wa.WD_SCHEDULE_PERSISTENT, 8, 16), (wa.WD_SCHEDULE_PHASED, 1, 16),
+                                (wa.WD_SCHEDULE_PHASED, 2, 16), (wa.WD_SCHEDULE_PHASED, 1, 8),
+                                (wa.WD_SCHEDULE_PERSISTENT, 3, 8)):
+            batch.configure(which, sup, kpt)
             for cloud, view, state in jobs:  # stale results must not pass for new ones
                 state.sort(cloud, wl.view_for(data[0], (0.0, 0.0, 50.0), 64, 64) or view)
             batch.run()
             ctx.sync()
             assert batch.stuck() == 0
             for (cloud, view, state), recs in zip(jobs, data):
-                check_job(state, recs, view, "schedule %d/%d scene n=%d" % (which, sup, len(recs)))
+                check_job(state, recs, view, "schedule %d/%d/kpt%d scene n=%d" % (which, sup, kpt, len(recs)))
         batch.destroy()
     finally:
         for cloud, _, state in jobs:

@@ -34,8 +34,8 @@ EXPORTS = [
     "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_state_ellipses", "wd_state_ellipse_buffer",
     "wd_state_read_ellipses", "wd_pool_create", "wd_pool_device", "wd_pool_destroy", "wd_particles_sort",
     "wd_profile", "wd_profile_read", "wd_time_sorts",
-    "wd_batch_create", "wd_batch_schedule", "wd_batch_tune", "wd_batch_set", "wd_batch_run", "wd_batch_stuck",
-    "wd_batch_destroy",
+    "wd_batch_create", "wd_batch_schedule", "wd_batch_configure", "wd_batch_tune", "wd_batch_set", "wd_batch_run",
+    "wd_batch_stuck", "wd_batch_destroy",
     "wd_mlp_create", "wd_mlp_destroy", "wd_mlp_set", "wd_mlp_get", "wd_mlp_save", "wd_mlp_load",
     "wd_mlp_grad", "wd_mlp_train", "wd_mlp_predict", "wd_pose_features",
     "wd_learned_create", "wd_learned_run", "wd_learned_destroy", "wd_state_quantiles",
@@ -88,10 +88,30 @@ class Stats(C.Structure):
                 ("overflow_bins", C.c_uint32)]
 
 
+WD_TUNE_CONFIGS = 6
+
+
+class Tuned(C.Structure):
+    """wd_tuned: one configuration the autotuner timed."""
+    _fields_ = [("schedule", C.c_int), ("super_tiles", C.c_uint32), ("keys_per_thread", C.c_uint32),
+                ("ms", C.c_double)]
+
+
 class Tuning(C.Structure):
-    """wd_tuning: the autotuner's choice and the mean ms a run of each configuration."""
-    _fields_ = [("schedule", C.c_int), ("super_tiles", C.c_uint32), ("ms_phased_1", C.c_double),
-                ("ms_phased_2", C.c_double), ("ms_persistent", C.c_double)]
+    """wd_tuning: the autotuner's choice (schedule, work item, keys a thread, tile, digits) and the mean ms a
+    run of each configuration it timed."""
+    _fields_ = [("schedule", C.c_int), ("super_tiles", C.c_uint32), ("keys_per_thread", C.c_uint32),
+                ("tile", C.c_uint32), ("radix_bits", C.c_uint32), ("passes", C.c_uint32),
+                ("timed", Tuned * WD_TUNE_CONFIGS)]
+
+    def report(self):
+        """The choice and every timed configuration, as plain data."""
+        names = ["phased", "persistent"]
+        return {"schedule": names[self.schedule], "super_tiles": self.super_tiles,
+                "keys_per_thread": self.keys_per_thread, "tile": self.tile,
+                "radix": "%d x %d-bit" % (self.passes, self.radix_bits),
+                "timed_ms": {"%s/%d/kpt%d" % (names[t.schedule], t.super_tiles, t.keys_per_thread): t.ms
+                             for t in self.timed}}
 
 
 class WindingError(RuntimeError):
@@ -152,6 +172,7 @@ def lib():
             "wd_time_sorts": (I, [P, P, P, P, U32, P]),
             "wd_batch_create": (I, [P, U32, C.POINTER(P)]),
             "wd_batch_schedule": (I, [P, I, U32]),
+            "wd_batch_configure": (I, [P, I, U32, U32]),
             "wd_batch_tune": (I, [P, P, U32, C.POINTER(Tuning)]),
             "wd_batch_set": (I, [P, C.POINTER(Job), U32]),
             "wd_batch_run": (I, [P, P]),
@@ -604,6 +625,10 @@ class Batch:
         """Phased: super_tiles (1 or 2) tiles a scatter workgroup takes at once. Persistent: tiles an
         item takes one after the other (1 to 64)."""
         check(lib().wd_batch_schedule(self.ptr, which, super_tiles))
+
+    def configure(self, which, super_tiles, keys_per_thread):
+        """The schedule and work item as schedule(), and the tile: 8 or 16 keys a thread (2048 or 4096)."""
+        check(lib().wd_batch_configure(self.ptr, which, super_tiles, keys_per_thread))
 
     def tune(self, reps=5):
         """Times the configurations on the set jobs and keeps the fastest: a Tuning."""

@@ -34,6 +34,9 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Werr
               "-D__HIP_PLATFORM_AMD__"]
 
 HIP_SOURCES = ["splat_sort.hip", "batch.hip", "mlp.hip", "learned.hip", "decode.hip"]
+# Sources built once more with other defines: the batch kernels at 8 keys a thread (tiles of 2048),
+# beside the default 16 (4096), for the batch autotuner.
+HIP_VARIANTS = [("batch.hip", "batch.kpt8.hip.o", ["-DWD_KEYS_PER_THREAD=8"])]
 HOST_SOURCES = ["api.cpp", "synth.cpp", "batch_host.cpp", "mlp_host.cpp", "learned_host.cpp", "decode_host.cpp"]
 
 
@@ -53,6 +56,10 @@ def _objects():
         if os.path.exists(src):
             obj = os.path.join(BUILD, s + ".o")
             jobs.append(([HIPCC] + HIP_FLAGS + inc + ["-c", src, "-o", obj], obj))
+    for s, obj_name, defines in HIP_VARIANTS:
+        src = os.path.join(CSRC, s)
+        obj = os.path.join(BUILD, obj_name)
+        jobs.append(([HIPCC] + HIP_FLAGS + defines + inc + ["-c", src, "-o", obj], obj))
     for s in HOST_SOURCES:
         src = os.path.join(CSRC, s)
         if os.path.exists(src):

@@ -373,7 +373,7 @@ int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
   wd_state *s = new wd_state();
   s->ctx = ctx;
   s->capacity = capacity;
-  s->tiles = (capacity + wd::kTile - 1) / wd::kTile;
+  s->tiles = (capacity + wd::kTileMin - 1) / wd::kTileMin;
   const uint64_t words = (capacity + 63) / 64;
   hipError_t e = hipSuccess;
   auto alloc = [&](void **p, size_t bytes) {

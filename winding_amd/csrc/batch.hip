@@ -285,6 +285,11 @@ __global__ __launch_bounds__(256) void kb_scan2(const BatchJob *__restrict__ job
 
 }  // namespace
 
+// This build's launchers: wd::kpt8 or wd::kpt16 by its keys a thread.
+#define WD_NS_CAT2(a, b) a##b
+#define WD_NS_CAT(a, b) WD_NS_CAT2(a, b)
+namespace WD_NS_CAT(kpt, WD_KEYS_PER_THREAD) {
+
 int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid) {
   int per_cu = 0;
   WD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch, 256, 0), "batch: occupancy");
@@ -324,4 +329,5 @@ int launch_batch(wd_context *ctx, wd_batch *b) {
   return ctx->prof_end(kKBatch, ev);
 }
 
+}  // namespace kpt8 / kpt16
 }  // namespace wd

@@ -83,7 +83,8 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
     B.tilehist3 = s->tilehist3;
     B.prefix_lo = s->prefix_lo;
     B.prefix_hi = s->prefix_hi;
-    B.tiles = (B.n + wd::kTile - 1) / wd::kTile;
+    const uint32_t tile = 256u * b->kpt;
+    B.tiles = (B.n + tile - 1) / tile;
     if (b->schedule == WD_SCHEDULE_PERSISTENT) {
       // a persistent item is a run of super_tiles tiles (scatters: super-tiles of kSuper)
       const uint32_t R = b->super_tiles, supers = (B.tiles + wd::kSuper - 1) / wd::kSuper;
@@ -136,8 +137,15 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
   }
   b->n = n;
   b->total = total;
-  return same_total && b->grid ? WD_OK : wd::batch_grid(ctx, total, &b->grid);
+  if (same_total && b->grid) return WD_OK;
+  return b->kpt == 8 ? wd::kpt8::batch_grid(ctx, total, &b->grid) : wd::kpt16::batch_grid(ctx, total, &b->grid);
 }
+
+namespace {
+int launch_batch(wd_context *ctx, wd_batch *b) {
+  return b->kpt == 8 ? wd::kpt8::launch_batch(ctx, b) : wd::kpt16::launch_batch(ctx, b);
+}
+}  // namespace
 
 int wd_batch_run(wd_context *ctx, wd_batch *b) {
   if (!ctx || !b || b->magic != 0x42415443 || b->ctx != ctx) return fail(WD_E_STATE, "batch: destroyed or of another device");
@@ -155,10 +163,10 @@ int wd_batch_run(wd_context *ctx, wd_batch *b) {
     if (b->api_jobs[i].state->ellipse != b->ellipse_set[i])
       return fail(WD_E_STATE, "batch: a job's ellipses were turned on or off since the jobs were set; set the jobs again");
   WD_HIP(hipSetDevice(ctx->device), "batch: set device");
-  if (ctx->profiling) return wd::launch_batch(ctx, b);  // timed launches go straight to the stream
+  if (ctx->profiling) return launch_batch(ctx, b);  // timed launches go straight to the stream
   if (!b->exec) {
     WD_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "batch: capture");
-    int rc = wd::launch_batch(ctx, b);
+    int rc = launch_batch(ctx, b);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) {
@@ -173,7 +181,7 @@ int wd_batch_run(wd_context *ctx, wd_batch *b) {
   return WD_OK;
 }
 
-int wd_batch_schedule(wd_batch *b, int schedule, uint32_t super_tiles) {
+int wd_batch_configure(wd_batch *b, int schedule, uint32_t super_tiles, uint32_t keys_per_thread) {
   if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
   if (schedule != WD_SCHEDULE_PHASED && schedule != WD_SCHEDULE_PERSISTENT)
     return fail(WD_E_ARG, "batch: schedule %d is neither phased (0) nor persistent (1)", schedule);
@@ -181,17 +189,26 @@ int wd_batch_schedule(wd_batch *b, int schedule, uint32_t super_tiles) {
     return fail(WD_E_ARG, "batch: phased super-tiles of %u; 1 or 2", super_tiles);
   if (schedule == WD_SCHEDULE_PERSISTENT && (super_tiles < 1 || super_tiles > wd::kMaxRun))
     return fail(WD_E_ARG, "batch: persistent runs of %u tiles; 1 to %u", super_tiles, wd::kMaxRun);
-  if (schedule == b->schedule && super_tiles == b->super_tiles) return WD_OK;
+  if (keys_per_thread != 8 && keys_per_thread != 16)
+    return fail(WD_E_ARG, "batch: %u keys a thread; 8 or 16", keys_per_thread);
+  if (schedule == b->schedule && super_tiles == b->super_tiles && keys_per_thread == b->kpt) return WD_OK;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
+  b->grid = 0;  // the persistent kernel's occupancy is the build's
   b->schedule = schedule;
   b->super_tiles = super_tiles;
+  b->kpt = keys_per_thread;
   if (b->n == 0) return WD_OK;
-  // the item starts depend on the super-tile size: set the same jobs again
+  // the item starts depend on the work item and the tile size: set the same jobs again
   const std::vector<wd_job> jobs = b->api_jobs;
   return wd_batch_set(b, jobs.data(), (uint32_t)jobs.size());
+}
+
+int wd_batch_schedule(wd_batch *b, int schedule, uint32_t super_tiles) {
+  if (!b || b->magic != 0x42415443) return fail(WD_E_STATE, "batch: destroyed");
+  return wd_batch_configure(b, schedule, super_tiles, b->kpt);
 }
 
 int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, wd_tuning *out) {
@@ -199,15 +216,22 @@ int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, wd_tuning *out) {
   if (b->n == 0) return fail(WD_E_STATE, "batch: no jobs set");
   if (ctx->profiling) return fail(WD_E_STATE, "batch: tuning times whole runs; switch kernel profiling off first");
   reps = std::max(reps, 1u);
-  const int sched[3] = {WD_SCHEDULE_PHASED, WD_SCHEDULE_PHASED, WD_SCHEDULE_PERSISTENT};
-  const uint32_t sup[3] = {1, 2, wd::kPersistentRun};
-  double ms[3] = {0, 0, 0};
+  // schedule, work item, keys a thread (tiles of 256 x keys)
+  struct Config {
+    int schedule;
+    uint32_t item, kpt;
+  };
+  static const Config cfg[WD_TUNE_CONFIGS] = {
+      {WD_SCHEDULE_PHASED, 1, 16}, {WD_SCHEDULE_PHASED, 2, 16}, {WD_SCHEDULE_PHASED, 1, 8},
+      {WD_SCHEDULE_PHASED, 2, 8},  {WD_SCHEDULE_PERSISTENT, wd::kPersistentRun, 16},
+      {WD_SCHEDULE_PERSISTENT, wd::kPersistentRun, 8}};
+  double ms[WD_TUNE_CONFIGS] = {};
   hipEvent_t a = nullptr, z = nullptr;
   WD_HIP(hipEventCreate(&a), "batch: tune events");
   hipError_t e = hipEventCreate(&z);
   int rc = WD_OK;
-  for (int c = 0; c < 3 && e == hipSuccess && rc == WD_OK; ++c) {
-    rc = wd_batch_schedule(b, sched[c], sup[c]);
+  for (int c = 0; c < WD_TUNE_CONFIGS && e == hipSuccess && rc == WD_OK; ++c) {
+    rc = wd_batch_configure(b, cfg[c].schedule, cfg[c].item, cfg[c].kpt);
     if (!rc) rc = wd_batch_run(ctx, b);  // capture and warm
     if (!rc) e = hipEventRecord(a, ctx->stream);
     for (uint32_t r = 0; r < reps && !rc && e == hipSuccess; ++r) rc = wd_batch_run(ctx, b);
@@ -222,15 +246,22 @@ int wd_batch_tune(wd_context *ctx, wd_batch *b, uint32_t reps, wd_tuning *out) {
   if (rc) return rc;
   if (e != hipSuccess) return wd::hip_fail(e, "batch: tune");
   int best = 0;
-  for (int c = 1; c < 3; ++c)
+  for (int c = 1; c < WD_TUNE_CONFIGS; ++c)
     if (ms[c] < ms[best]) best = c;
-  if ((rc = wd_batch_schedule(b, sched[best], sup[best]))) return rc;
+  if ((rc = wd_batch_configure(b, cfg[best].schedule, cfg[best].item, cfg[best].kpt))) return rc;
   if (out) {
-    out->schedule = sched[best];
-    out->super_tiles = sup[best];
-    out->ms_phased_1 = ms[0];
-    out->ms_phased_2 = ms[1];
-    out->ms_persistent = ms[2];
+    out->schedule = cfg[best].schedule;
+    out->super_tiles = cfg[best].item;
+    out->keys_per_thread = cfg[best].kpt;
+    out->tile = 256u * cfg[best].kpt;
+    out->radix_bits = 8;
+    out->passes = 2;
+    for (int c = 0; c < WD_TUNE_CONFIGS; ++c) {
+      out->timed[c].schedule = cfg[c].schedule;
+      out->timed[c].super_tiles = cfg[c].item;
+      out->timed[c].keys_per_thread = cfg[c].kpt;
+      out->timed[c].ms = ms[c];
+    }
   }
   return WD_OK;
 }

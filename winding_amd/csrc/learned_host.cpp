@@ -131,7 +131,7 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
            "learned: item table");
     WD_HIP(hipStreamSynchronize(ctx->stream), "learned: item table");
     const uint32_t items = n_views * (2 * runs + 2 * sruns + 512 / wd::kBatchScanCols + 256 / wd::kBatchScanCols);
-    const int rc = wd::batch_grid(ctx, items, &l->grid);
+    const int rc = wd::kpt16::batch_grid(ctx, items, &l->grid);
     if (rc) return rc;
     l->last_views = n_views;
     l->last_tiles = tiles;
@@ -181,7 +181,8 @@ int wd_learned_run(wd_context *ctx, wd_learned *l, wd_mlp *mlp, const wd_cloud *
   l->staged_used[f] = true;
   l->flip ^= 1;
   if ((rc = wd::launch_learned(ctx, l->jobs, n_views, n, all_std, l->gate))) return rc;
-  return wd::launch_persistent(ctx, l->jobs, l->starts, n_views, l->max_views, l->bctl, l->grid, l->gate, wd::kPersistentRun);
+  return wd::kpt16::launch_persistent(ctx, l->jobs, l->starts, n_views, l->max_views, l->bctl, l->grid, l->gate,
+                                      wd::kPersistentRun);
 }
 
 int wd_learned_destroy(wd_learned *l) {

@@ -26,13 +26,16 @@ int hip_fail(hipError_t e, const char *what);
   } while (0)
 
 constexpr int kThreads = 256;        // workgroup size of every kernel here
-// Keys a thread ranks in a scatter tile (a build option: 8, 16 or 32).
+// Keys a thread ranks in a scatter tile. Set per translation unit by the
+// build: 16 everywhere, and batch.hip is built a second time at 8 (its
+// launchers then live in namespace wd::kpt8, the default's in wd::kpt16).
 #ifndef WD_KEYS_PER_THREAD
 #define WD_KEYS_PER_THREAD 16
 #endif
 constexpr int kKeysPerThread = WD_KEYS_PER_THREAD;
 static_assert(kKeysPerThread == 8 || kKeysPerThread == 16 || kKeysPerThread == 32, "8, 16 or 32 keys a thread");
 constexpr int kTile = 256 * kKeysPerThread;  // splats per projection chunk and scatter tile
+constexpr int kTileMin = 256 * 8;  // the smallest tile any kernel here takes: a state's per-tile buffers hold this many
 constexpr uint32_t kScanCols = 4;    // histogram columns a scan workgroup takes (a single sort)
 constexpr uint32_t kBatchScanCols = 8;  // the same in a batch's scans
 
@@ -271,7 +274,8 @@ struct wd_batch {
   hipGraphExec_t exec = nullptr;
   uint32_t grid = 0, total = 0;
   int schedule = WD_SCHEDULE_PHASED;
-  uint32_t super_tiles = 1;                     // the scatter work item: tiles a workgroup (1 or 2)
+  uint32_t super_tiles = 1;  // phased: tiles a scatter workgroup takes at once (1 or 2); persistent: tiles an item
+  uint32_t kpt = 16;         // keys a thread, 8 or 16: tiles of 256 kpt splats
   std::vector<wd_job> api_jobs;                 // the jobs as set, to rebuild the tables on a change
   std::vector<std::pair<const uint32_t *, uint32_t>> sh_set;  // each job's harmonics (words, degree) as set
   std::vector<const float *> ellipse_set;                      // each job's ellipse buffer as set
@@ -294,12 +298,12 @@ struct wd_state {
   uint32_t *order = nullptr;   // the draw order
   float *cov = nullptr;        // n x 4 floats
   float *ellipse = nullptr;    // n x 8 floats when the state takes the draw's ellipses, else null
-  uint16_t *tilehist = nullptr;   // per 4096-splat chunk: 256 low-byte then 256 high-byte counts
+  uint16_t *tilehist = nullptr;   // per chunk (one tile): 256 low-byte then 256 high-byte counts
   uint16_t *tilehist3 = nullptr;  // per tile of the low-byte pass's output: 256 high-byte counts
   uint32_t *prefix_lo = nullptr, *prefix_hi = nullptr;  // [tiles][256]: where a tile's digit run starts
   uint16_t *lut = nullptr;                             // learned sort: 16-key block -> bin and offset
   uint32_t *prefix_bin = nullptr;                      // learned sort: [tiles][512]
-  uint64_t tiles = 0;
+  uint64_t tiles = 0;  // per-tile buffers hold this many tiles of kTileMin splats
   uint32_t magic = 0x53544154;
 };
 
@@ -346,7 +350,6 @@ struct SortBuffers {
 int launch_sort_into(wd_context *ctx, const SortInput &in, const wd_view &view, const SortBuffers &b);
 // The no-allocation sort's scratch on the current device (hipGetSymbolAddress of the array).
 int noalloc_scratch(void **ptr);
-int launch_batch(wd_context *ctx, wd_batch *b);
 // ---- capture decoders (decode.hip, decode_host.cpp) ----
 // Where a decoder writes: the cloud's centres (x, y, z, rgba bits), Sigma (six
 // floats), harmonics (sh_words words a splat, zeroed before), the least index
@@ -397,9 +400,22 @@ int launch_decode_sog(wd_context *ctx, const SogLayout &L, const DecodeOut &o);
 // edges: the 4096 FP64 bucket edges on the device).
 int launch_particle_sort(wd_context *ctx, const uint8_t *pool, uint64_t first, uint32_t n, const double *view6,
                          const double *edges, wd_state *st);
-// The persistent batch kernel over a job table; skipped unless *gate != 0 when gate is given.
-int launch_persistent(wd_context *ctx, const BatchJob *jobs, const uint32_t *starts, uint32_t nj, uint32_t max_jobs,
-                      BatchCtl *bctl, uint32_t grid, const uint32_t *gate, uint32_t run);
+// The batch kernels (batch.hip), one set a keys-a-thread build. launch_batch:
+// one run of a batch on its schedule. launch_persistent: the persistent
+// kernel over a job table, skipped unless *gate != 0 when gate is given.
+// batch_grid: the persistent kernel's grid for `items` items.
+#define WD_BATCH_LAUNCHERS                                                                                     \
+  int launch_batch(wd_context *ctx, wd_batch *b);                                                              \
+  int launch_persistent(wd_context *ctx, const BatchJob *jobs, const uint32_t *starts, uint32_t nj,            \
+                        uint32_t max_jobs, BatchCtl *bctl, uint32_t grid, const uint32_t *gate, uint32_t run); \
+  int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid);
+namespace kpt8 {
+WD_BATCH_LAUNCHERS
+}
+namespace kpt16 {
+WD_BATCH_LAUNCHERS
+}
+#undef WD_BATCH_LAUNCHERS
 // The 257 equal-mass edges of a sorted state's keys (learned.hip).
 int launch_quantiles(wd_context *ctx, const wd_state *s, uint32_t *edges_dev);
 // The learned sort's kernels (learned.hip) over nv views of one n-splat cloud.
@@ -407,7 +423,6 @@ int launch_learned(wd_context *ctx, const BatchJob *jobs, uint32_t nv, uint32_t 
 // The predictor's forward pass over `rows` poses already in device memory;
 // *probs gets the device address of the rows x 256 softmax (mlp_host.cpp).
 int mlp_infer(wd_context *ctx, wd_mlp *m, const float *poses_dev, uint32_t rows, const float **probs);
-int batch_grid(wd_context *ctx, uint32_t items, uint32_t *grid);
 int launch_copy(wd_context *ctx, const void *src, void *dst, size_t bytes);
 // Predictor kernels (mlp.hip).
 int launch_gemm(wd_context *ctx, const MlpGemm &g);

@@ -716,7 +716,8 @@ static napi_value js_batch_run(napi_env env, napi_callback_info info) {
   return rc ? throw_wd(env, rc) : NULL;
 }
 
-/* batchTune(context, batch, reps) -> {schedule: 'phased'|'persistent', superTiles, msPhased1, msPhased2, msPersistent} */
+/* batchTune(context, batch, reps) -> {schedule: 'phased'|'persistent', superTiles, keysPerThread, tile, radixBits,
+ * passes, timed: [{schedule, superTiles, keysPerThread, ms}, ...]} */
 static napi_value js_batch_tune(napi_env env, napi_callback_info info) {
   napi_value argv[3], out, sv;
   uint32_t reps;
@@ -732,9 +733,24 @@ static napi_value js_batch_tune(napi_env env, napi_callback_info info) {
   napi_create_string_utf8(env, t.schedule == WD_SCHEDULE_PERSISTENT ? "persistent" : "phased", NAPI_AUTO_LENGTH, &sv);
   napi_set_named_property(env, out, "schedule", sv);
   napi_set_named_property(env, out, "superTiles", num(env, t.super_tiles));
-  napi_set_named_property(env, out, "msPhased1", num(env, t.ms_phased_1));
-  napi_set_named_property(env, out, "msPhased2", num(env, t.ms_phased_2));
-  napi_set_named_property(env, out, "msPersistent", num(env, t.ms_persistent));
+  napi_set_named_property(env, out, "keysPerThread", num(env, t.keys_per_thread));
+  napi_set_named_property(env, out, "tile", num(env, t.tile));
+  napi_set_named_property(env, out, "radixBits", num(env, t.radix_bits));
+  napi_set_named_property(env, out, "passes", num(env, t.passes));
+  napi_value list;
+  CHECK_NAPI(napi_create_array_with_length(env, WD_TUNE_CONFIGS, &list));
+  for (uint32_t i = 0; i < WD_TUNE_CONFIGS; ++i) {
+    napi_value e, es;
+    CHECK_NAPI(napi_create_object(env, &e));
+    napi_create_string_utf8(env, t.timed[i].schedule == WD_SCHEDULE_PERSISTENT ? "persistent" : "phased",
+                            NAPI_AUTO_LENGTH, &es);
+    napi_set_named_property(env, e, "schedule", es);
+    napi_set_named_property(env, e, "superTiles", num(env, t.timed[i].super_tiles));
+    napi_set_named_property(env, e, "keysPerThread", num(env, t.timed[i].keys_per_thread));
+    napi_set_named_property(env, e, "ms", num(env, t.timed[i].ms));
+    CHECK_NAPI(napi_set_element(env, list, i, e));
+  }
+  napi_set_named_property(env, out, "timed", list);
   return out;
 }
 
