@@ -179,3 +179,34 @@ def test_a_stuck_persistent_run_fails_its_states_once(ctx):
         for cloud, _, state in jobs:
             cloud.destroy()
             state.destroy()
+
+
+def test_batch_job_past_the_held_scan_rows(ctx):
+    """A job of 9,000,000 splats beside a small one: at tiles of 4096 its 2,198 tiles are more than
+    the 8 rows a scan thread holds (256 x 8 = 2,048 tiles), and at tiles of 2048 more again, so the
+    batch scans take the path that reads the rows twice. Every schedule and tile, against the oracle."""
+    sizes = [9_000_000, 3000]
+    st = wl.Stream(31)
+    jobs, data = [], []
+    try:
+        for j, n in enumerate(sizes):
+            recs = wa.synth_records(j % 2, 700 + j, n)
+            view = wl.view_for(recs, wl.orbit_pose(st), 1920, 1080) or wl.view_for(recs, (0.0, 0.0, 10.0), 1920, 1080)
+            jobs.append((wa.Cloud(ctx, recs), view, wa.State(ctx, n)))
+            data.append(recs)
+        batch = wa.Batch(ctx, len(jobs))
+        batch.set(jobs)
+        for which, sup, kpt in ((wa.WD_SCHEDULE_PHASED, 1, 16), (wa.WD_SCHEDULE_PHASED, 2, 8),
+                                (wa.WD_SCHEDULE_PERSISTENT, 8, 16), (wa.WD_SCHEDULE_PERSISTENT, 4, 8)):
+            batch.configure(which, sup, kpt)
+            batch.run()
+            ctx.sync()
+            assert batch.stuck() == 0
+            print("schedule %d/%d/kpt%d ran" % (which, sup, kpt), flush=True)
+            for (cloud, view, state), recs in zip(jobs, data):
+                check_job(state, recs, view, "schedule %d/%d/kpt%d n=%d" % (which, sup, kpt, len(recs)))
+        batch.destroy()
+    finally:
+        for cloud, _, state in jobs:
+            cloud.destroy()
+            state.destroy()
