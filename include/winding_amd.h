@@ -51,8 +51,14 @@ extern "C" {
 #define WD_BUCKETS 65536
 /* Default cull, opacity x pixels. src/render/splats.js:78 */
 #define WD_SPLAT_CULL 0.5f
-/* Most splats one sort takes: the stable scatter packs an index in 24 bits. */
-#define WD_MAX_SORT (1u << 24)
+/*
+ * Most splats one cloud and one sort take. Up to 2^24 the stable scatter
+ * carries high byte << 24 | index between its passes (4 bytes a survivor);
+ * past that, 8 bytes. A batch job and a particle pool take up to 2^24.
+ */
+#define WD_MAX_SORT (1u << 28)
+/* Most splats a batch job takes, and most particles a pool holds. */
+#define WD_MAX_NARROW (1u << 24)
 /* Coarse depth bins the learned predictor works in: the key's high byte. */
 #define WD_COARSE_BINS 256
 /* Pose features the predictor reads: the model-view's top three rows. */
@@ -328,6 +334,7 @@ int wd_state_read_ellipses(wd_context *ctx, wd_state *state, float *out,
  * reference's `particles` buffer (src/render/particles.js:720-730). A slot
  * whose age is not below its lifetime is dead (a zeroed slot is). The device
  * address (wd_pool_device) is for a simulation that writes the pool in place.
+ * Refuses more than WD_MAX_NARROW particles by name.
  */
 int wd_pool_create(wd_context *ctx, const void *particles, uint64_t count,
                    wd_pool **out);

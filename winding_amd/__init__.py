@@ -17,7 +17,8 @@ WD_OK = 0
 WD_RECORD_BYTES = 32
 WD_BUCKETS = 65536
 WD_SPLAT_CULL = 0.5
-WD_MAX_SORT = 1 << 24
+WD_MAX_SORT = 1 << 28
+WD_MAX_NARROW = 1 << 24
 WD_NOALLOC_MAX_SPLATS = 1 << 21
 WD_COARSE_BINS = 256
 WD_POSE_FLOATS = 12

@@ -383,7 +383,7 @@ int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
   alloc((void **)&s->ctl, sizeof(wd::Ctl));
   alloc((void **)&s->keys, capacity * sizeof(uint16_t));
   alloc((void **)&s->mask, words * sizeof(uint64_t));
-  alloc((void **)&s->packed, capacity * sizeof(uint32_t));
+  alloc((void **)&s->packed, capacity * (capacity > wd::kNarrowMax ? sizeof(uint64_t) : sizeof(uint32_t)));
   alloc((void **)&s->order, capacity * sizeof(uint32_t));
   alloc((void **)&s->cov, capacity * 4 * sizeof(float));
   alloc((void **)&s->tilehist, s->tiles * 512 * sizeof(uint16_t));
@@ -427,9 +427,9 @@ int wd_state_buffers(wd_state *s, void **order, void **count, void **cov2d) {
 int wd_pool_create(wd_context *ctx, const void *particles, uint64_t count, wd_pool **out) {
   if (!ctx || !particles || !out) return fail(WD_E_ARG, "particles: null argument");
   *out = nullptr;
-  if (count == 0 || count > WD_MAX_SORT)
+  if (count == 0 || count > WD_MAX_NARROW)
     return fail(WD_E_CAPACITY, "particles: a pool of %llu particles; this sort takes 1 to %u", (unsigned long long)count,
-                WD_MAX_SORT);
+                WD_MAX_NARROW);
   WD_HIP(hipSetDevice(ctx->device), "particles: set device");
   wd_pool *p = new wd_pool();
   p->ctx = ctx;

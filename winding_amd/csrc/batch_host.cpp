@@ -56,6 +56,9 @@ int wd_batch_set(wd_batch *b, const wd_job *jobs, uint32_t n) {
       return fail(WD_E_ARG, "batch: job %u: splats with harmonics or given as centres and covariances, and states "
                   "that take the draw's ellipses, are batched under the reverse-Z perspective only; sort them with "
                   "wd_sort", i);
+    if (J.cloud->n > wd::kNarrowMax)
+      return fail(WD_E_CAPACITY, "batch: job %u has %llu splats; a batch job takes up to %u, sort it with wd_sort", i,
+                  (unsigned long long)J.cloud->n, wd::kNarrowMax);
     if (J.cloud->n > J.state->capacity)
       return fail(WD_E_CAPACITY, "batch: job %u: %llu splats are more than its state holds, %llu", i,
                   (unsigned long long)J.cloud->n, (unsigned long long)J.state->capacity);

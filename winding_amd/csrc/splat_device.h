@@ -702,35 +702,49 @@ struct HistSmem {
   uint32_t h[4][256];
 };
 
-__device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32_t *__restrict__ packed,
+// Pack: the low-byte pass's element, high byte << 24 | index (uint32_t), or,
+// for clouds past 2^24 splats, high byte << 32 | index (uint64_t).
+template <typename Pack>
+__device__ __forceinline__ uint32_t pack_digit(Pack v) {
+  return sizeof(Pack) == 8 ? (uint32_t)((uint64_t)v >> 32) : (uint32_t)v >> 24;
+}
+
+template <typename Pack>
+__device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const Pack *__restrict__ packed,
                                           const Ctl *__restrict__ ctl, uint16_t *__restrict__ tilehist3) {
   constexpr int KPT = kKeysPerThread;
   const uint32_t tid = threadIdx.x, w = tid >> 6;
   const uint32_t limit = ctl->count, tile0 = t * (uint32_t)kTile;
   if (tile0 >= limit) return;
-  // Counting needs no order: thread tid takes 16 consecutive elements, in four 16-byte loads.
-  uint32_t raw[KPT];
+  // Counting needs no order: thread tid takes 16 consecutive elements, in 16-byte loads.
+  uint32_t raw[KPT];  // the digits
   const uint32_t first = tile0 + tid * (uint32_t)KPT;
   if (first + KPT <= limit) {
     const uint4 *q = reinterpret_cast<const uint4 *>(packed + first);
+    constexpr int kPer = 16 / (int)sizeof(Pack);  // elements a 16-byte load
 #pragma unroll
-    for (int k = 0; k < KPT / 4; ++k) {
+    for (int k = 0; k < KPT / kPer; ++k) {
       const uint4 v = q[k];
-      raw[4 * k] = v.x;
-      raw[4 * k + 1] = v.y;
-      raw[4 * k + 2] = v.z;
-      raw[4 * k + 3] = v.w;
+      if (sizeof(Pack) == 8) {
+        raw[2 * k] = v.y;
+        raw[2 * k + 1] = v.w;
+      } else {
+        raw[4 * k] = v.x >> 24;
+        raw[4 * k + 1] = v.y >> 24;
+        raw[4 * k + 2] = v.z >> 24;
+        raw[4 * k + 3] = v.w >> 24;
+      }
     }
   } else {
 #pragma unroll
-    for (int r = 0; r < KPT; ++r) raw[r] = first + r < limit ? packed[first + r] : 0u;
+    for (int r = 0; r < KPT; ++r) raw[r] = first + r < limit ? pack_digit(packed[first + r]) : 0u;
   }
   __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 1024; k += 256) (&sm.h[0][0])[k] = 0;
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < KPT; ++r)
-    if (first + r < limit) atomicAdd(&sm.h[w][raw[r] >> 24], 1u);
+    if (first + r < limit) atomicAdd(&sm.h[w][raw[r]], 1u);
   __syncthreads();
   tilehist3[(size_t)t * 256 + tid] = (uint16_t)(sm.h[0][tid] + sm.h[1][tid] + sm.h[2][tid] + sm.h[3][tid]);
 }
@@ -748,9 +762,10 @@ __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const uint32
 // One LDS word an element: digit << 24 | payload. HI: the payload is the
 // index. LO: the high byte << kLocalBits | the index within the super-tile.
 // 40 KB at S = 2, four workgroups a CU.
-template <int S>
+template <int S, bool WIDE = false>
 struct ScatterSmem {
   uint32_t ent[S * kTile];
+  uint8_t dig[WIDE ? S * kTile : 4];  // WIDE, high-byte pass: each slot's digit (ent holds the index)
   uint32_t wh[4][256];   // per-wave running digit counts, then wave offsets
   uint32_t sub[S][256];  // each tile's start of digit d in the super-tile's LDS order
   uint32_t shift[256];   // global start - LDS start of digit d
@@ -758,14 +773,19 @@ struct ScatterSmem {
   uint32_t nvalid;
 };
 
-template <bool HI, int S>
-__device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, const uint16_t *__restrict__ keys,
-                                              const uint64_t *__restrict__ mask,
-                                              const uint32_t *__restrict__ in_packed, uint32_t n,
-                                              const uint16_t *__restrict__ hist, uint32_t hstride,
+// WIDE: the elements between the passes are 64-bit (high byte << 32 |
+// index), so indices may pass 2^24; the high-byte pass then keeps each slot's
+// digit in its own LDS byte. out: the low-byte pass's elements (Pack) or the
+// draw order (uint32_t).
+template <bool HI, int S, bool WIDE = false>
+__device__ __forceinline__ void scatter_super(ScatterSmem<S, WIDE> &sm, uint32_t st,
+                                              const uint16_t *__restrict__ keys, const uint64_t *__restrict__ mask,
+                                              const typename std::conditional<WIDE, uint64_t, uint32_t>::type
+                                                  *__restrict__ in_packed,
+                                              uint32_t n, const uint16_t *__restrict__ hist, uint32_t hstride,
                                               const uint32_t *__restrict__ prefix, const uint32_t *__restrict__ total,
-                                              Ctl *__restrict__ ctl, uint32_t *__restrict__ out,
-                                              uint32_t out_base = 0) {
+                                              Ctl *__restrict__ ctl, void *__restrict__ out_v, uint32_t out_base = 0) {
+  using Pack = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
   constexpr int KPT = kKeysPerThread;
   constexpr uint32_t kLocalBits = S * kTile <= 8192 ? 13u : 14u;
   static_assert(S * kTile <= 16384, "index within the super-tile in 14 bits");
@@ -779,7 +799,12 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
   const uint32_t nwords = (n + 63u) >> 6;
   // Every global read of the item is issued here, before any is waited on:
   // the tiles' elements (and survivor words), the digit counts, the starts.
-  uint32_t raw[S][KPT];
+  uint32_t raw[S][KPT];  // HI: the element (WIDE: its index); LO: the key
+  uint32_t dgt[S][KPT];  // HI and WIDE: the element's digit
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) dgt[s][r] = 0u;
   uint64_t vm[S];  // LO: lane k < 16 holds the survivor word of round k of this wave
   uint32_t cnt[S];
 #pragma unroll
@@ -788,7 +813,13 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
       const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-      raw[s][r] = HI ? (i < limit ? in_packed[i] : 0u) : (i < n ? (uint32_t)keys[i] : 0u);
+      if (HI) {
+        const Pack v = i < limit ? in_packed[i] : (Pack)0;
+        raw[s][r] = (uint32_t)v;
+        if (WIDE) dgt[s][r] = pack_digit(v);
+      } else {
+        raw[s][r] = i < n ? (uint32_t)keys[i] : 0u;
+      }
     }
     if (!HI) {
       const uint32_t wi = (wbase >> 6) + lane;
@@ -834,7 +865,7 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
         uint32_t dd;
         if (HI) {
           valid = FULL ? ~0ull : __ballot(i < limit);
-          dd = raw[s][r] >> 24;
+          dd = WIDE ? dgt[s][r] : raw[s][r] >> 24;
         } else {
           const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)vm[s], r);
           const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(vm[s] >> 32), r);
@@ -868,14 +899,18 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
     // every slot read first, then every write: no read waits behind a write
     uint32_t pos[KPT];
 #pragma unroll
-    for (int r = 0; r < KPT; ++r) pos[r] = sm.wh[w][HI ? raw[s][r] >> 24 : raw[s][r] & 255u] + rank[r];
+    for (int r = 0; r < KPT; ++r)
+      pos[r] = sm.wh[w][HI ? (WIDE ? dgt[s][r] : raw[s][r] >> 24) : raw[s][r] & 255u] + rank[r];
 #pragma unroll
     for (int r = 0; r < KPT; ++r) {
       const uint32_t i = wbase + (uint32_t)r * 64u + lane;
-      const uint32_t dd = HI ? raw[s][r] >> 24 : raw[s][r] & 255u;
+      const uint32_t dd = HI ? (WIDE ? dgt[s][r] : raw[s][r] >> 24) : raw[s][r] & 255u;
       const uint32_t e = HI ? raw[s][r] : (dd << 24) | ((raw[s][r] >> 8) << kLocalBits) | (i - sbase);
       // pos is always in range when the counts agree; the test is a guard
-      if (((okbits >> r) & 1u) && pos[r] < (uint32_t)(S * kTile)) sm.ent[pos[r]] = e;
+      if (((okbits >> r) & 1u) && pos[r] < (uint32_t)(S * kTile)) {
+        sm.ent[pos[r]] = e;
+        if (HI && WIDE) sm.dig[pos[r]] = (uint8_t)dd;
+      }
     }
     __syncthreads();
   }
@@ -889,20 +924,28 @@ __device__ __forceinline__ void scatter_super(ScatterSmem<S> &sm, uint32_t st, c
       e[k] = p < nv ? sm.ent[p] : 0u;
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) g[k] = sm.shift[e[k] >> 24] + p0 + (uint32_t)k * 256u + tid;
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t p = p0 + (uint32_t)k * 256u + tid;
+      const uint32_t dd = HI && WIDE ? (p < nv ? (uint32_t)sm.dig[p] : 0u) : e[k] >> 24;
+      g[k] = sm.shift[dd] + p;
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const uint32_t p = p0 + (uint32_t)k * 256u + tid;
-      const uint32_t value = HI ? (e[k] & 0xFFFFFFu) + out_base
-                                : (((e[k] >> kLocalBits) & 255u) << 24) | (sbase + (e[k] & ((1u << kLocalBits) - 1u)));
       // n bounds both outputs; a guard against a miscount. The draw order is the
       // path's output: streamed out, so that the next launch does not start behind
       // its write-back.
       if (p < nv && g[k] < n) {
-        if (HI)
-          __builtin_nontemporal_store(value, &out[g[k]]);
-        else
-          out[g[k]] = value;
+        if (HI) {
+          const uint32_t index = (WIDE ? e[k] : e[k] & 0xFFFFFFu) + out_base;
+          __builtin_nontemporal_store(index, &static_cast<uint32_t *>(out_v)[g[k]]);
+        } else {
+          const uint32_t hb = (e[k] >> kLocalBits) & 255u, index = sbase + (e[k] & ((1u << kLocalBits) - 1u));
+          if (WIDE)
+            static_cast<uint64_t *>(out_v)[g[k]] = ((uint64_t)hb << 32) | index;
+          else
+            static_cast<uint32_t *>(out_v)[g[k]] = (hb << 24) | index;
+        }
       }
     }
   }

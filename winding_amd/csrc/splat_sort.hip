@@ -146,25 +146,29 @@ __global__ __launch_bounds__(256) void k_scan(uint32_t T, const uint16_t *__rest
 
 // Pass 2 and 3: a workgroup a super-tile. count_out (the low-byte pass, when
 // given) gets V as well as the control block does.
-template <bool HI>
+// WIDE (clouds past 2^24 splats): the elements between the passes are 64-bit.
+template <bool HI, bool WIDE>
 __global__ __launch_bounds__(256) void k_scatter(const uint16_t *__restrict__ keys, const uint64_t *__restrict__ mask,
-                                                 const uint32_t *__restrict__ in_packed, uint32_t n,
+                                                 const void *__restrict__ in_packed, uint32_t n,
                                                  const uint16_t *__restrict__ hist, uint32_t hstride,
                                                  const uint32_t *__restrict__ prefix, Ctl *__restrict__ ctl,
-                                                 uint32_t *__restrict__ out, uint32_t out_base,
+                                                 void *__restrict__ out, uint32_t out_base,
                                                  uint32_t *__restrict__ count_out) {
-  __shared__ ScatterSmem<kSuper> sm;
+  using Pack = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+  __shared__ ScatterSmem<kSuper, WIDE> sm;
   const uint32_t st = xcd_item(blockIdx.x, gridDim.x);
-  scatter_super<HI, kSuper>(sm, st, keys, mask, in_packed, n, hist, hstride, prefix,
-                            HI ? ctl->total + 256 : ctl->total, ctl, out, out_base);
+  scatter_super<HI, kSuper, WIDE>(sm, st, keys, mask, static_cast<const Pack *>(in_packed), n, hist, hstride, prefix,
+                                  HI ? ctl->total + 256 : ctl->total, ctl, out, out_base);
   if (!HI && count_out && st == 0 && threadIdx.x == 0) *count_out = ctl->count;  // this thread wrote it
 }
 
 // Upsweep of pass 3: a workgroup a tile of pass 2's output.
-__global__ __launch_bounds__(256) void k_hist(const uint32_t *__restrict__ packed, const Ctl *__restrict__ ctl,
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_hist(const void *__restrict__ packed, const Ctl *__restrict__ ctl,
                                               uint16_t *__restrict__ tilehist3) {
+  using Pack = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
   __shared__ HistSmem sm;
-  hist_tile(sm, blockIdx.x, packed, ctl, tilehist3);
+  hist_tile(sm, blockIdx.x, static_cast<const Pack *>(packed), ctl, tilehist3);
 }
 
 // Scan of pass 3: high-byte tile prefixes over the tiles of V.
@@ -390,16 +394,19 @@ int launch_sort_into(wd_context *ctx, const SortInput &in, const wd_view &view, 
             b.capacity, (const uint2 *)in.covs);
   WD_LAUNCH(kKScan, "sort: scan launch", k_scan, dim3(512 / kScanCols), dim3(256), 0, ctx->stream, tiles, b.tilehist,
             b.prefix_lo, b.ctl);
-  WD_LAUNCH(kKScatterLo, "sort: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0, ctx->stream,
-            b.keys, b.mask, (const uint32_t *)nullptr, n, b.tilehist, 512u, b.prefix_lo, b.ctl, b.packed, 0u,
-            b.count_out);
-  WD_LAUNCH(kKHist, "sort: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, b.packed, b.ctl,
-            b.tilehist3);
+  // past 2^24 splats the elements between the passes carry the index in 32 bits of their own
+  const bool wide = n > kNarrowMax;
+  WD_LAUNCH(kKScatterLo, "sort: low-byte pass launch", (wide ? k_scatter<false, true> : k_scatter<false, false>),
+            dim3(supers), dim3(256), 0, ctx->stream, b.keys, b.mask, (const void *)nullptr, n, b.tilehist, 512u,
+            b.prefix_lo, b.ctl, (void *)b.packed, 0u, b.count_out);
+  WD_LAUNCH(kKHist, "sort: histogram launch", (wide ? k_hist<true> : k_hist<false>), dim3(tiles), dim3(256), 0,
+            ctx->stream, (const void *)b.packed, b.ctl, b.tilehist3);
   WD_LAUNCH(kKScan2, "sort: second scan launch", k_scan2, dim3(256 / kScanCols), dim3(256), 0, ctx->stream,
             b.tilehist3, b.prefix_hi, b.ctl);
-  WD_LAUNCH(kKScatterHi, "sort: high-byte pass launch", k_scatter<true>, dim3(supers), dim3(256), 0, ctx->stream,
-            (const uint16_t *)nullptr, (const uint64_t *)nullptr, b.packed, n, b.tilehist3, 256u, b.prefix_hi, b.ctl,
-            b.order, 0u, (uint32_t *)nullptr);
+  WD_LAUNCH(kKScatterHi, "sort: high-byte pass launch", (wide ? k_scatter<true, true> : k_scatter<true, false>),
+            dim3(supers), dim3(256), 0, ctx->stream, (const uint16_t *)nullptr, (const uint64_t *)nullptr,
+            (const void *)b.packed, n, b.tilehist3, 256u, b.prefix_hi, b.ctl, (void *)b.order, 0u,
+            (uint32_t *)nullptr);
 #undef WD_LAUNCH
   return WD_OK;
 }
@@ -453,16 +460,16 @@ int launch_particle_sort(wd_context *ctx, const uint8_t *pool, uint64_t first, u
             (const uint4 *)(pool + 32 * first), n, v, edges, st->keys, st->mask, st->tilehist);
   WD_LAUNCH(kKScan, "particles: scan launch", k_scan, dim3(512 / kScanCols), dim3(256), 0, ctx->stream, tiles,
             st->tilehist, st->prefix_lo, st->ctl);
-  WD_LAUNCH(kKScatterLo, "particles: low-byte pass launch", k_scatter<false>, dim3(supers), dim3(256), 0,
-            ctx->stream, st->keys, st->mask, (const uint32_t *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->ctl,
-            st->packed, 0u, (uint32_t *)nullptr);
-  WD_LAUNCH(kKHist, "particles: histogram launch", k_hist, dim3(tiles), dim3(256), 0, ctx->stream, st->packed,
-            st->ctl, st->tilehist3);
+  WD_LAUNCH(kKScatterLo, "particles: low-byte pass launch", (k_scatter<false, false>), dim3(supers), dim3(256), 0,
+            ctx->stream, st->keys, st->mask, (const void *)nullptr, n, st->tilehist, 512u, st->prefix_lo, st->ctl,
+            (void *)st->packed, 0u, (uint32_t *)nullptr);
+  WD_LAUNCH(kKHist, "particles: histogram launch", k_hist<false>, dim3(tiles), dim3(256), 0, ctx->stream,
+            (const void *)st->packed, st->ctl, st->tilehist3);
   WD_LAUNCH(kKScan2, "particles: second scan launch", k_scan2, dim3(256 / kScanCols), dim3(256), 0, ctx->stream,
             st->tilehist3, st->prefix_hi, st->ctl);
-  WD_LAUNCH(kKScatterHi, "particles: high-byte pass launch", k_scatter<true>, dim3(supers), dim3(256), 0,
-            ctx->stream, (const uint16_t *)nullptr, (const uint64_t *)nullptr, st->packed, n, st->tilehist3, 256u,
-            st->prefix_hi, st->ctl, st->order, (uint32_t)first, (uint32_t *)nullptr);
+  WD_LAUNCH(kKScatterHi, "particles: high-byte pass launch", (k_scatter<true, false>), dim3(supers), dim3(256), 0,
+            ctx->stream, (const uint16_t *)nullptr, (const uint64_t *)nullptr, (const void *)st->packed, n,
+            st->tilehist3, 256u, st->prefix_hi, st->ctl, (void *)st->order, (uint32_t)first, (uint32_t *)nullptr);
 #undef WD_LAUNCH
   return WD_OK;
 }

@@ -35,6 +35,9 @@ constexpr int kThreads = 256;        // workgroup size of every kernel here
 constexpr int kKeysPerThread = WD_KEYS_PER_THREAD;
 static_assert(kKeysPerThread == 8 || kKeysPerThread == 16 || kKeysPerThread == 32, "8, 16 or 32 keys a thread");
 constexpr int kTile = 256 * kKeysPerThread;  // splats per projection chunk and scatter tile
+// Clouds up to this many splats carry high byte << 24 | index between the two
+// passes (4 bytes); bigger ones carry 8 bytes (high byte << 32 | index).
+constexpr uint32_t kNarrowMax = WD_MAX_NARROW;
 constexpr int kTileMin = 256 * 8;  // the smallest tile any kernel here takes: a state's per-tile buffers hold this many
 constexpr uint32_t kScanCols = 4;    // histogram columns a scan workgroup takes (a single sort)
 constexpr uint32_t kBatchScanCols = 8;  // the same in a batch's scans
@@ -294,7 +297,7 @@ struct wd_state {
   wd::Ctl *ctl = nullptr;      // device
   uint16_t *keys = nullptr;    // n keys, every splat
   uint64_t *mask = nullptr;    // survivor bits, ceil(n/64) words
-  uint32_t *packed = nullptr;  // low-digit pass output: high byte << 24 | index
+  uint32_t *packed = nullptr;  // low-digit pass output: high byte << 24 | index (8 bytes an element past kNarrowMax)
   uint32_t *order = nullptr;   // the draw order
   float *cov = nullptr;        // n x 4 floats
   float *ellipse = nullptr;    // n x 8 floats when the state takes the draw's ellipses, else null
