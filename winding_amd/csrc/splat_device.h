@@ -386,14 +386,17 @@ struct Rgb {
   float r, g, b;
 };
 
+// A splat's harmonics row (sh_words(SHD) words), loaded with its record so
+// that the row's latency hides behind the tiles worked on meanwhile.
 template <int SHD>
-__device__ __forceinline__ uint32_t shade(const ViewK &v, const uint32_t *__restrict__ sh, uint32_t i, float px,
-                                          float py, float pz, uint32_t rgba) {
-  constexpr uint32_t W = sh_words(SHD);
-  const uint32_t *s = sh + (size_t)i * W;
-  uint32_t w[W];
-#pragma unroll
-  for (uint32_t k = 0; k < W; ++k) w[k] = s[k];
+struct ShRow {
+  uint32_t w[SHD > 0 ? sh_words(SHD) : 1];
+};
+
+template <int SHD>
+__device__ __forceinline__ uint32_t shade(const ViewK &v, const ShRow<SHD> &row, float px, float py, float pz,
+                                          uint32_t rgba) {
+  const uint32_t(&w)[SHD > 0 ? sh_words(SHD) : 1] = row.w;
   auto half = [](uint32_t bits) { return (float)__builtin_bit_cast(_Float16, (uint16_t)bits); };
   // coefficient k: three halves from the two words they lie across
   auto coef = [&](uint32_t k) -> Rgb {
@@ -515,9 +518,12 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
   // one tile in flight does at five.
   // SIG: rec holds the centres (16 bytes a splat) and sig the six covariance
   // floats (24 bytes a splat, three 8-byte loads); a is the centre, b and c Sigma.
+  // SHD > 0: the slot holds the splat's harmonics row too (loaded for every
+  // splat of the tile: the cull is not known yet).
   struct Slot {
     uint4 a, b;
     uint2 c;
+    ShRow<SHD> h;
   };
   auto load = [&](uint32_t tile, Slot &s) {
     const uint32_t i = (tile << 8) + tid;
@@ -530,6 +536,11 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       } else {
         s.a = rec[2 * (size_t)i];
         s.b = rec[2 * (size_t)i + 1];
+      }
+      if constexpr (SHD > 0) {
+        const uint32_t *row = sh + (size_t)i * sh_words(SHD);
+#pragma unroll
+        for (uint32_t k = 0; k < sh_words(SHD); ++k) s.h.w[k] = row[k];
       }
     }
   };
@@ -558,7 +569,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       // of the launch are written back before the next launch starts.
       uint32_t colour = rgba;
       if constexpr (SHD > 0)
-        if (p.keep) colour = shade<SHD>(v, sh, i, f32_of(a.x), f32_of(a.y), f32_of(a.z), rgba);
+        if (p.keep) colour = shade<SHD>(v, s.h, f32_of(a.x), f32_of(a.y), f32_of(a.z), rgba);
       const f32x4 c = p.keep ? f32x4{p.xx, p.xy, p.yy, __uint_as_float(colour)} : f32x4{0.f, 0.f, 0.f, 0.f};
       __builtin_nontemporal_store(c, (f32x4 *)&cov[i]);
       if constexpr (ELL) {
@@ -577,7 +588,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       atomicAdd(&sm.h[256u + (p.key >> 8)], 1u);
     }
   };
-  Slot s0{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint2(0, 0)}, s1 = s0, s2 = s0;
+  Slot s0{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint2(0, 0), ShRow<SHD>{}}, s1 = s0, s2 = s0;
   load(first, s0);
   load(first + 1, s1);
   for (uint32_t tile = first; tile < end; tile += 3) {
