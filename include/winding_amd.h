@@ -63,7 +63,7 @@ extern "C" {
 #define WD_COARSE_BINS 256
 /* Pose features the predictor reads: the model-view's top three rows. */
 #define WD_POSE_FLOATS 12
-/* Most splats wd_sort_noalloc takes (its intermediates live in a fixed array of about 14 MiB). */
+/* Most splats wd_sort_noalloc takes (its intermediates live in a fixed array of about 16 MiB). */
 #define WD_NOALLOC_MAX_SPLATS (1u << 21)
 /* Bytes in one particle: position, age, velocity, lifetime. src/render/particles.js:51-56 */
 #define WD_PARTICLE_BYTES 32
@@ -322,6 +322,16 @@ int wd_state_read_keys(wd_context *ctx, wd_state *state, uint16_t *keys,
  * copies the first n splats' 8 floats (clip position, then axes) to the host.
  */
 int wd_state_ellipses(wd_context *ctx, wd_state *state, int on);
+/*
+ * The tile this state's single sorts (wd_sort, wd_particles_sort) take:
+ * keys_per_thread 4, 8 or 16 (tiles of 1024, 2048 or 4096 splats, 256
+ * threads a workgroup), or 0, the default: by the cloud's size, 4 up to
+ * 512Ki splats, 8 up to 3Mi, 16 past that (the fastest of the three at each
+ * size on MI355X). A sort whose tiles are more than the state's per-tile
+ * buffers hold (a forced 4 past 512Ki splats) is refused with WD_E_CAPACITY.
+ * No counterpart in the reference, whose sort has no tiles.
+ */
+int wd_state_keys_per_thread(wd_state *state, uint32_t keys_per_thread);
 int wd_state_ellipse_buffer(wd_state *state, void **ellipses);
 int wd_state_read_ellipses(wd_context *ctx, wd_state *state, float *out,
                            uint64_t n);

@@ -13,6 +13,7 @@ import pytest
 
 import oracle
 import winding_amd as wa
+from winding_amd import workloads as wl
 
 pytestmark = pytest.mark.gpu
 
@@ -336,3 +337,46 @@ def test_record_form_equals_splats_data_form_on_gpu(ctx, which):
         st.destroy()
         a.destroy()
         b.destroy()
+
+
+@pytest.mark.parametrize("kpt", [4, 8, 16])
+@pytest.mark.parametrize("n", [1, 1023, 1025, 2049, 4097, 70001, 300000])
+def test_every_tile_size_matches_oracle(ctx, kpt, n):
+    """The single sort's three tile builds (1024, 2048 and 4096 splats a tile) forced on the same inputs:
+    each bit-exact against the oracle, at sizes on and beside every tile edge."""
+    recs = wa.synth_records(n % 2, 90 + n, n)
+    view = wl.view_for(recs, (0.5, 1.0, 10.0), 1280, 720)
+    st = wa.State(ctx, n)
+    st.set_keys_per_thread(kpt)
+    try:
+        order, keys, keep, cov, stats = run_gpu(ctx, recs, view, st)
+    finally:
+        st.destroy()
+    o_order, o_keys, o_keep, o_cov = oracle.run(recs, view)
+    assert np.array_equal(keep, o_keep) and np.array_equal(keys[keep], o_keys[o_keep])
+    assert np.array_equal(order, o_order), "kpt %d n %d: order differs" % (kpt, n)
+    assert cov_error(cov, o_cov, keep) <= COV_TOL
+
+
+def test_forced_small_tiles_past_the_buffers_are_refused(ctx):
+    """A state of 600,000 splats holds per-tile buffers for tiles of 2048 (293) or, up to 512Ki splats,
+    of 1024 (512): tiles of 1024 over 600,000 splats (586) are refused by name, and the size's own
+    choice then runs."""
+    n = 600_000
+    recs = wa.synth_records(0, 5, n)
+    view = wl.view_for(recs, (0.0, 0.0, 10.0), 1280, 720)
+    cloud = wa.Cloud(ctx, recs)
+    st = wa.State(ctx, n)
+    try:
+        st.set_keys_per_thread(4)
+        with pytest.raises(wa.WindingError, match="586 tiles; this state's buffers hold 512"):
+            st.sort(cloud, view)
+        with pytest.raises(wa.WindingError, match="keys a thread; 0"):
+            st.set_keys_per_thread(5)
+        st.set_keys_per_thread(0)
+        st.sort(cloud, view)
+        order, _, _ = st.read()
+        assert np.array_equal(order, oracle.run(recs, view)[0])
+    finally:
+        cloud.destroy()
+        st.destroy()

@@ -33,7 +33,8 @@ EXPORTS = [
     "wd_cloud_sh", "wd_cloud_create_splats", "wd_cloud_read", "wd_decode_ply", "wd_decode_spz", "wd_decode_sog",
     "wd_view_prepare", "wd_state_create", "wd_state_destroy", "wd_state_buffers", "wd_sort", "wd_sort_noalloc",
     "wd_state_read", "wd_state_stats", "wd_state_read_keys", "wd_state_ellipses", "wd_state_ellipse_buffer",
-    "wd_state_read_ellipses", "wd_pool_create", "wd_pool_device", "wd_pool_destroy", "wd_particles_sort",
+    "wd_state_read_ellipses", "wd_state_keys_per_thread", "wd_pool_create", "wd_pool_device", "wd_pool_destroy",
+    "wd_particles_sort",
     "wd_profile", "wd_profile_read", "wd_time_sorts",
     "wd_batch_create", "wd_batch_schedule", "wd_batch_configure", "wd_batch_tune", "wd_batch_set", "wd_batch_run",
     "wd_batch_stuck", "wd_batch_destroy",
@@ -161,6 +162,7 @@ def lib():
             "wd_state_stats": (I, [P, P, C.POINTER(Stats)]),
             "wd_state_read_keys": (I, [P, P, P, P, U64]),
             "wd_state_ellipses": (I, [P, P, I]),
+            "wd_state_keys_per_thread": (I, [P, U32]),
             "wd_state_ellipse_buffer": (I, [P, C.POINTER(P)]),
             "wd_state_read_ellipses": (I, [P, P, P, U64]),
             "wd_pool_create": (I, [P, P, U64, C.POINTER(P)]),
@@ -576,6 +578,10 @@ class State:
         """Sorts slots [first, first + count) of `pool` far to near (wd_particles_sort)."""
         count = pool.count - first if count is None else count
         check(lib().wd_particles_sort(self.ctx.ptr, pool.ptr, first, count, C.byref(view), self.ptr))
+
+    def set_keys_per_thread(self, kpt):
+        """The tile this state's sorts take: 4, 8 or 16 keys a thread, or 0 (by the cloud's size)."""
+        check(lib().wd_state_keys_per_thread(self.ptr, kpt))
 
     def set_ellipses(self, on):
         """Whether each sort also writes the draw's ellipse set-up (8 floats a splat) beside cov2d."""

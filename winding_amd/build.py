@@ -33,10 +33,12 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Werror", "-Wno-unknown-pragmas",
               "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["splat_sort.hip", "batch.hip", "mlp.hip", "learned.hip", "decode.hip"]
-# Sources built once more with other defines: the batch kernels at 8 keys a thread (tiles of 2048),
-# beside the default 16 (4096), for the batch autotuner.
-HIP_VARIANTS = [("batch.hip", "batch.kpt8.hip.o", ["-DWD_KEYS_PER_THREAD=8"])]
+HIP_SOURCES = ["splat_sort.hip", "sort_passes.hip", "batch.hip", "mlp.hip", "learned.hip", "decode.hip"]
+# Sources built once more with other defines: the single sort's passes at 4 and 8 keys a thread
+# (tiles of 1024 and 2048, for smaller clouds) and the batch kernels at 8, beside the default 16 (4096).
+HIP_VARIANTS = [("sort_passes.hip", "sort_passes.kpt4.hip.o", ["-DWD_KEYS_PER_THREAD=4"]),
+                ("sort_passes.hip", "sort_passes.kpt8.hip.o", ["-DWD_KEYS_PER_THREAD=8"]),
+                ("batch.hip", "batch.kpt8.hip.o", ["-DWD_KEYS_PER_THREAD=8"])]
 HOST_SOURCES = ["api.cpp", "synth.cpp", "batch_host.cpp", "mlp_host.cpp", "learned_host.cpp", "decode_host.cpp"]
 
 

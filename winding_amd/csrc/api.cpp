@@ -373,7 +373,7 @@ int wd_state_create(wd_context *ctx, uint64_t capacity, wd_state **out) {
   wd_state *s = new wd_state();
   s->ctx = ctx;
   s->capacity = capacity;
-  s->tiles = (capacity + wd::kTileMin - 1) / wd::kTileMin;
+  s->tiles = wd::sort_tiles_capacity(capacity);
   const uint64_t words = (capacity + 63) / 64;
   hipError_t e = hipSuccess;
   auto alloc = [&](void **p, size_t bytes) {
@@ -463,6 +463,8 @@ int wd_pool_destroy(wd_pool *pool) {
   return WD_OK;
 }
 
+static int check_tiles(const wd_state *s, uint64_t n, const char *who);
+
 int wd_particles_sort(wd_context *ctx, const wd_pool *pool, uint64_t first, uint64_t count,
                       const wd_particle_view *view, wd_state *s) {
   if (!ctx || !view) return fail(WD_E_ARG, "particles: null argument");
@@ -476,6 +478,7 @@ int wd_particles_sort(wd_context *ctx, const wd_pool *pool, uint64_t first, uint
     return fail(WD_E_CAPACITY, "particles: %llu particles are more than this state holds, %llu",
                 (unsigned long long)count, (unsigned long long)s->capacity);
   if (first + count > 0xFFFFFFFFull) return fail(WD_E_ARG, "particles: slots past 2^32 are not addressed");
+  if (const int rc = check_tiles(s, count, "particles")) return rc;
   WD_HIP(hipSetDevice(ctx->device), "particles: set device");
   if (!ctx->particle_edges) {
     // bucket b's lower edge, 2^(b 30 / 4095 - 10) (LOG_NEAR = -10, LOG_FAR = 20, 4096 buckets)
@@ -532,6 +535,25 @@ int wd_state_read_ellipses(wd_context *ctx, wd_state *s, float *out, uint64_t n)
   return WD_OK;
 }
 
+// The tiles a single sort of n splats takes fit the state's per-tile buffers.
+static int check_tiles(const wd_state *s, uint64_t n, const char *who) {
+  const uint32_t kpt = wd::sort_keys_per_thread((uint32_t)n, s->kpt);
+  const uint64_t tile = 256ull * kpt, need = (n + tile - 1) / tile;
+  if (need > s->tiles)
+    return fail(WD_E_CAPACITY, "%s: %llu splats in tiles of %llu are %llu tiles; this state's buffers hold %llu", who,
+                (unsigned long long)n, (unsigned long long)tile, (unsigned long long)need,
+                (unsigned long long)s->tiles);
+  return WD_OK;
+}
+
+int wd_state_keys_per_thread(wd_state *s, uint32_t kpt) {
+  if (!s || s->magic != 0x53544154) return fail(WD_E_STATE, "state: destroyed");
+  if (kpt != 0 && kpt != 4 && kpt != 8 && kpt != 16)
+    return fail(WD_E_ARG, "state: %u keys a thread; 0 (by size), 4, 8 or 16", kpt);
+  s->kpt = kpt;
+  return WD_OK;
+}
+
 static int check_sort_args(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_state *s) {
   if (!ctx || !view) return fail(WD_E_ARG, "sort: null argument");
   if (!cloud || !ctx->live.count(cloud))
@@ -543,7 +565,7 @@ static int check_sort_args(wd_context *ctx, const wd_cloud *cloud, const wd_view
                 (unsigned long long)s->capacity);
   if (!(view->cull >= 0 && std::isfinite(view->cull)))
     return fail(WD_E_ARG, "splatCull must be 0 or more, got %g", (double)view->cull);
-  return WD_OK;
+  return check_tiles(s, cloud->n, "sort");
 }
 
 int wd_sort(wd_context *ctx, const wd_cloud *cloud, const wd_view *view, wd_state *s) {
@@ -572,7 +594,8 @@ int wd_sort_noalloc(wd_context *ctx, const void *records, uint64_t n, const wd_v
                           (uint16_t *)(base + L.tilehist),  (uint16_t *)(base + L.tilehist3),
                           (uint32_t *)(base + L.prefix_lo), (uint32_t *)(base + L.prefix_hi),
                           nullptr,                          n,
-                          count,                            false};
+                          count,                            false,
+                          0};
   return wd::launch_sort_into(ctx, in, *view, b);
 }
 

@@ -79,7 +79,7 @@ __device__ __forceinline__ void count_done(uint32_t *p) {
 template <bool SIG, int SHD, bool ELL>
 __device__ __forceinline__ void project_form(ProjectSmem &sm, const BatchJob &J, uint32_t t) {
   float4 *ell = (float4 *)J.ellipse;
-  project_chunk<true, false, SHD, ELL, SIG>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov,
+  project_chunk<true, false, SHD, ELL, SIG, false>(sm, (const uint4 *)J.rec, J.n, J.view, J.keys, J.mask, (float4 *)J.cov,
                                             J.tilehist, &J.ctl->recomputed_acc, t, nullptr, (const uint32_t *)J.sh,
                                             ell, ELL ? reinterpret_cast<const float *>(ell + 2 * J.capacity) : nullptr,
                                             (const uint2 *)J.sig, J.capacity);

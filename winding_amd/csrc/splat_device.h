@@ -492,7 +492,12 @@ __device__ __forceinline__ f32x4 ellipse_axes(float xx, float xy, float yy, uint
 // ELL: each splat's draw set-up goes to two planes of n_cap float4s, its clip
 // position to ell[i] and its ellipse_axes to ell[n_cap + i] (each store of a
 // wave covers whole lines), zeros for one not drawn.
-template <bool STD, bool BINS = false, int SHD = 0, bool ELL = false, bool SIG = false>
+// SHPRE (SHD > 0): each splat's harmonics row is loaded with its record, two
+// tiles ahead; off, a survivor's row is loaded when it is shaded. A kernel
+// whose one body serves every degree (the batch) keeps it off: its registers
+// are the most any branch needs, and three tiles of rows would halve its
+// occupancy for clouds without harmonics too.
+template <bool STD, bool BINS = false, int SHD = 0, bool ELL = false, bool SIG = false, bool SHPRE = true>
 __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__restrict__ rec, uint32_t n,
                                               const ViewK &v, uint16_t *__restrict__ keys,
                                               uint64_t *__restrict__ mask, float4 *__restrict__ cov,
@@ -523,7 +528,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
   struct Slot {
     uint4 a, b;
     uint2 c;
-    ShRow<SHD> h;
+    ShRow<SHPRE ? SHD : 0> h;
   };
   auto load = [&](uint32_t tile, Slot &s) {
     const uint32_t i = (tile << 8) + tid;
@@ -537,7 +542,7 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
         s.a = rec[2 * (size_t)i];
         s.b = rec[2 * (size_t)i + 1];
       }
-      if constexpr (SHD > 0) {
+      if constexpr (SHD > 0 && SHPRE) {
         const uint32_t *row = sh + (size_t)i * sh_words(SHD);
 #pragma unroll
         for (uint32_t k = 0; k < sh_words(SHD); ++k) s.h.w[k] = row[k];
@@ -569,7 +574,17 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       // of the launch are written back before the next launch starts.
       uint32_t colour = rgba;
       if constexpr (SHD > 0)
-        if (p.keep) colour = shade<SHD>(v, s.h, f32_of(a.x), f32_of(a.y), f32_of(a.z), rgba);
+        if (p.keep) {
+          if constexpr (SHPRE) {
+            colour = shade<SHD>(v, s.h, f32_of(a.x), f32_of(a.y), f32_of(a.z), rgba);
+          } else {
+            ShRow<SHD> row;
+            const uint32_t *src = sh + (size_t)i * sh_words(SHD);
+#pragma unroll
+            for (uint32_t k = 0; k < sh_words(SHD); ++k) row.w[k] = src[k];
+            colour = shade<SHD>(v, row, f32_of(a.x), f32_of(a.y), f32_of(a.z), rgba);
+          }
+        }
       const f32x4 c = p.keep ? f32x4{p.xx, p.xy, p.yy, __uint_as_float(colour)} : f32x4{0.f, 0.f, 0.f, 0.f};
       __builtin_nontemporal_store(c, (f32x4 *)&cov[i]);
       if constexpr (ELL) {
@@ -588,7 +603,8 @@ __device__ __forceinline__ void project_chunk(ProjectSmem &sm, const uint4 *__re
       atomicAdd(&sm.h[256u + (p.key >> 8)], 1u);
     }
   };
-  Slot s0{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint2(0, 0), ShRow<SHD>{}}, s1 = s0, s2 = s0;
+  Slot s0{make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint2(0, 0), ShRow<SHPRE ? SHD : 0>{}}, s1 = s0,
+       s2 = s0;
   load(first, s0);
   load(first + 1, s1);
   for (uint32_t tile = first; tile < end; tile += 3) {

@@ -27,18 +27,30 @@ int hip_fail(hipError_t e, const char *what);
 
 constexpr int kThreads = 256;        // workgroup size of every kernel here
 // Keys a thread ranks in a scatter tile. Set per translation unit by the
-// build: 16 everywhere, and batch.hip is built a second time at 8 (its
-// launchers then live in namespace wd::kpt8, the default's in wd::kpt16).
+// build: 16 by default; sort_passes.hip is built at 4, 8 and 16 (launchers in
+// wd::passes4, passes8, passes16) and batch.hip at 8 and 16 (wd::kpt8,
+// wd::kpt16).
 #ifndef WD_KEYS_PER_THREAD
 #define WD_KEYS_PER_THREAD 16
 #endif
 constexpr int kKeysPerThread = WD_KEYS_PER_THREAD;
-static_assert(kKeysPerThread == 8 || kKeysPerThread == 16 || kKeysPerThread == 32, "8, 16 or 32 keys a thread");
+static_assert(kKeysPerThread == 4 || kKeysPerThread == 8 || kKeysPerThread == 16 || kKeysPerThread == 32,
+              "4, 8, 16 or 32 keys a thread");
 constexpr int kTile = 256 * kKeysPerThread;  // splats per projection chunk and scatter tile
 // Clouds up to this many splats carry high byte << 24 | index between the two
 // passes (4 bytes); bigger ones carry 8 bytes (high byte << 32 | index).
 constexpr uint32_t kNarrowMax = WD_MAX_NARROW;
-constexpr int kTileMin = 256 * 8;  // the smallest tile any kernel here takes: a state's per-tile buffers hold this many
+// A single sort of n splats takes tiles of 1024 up to kSortSmallMax, of 2048
+// up to kSortMidMax, of 4096 past it (sort_keys_per_thread); batches take
+// 2048 or 4096. The per-tile buffers of a state of capacity C hold the most
+// tiles any of those needs.
+constexpr uint32_t kSortSmallMax = 1u << 19;
+constexpr uint32_t kSortMidMax = 3u << 20;
+constexpr uint64_t sort_tiles_capacity(uint64_t C) {
+  return ((C + 2047) / 2048) > ((C < kSortSmallMax ? C : kSortSmallMax) + 1023) / 1024
+             ? (C + 2047) / 2048
+             : ((C < kSortSmallMax ? C : kSortSmallMax) + 1023) / 1024;
+}
 constexpr uint32_t kScanCols = 4;    // histogram columns a scan workgroup takes (a single sort)
 constexpr uint32_t kBatchScanCols = 8;  // the same in a batch's scans
 
@@ -48,7 +60,7 @@ constexpr uint64_t align256(uint64_t x) { return (x + 255u) & ~uint64_t(255); }
 struct NoallocLayout {
   uint64_t ctl, keys, mask, packed, tilehist, tilehist3, prefix_lo, prefix_hi, bytes;
 };
-constexpr uint64_t kNoallocTiles = (WD_NOALLOC_MAX_SPLATS + kTile - 1) / kTile;
+constexpr uint64_t kNoallocTiles = sort_tiles_capacity(WD_NOALLOC_MAX_SPLATS);
 
 // Polls of a bounded wait before it gives up and flags the sort as failed.
 constexpr uint32_t kSpinLimit = 1u << 20;
@@ -306,7 +318,8 @@ struct wd_state {
   uint32_t *prefix_lo = nullptr, *prefix_hi = nullptr;  // [tiles][256]: where a tile's digit run starts
   uint16_t *lut = nullptr;                             // learned sort: 16-key block -> bin and offset
   uint32_t *prefix_bin = nullptr;                      // learned sort: [tiles][512]
-  uint64_t tiles = 0;  // per-tile buffers hold this many tiles of kTileMin splats
+  uint64_t tiles = 0;  // per-tile buffers hold this many tiles (sort_tiles_capacity)
+  uint32_t kpt = 0;    // keys a thread its single sorts take; 0: by their size
   uint32_t magic = 0x53544154;
 };
 
@@ -349,7 +362,25 @@ struct SortBuffers {
   uint64_t capacity;
   uint32_t *count_out;
   bool profile;
+  uint32_t kpt;  // keys a thread the sort takes; 0: by its size
 };
+// The keys a thread (4, 8 or 16) a single sort of n splats takes; forced: 0 or the caller's choice.
+uint32_t sort_keys_per_thread(uint32_t n, uint32_t forced);
+// The sort's passes at one tile size (sort_passes.hip).
+#define WD_SORT_PASSES                                                                                          \
+  int launch_sort_passes(wd_context *ctx, const SortInput &in, const wd_view &view, const SortBuffers &b);      \
+  int launch_particle_passes(wd_context *ctx, const uint8_t *pool, uint64_t first, uint32_t n, const double *view6, \
+                             const double *edges, wd_state *st);
+namespace passes4 {
+WD_SORT_PASSES
+}
+namespace passes8 {
+WD_SORT_PASSES
+}
+namespace passes16 {
+WD_SORT_PASSES
+}
+#undef WD_SORT_PASSES
 int launch_sort_into(wd_context *ctx, const SortInput &in, const wd_view &view, const SortBuffers &b);
 // The no-allocation sort's scratch on the current device (hipGetSymbolAddress of the array).
 int noalloc_scratch(void **ptr);
