@@ -1,6 +1,6 @@
 """wd_sort_noalloc, the sort for callers that cannot allocate, against wd_sort (the entry point it is a
-variant of) and the oracle: order, count and cov2d bit for bit equal to wd_sort's, on two calls in a row
-whose second is smaller (its intermediates share one array per device, which keeps the last call's
+variant of) and the oracle: order, count and cov2d bit for bit equal to wd_sort's, on calls in a row
+of falling size, the first at its limit (its intermediates share one array per device, which keeps the last call's
 contents), and a refusal past its limit. The test's own device buffers come from the HIP runtime the
 library itself links (libamdhip64 of /opt/rocm), through ctypes."""
 import ctypes as C
@@ -52,7 +52,10 @@ class DeviceBuffer:
 
 def test_noalloc_equals_wd_sort_on_two_calls(ctx):
     L = hip()
-    cases = [(1, 5, 1_500_000, (0.0, 2.0, 12.0), 1920, 1080), (0, 9, 70_001, (3.0, -1.0, 8.0), 1280, 720)]
+    # the largest size it takes (its scratch's bound; tiles of 2048), then smaller ones on the same
+    # scratch: tiles of 2048 again and tiles of 1024
+    cases = [(0, 11, wa.WD_NOALLOC_MAX_SPLATS, (1.0, 1.0, 11.0), 1920, 1080),
+             (1, 5, 1_500_000, (0.0, 2.0, 12.0), 1920, 1080), (0, 9, 70_001, (3.0, -1.0, 8.0), 1280, 720)]
     cap = max(c[2] for c in cases)
     records = DeviceBuffer(L, 32 * cap)
     order = DeviceBuffer(L, 4 * cap)
