@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_hist(const void *__restrict__ packed, c
                                               uint16_t *__restrict__ tilehist3) {
   using Pack = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
   __shared__ HistSmem sm;
-  hist_tile(sm, blockIdx.x, static_cast<const Pack *>(packed), ctl, tilehist3);
+  hist_tile<Pack, true>(sm, blockIdx.x, static_cast<const Pack *>(packed), ctl, tilehist3);
 }
 
 // Scan of pass 3: high-byte tile prefixes over the tiles of V.

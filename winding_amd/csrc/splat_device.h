@@ -736,7 +736,11 @@ __device__ __forceinline__ uint32_t pack_digit(Pack v) {
   return sizeof(Pack) == 8 ? (uint32_t)((uint64_t)v >> 32) : (uint32_t)v >> 24;
 }
 
-template <typename Pack>
+// RUNS: the thread's digits are sorted first and each run of equal digits is
+// one add (below); measured on MI355X it saves 1.4 µs of 12.7 on slab-8m,
+// whose survivors share a few high bytes, and costs 5 µs on batch-64, whose
+// scenes' high bytes spread, so the single sort takes it and the batch not.
+template <typename Pack, bool RUNS = false>
 __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const Pack *__restrict__ packed,
                                           const Ctl *__restrict__ ctl, uint16_t *__restrict__ tilehist3) {
   constexpr int KPT = kKeysPerThread;
@@ -769,9 +773,45 @@ __device__ __forceinline__ void hist_tile(HistSmem &sm, uint32_t t, const Pack *
   __syncthreads();  // the LDS may still be read by the previous work item
   for (uint32_t k = tid; k < 1024; k += 256) (&sm.h[0][0])[k] = 0;
   __syncthreads();
+  if (!RUNS) {
+#pragma unroll
+    for (int r = 0; r < KPT; ++r)
+      if (first + r < limit) atomicAdd(&sm.h[w][raw[r]], 1u);
+    __syncthreads();
+    tilehist3[(size_t)t * 256 + tid] = (uint16_t)(sm.h[0][tid] + sm.h[1][tid] + sm.h[2][tid] + sm.h[3][tid]);
+    return;
+  }
+  // The thread's digits sorted in registers (a bitonic network, min and max a
+  // comparator), so each run of equal digits costs one LDS add of its length:
+  // a slab's few high bytes then collide in the LDS a few times a thread, not
+  // KPT times. Elements past V sort last as 256 and are not counted.
 #pragma unroll
   for (int r = 0; r < KPT; ++r)
-    if (first + r < limit) atomicAdd(&sm.h[w][raw[r]], 1u);
+    if (first + r >= limit) raw[r] = 256u;
+#pragma unroll
+  for (int k = 2; k <= KPT; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < KPT; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint32_t a = raw[i], b = raw[l];
+          const bool up = (i & k) == 0;
+          raw[i] = up ? min(a, b) : max(a, b);
+          raw[l] = up ? max(a, b) : min(a, b);
+        }
+      }
+  uint32_t run = 1;
+#pragma unroll
+  for (int r = 0; r < KPT; ++r) {
+    if (r == KPT - 1 || raw[r] != raw[r + 1 < KPT ? r + 1 : r]) {
+      if (raw[r] < 256u) atomicAdd(&sm.h[w][raw[r]], run);
+      run = 1;
+    } else {
+      ++run;
+    }
+  }
   __syncthreads();
   tilehist3[(size_t)t * 256 + tid] = (uint16_t)(sm.h[0][tid] + sm.h[1][tid] + sm.h[2][tid] + sm.h[3][tid]);
 }
