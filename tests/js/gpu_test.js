@@ -114,6 +114,19 @@ test('three clouds sorted in one batch match each cloud sorted alone', () => {
     assert.strictEqual(batched[k].count, one.count);
     assert.deepStrictEqual(Array.from(batched[k].order), Array.from(one.order));
   });
+  // the autotuner keeps the fastest configuration; the sorts after it order the same
+  const report = pass.tune(2);
+  assert.ok(['phased', 'persistent'].includes(report.schedule));
+  assert.strictEqual(report.tile, 256 * report.keysPerThread);
+  assert.strictEqual(report.timed.length, 6);
+  assert.ok(report.timed.every((t) => t.ms > 0));
+  assert.throws(() => gpu.native.batchConfigure(pass._batch, 'phased', 3, 16), /super-tiles of 3/);
+  assert.throws(() => gpu.native.batchConfigure(pass._batch, 'sideways', 1, 16), /'phased' or 'persistent'/);
+  gpu.native.batchConfigure(pass._batch, 'persistent', 8, 8);
+  pass.prepare(scene, cam, null, 1280, 720);
+  pass._items.forEach((i) => { i.sort = true; });
+  assert.strictEqual(pass.sort(), 3);
+  entities.forEach((e, k) => assert.deepStrictEqual(Array.from(pass.read(e).order), Array.from(batched[k].order)));
   pass.destroy();
   clouds.forEach((c) => c.destroy());
 });

@@ -474,6 +474,16 @@ class SplatPass {
     return marked.length;
   }
 
+  /**
+   * Times the batch's configurations on the clouds of the last sort (schedule, work item, tile)
+   * and keeps the fastest for the sorts after it; returns the tuner's report, or null when the
+   * last sort was not a batch. No counterpart in the reference, whose sort has one form.
+   */
+  tune(reps = 5) {
+    if (this._batch === undefined) return null;
+    return this.native.batchTune(this.gpu.handle, this._batch, reps);
+  }
+
   /** The clouds of the last prepare, far first: [{ entity, count }]. */
   clouds() {
     return this._items.map((i) => ({ entity: i.entity, count: i.count }));

@@ -716,6 +716,42 @@ static napi_value js_batch_run(napi_env env, napi_callback_info info) {
   return rc ? throw_wd(env, rc) : NULL;
 }
 
+/* batchConfigure(batch, schedule: 'phased'|'persistent', superTiles, keysPerThread) */
+static napi_value js_batch_configure(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  char which[16];
+  size_t len = 0;
+  uint32_t super_tiles, kpt;
+  if (!get_args(env, info, 4, argv)) return NULL;
+  Box *b = get_box(env, argv[0], K_BATCH);
+  if (!b) return NULL;
+  if (napi_get_value_string_utf8(env, argv[1], which, sizeof which, &len) != napi_ok) {
+    napi_throw_type_error(env, NULL, "batchConfigure: the schedule is 'phased' or 'persistent'");
+    return NULL;
+  }
+  if (!u32_arg(env, argv[2], &super_tiles) || !u32_arg(env, argv[3], &kpt)) return NULL;
+  int schedule = -1;
+  if (strcmp(which, "phased") == 0) schedule = WD_SCHEDULE_PHASED;
+  if (strcmp(which, "persistent") == 0) schedule = WD_SCHEDULE_PERSISTENT;
+  if (schedule < 0) {
+    napi_throw_type_error(env, NULL, "batchConfigure: the schedule is 'phased' or 'persistent'");
+    return NULL;
+  }
+  int rc = wd_batch_configure((wd_batch *)b->ptr, schedule, super_tiles, kpt);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
+/* stateKeysPerThread(state, keysPerThread): 0 (by the cloud's size), 4, 8 or 16 */
+static napi_value js_state_keys_per_thread(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  uint32_t kpt;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Box *s = get_box(env, argv[0], K_STATE);
+  if (!s || !u32_arg(env, argv[1], &kpt)) return NULL;
+  int rc = wd_state_keys_per_thread((wd_state *)s->ptr, kpt);
+  return rc ? throw_wd(env, rc) : NULL;
+}
+
 /* batchTune(context, batch, reps) -> {schedule: 'phased'|'persistent', superTiles, keysPerThread, tile, radixBits,
  * passes, timed: [{schedule, superTiles, keysPerThread, ms}, ...]} */
 static napi_value js_batch_tune(napi_env env, napi_callback_info info) {
@@ -921,7 +957,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"poolCreate", js_pool_create},   {"particlesSort", js_particles_sort},
       {"synth", js_synth},         {"profile", js_profile},       {"profileRead", js_profile_read},
       {"batchCreate", js_batch_create}, {"batchSet", js_batch_set}, {"batchRun", js_batch_run},
-      {"batchTune", js_batch_tune},     {"mlpCreate", js_mlp_create}, {"mlpSave", js_mlp_save},
+      {"batchTune", js_batch_tune},     {"batchConfigure", js_batch_configure},
+      {"stateKeysPerThread", js_state_keys_per_thread}, {"mlpCreate", js_mlp_create}, {"mlpSave", js_mlp_save},
       {"mlpLoad", js_mlp_load},         {"mlpTrain", js_mlp_train}, {"mlpPredict", js_mlp_predict},
       {"poseFeatures", js_pose_features}, {"stateQuantiles", js_state_quantiles},
       {"learnedCreate", js_learned_create}, {"learnedRun", js_learned_run},
