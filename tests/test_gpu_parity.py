@@ -380,3 +380,26 @@ def test_forced_small_tiles_past_the_buffers_are_refused(ctx):
     finally:
         cloud.destroy()
         st.destroy()
+
+
+def test_two_contexts_sort_side_by_side(ctx):
+    """Two contexts on one device, each its own stream, sorts enqueued in turn with no wait between:
+    the streams run side by side and share nothing but the device, so each result is the oracle's."""
+    other = wa.Context(0)
+    recs_a, view_a, _ = wl.plain("plain-1m")
+    recs_b = wa.synth_records(1, 77, 300_000)
+    view_b = wl.view_for(recs_b, (1.0, 0.5, 9.0), 1280, 720)
+    ca, cb = wa.Cloud(ctx, recs_a), wa.Cloud(other, recs_b)
+    sa, sb = wa.State(ctx, len(recs_a)), wa.State(other, len(recs_b))
+    try:
+        for _ in range(6):
+            sa.sort(ca, view_a)
+            sb.sort(cb, view_b)
+        ctx.sync()
+        other.sync()
+        assert np.array_equal(sa.read()[0], oracle.run(recs_a, view_a)[0])
+        assert np.array_equal(sb.read()[0], oracle.run(recs_b, view_b)[0])
+    finally:
+        for x in (ca, sa, cb, sb):
+            x.destroy()
+        other.close()
