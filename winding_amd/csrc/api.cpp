@@ -188,7 +188,8 @@ static int cloud_from(wd_context *ctx, const void *src, uint64_t bytes, bool dev
   if (e == hipSuccess)
     e = hipMemcpyAsync(c->records, src, bytes, device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                        ctx->stream);
-  if (e == hipSuccess) e = hipMalloc(&scratch, 12 * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMalloc(&scratch, (12 + (size_t)wd::kBoxWords * wd::bounds_partials(ctx)) * sizeof(uint32_t));
   if (e != hipSuccess) {
     if (scratch) (void)hipFree(scratch);
     (void)hipFree(c->records);

@@ -164,19 +164,22 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t x) {
   return x;
 }
 
-// The workgroup's decoded splats out: the box of the finite centres reduced in
-// the wave and across the workgroup (six atomics a workgroup), the least
-// index of a non-finite position, and centres, Sigma and harmonics staged in
-// LDS so that each goes out as one contiguous run of whole lines. lds holds
-// 256 (10 + W) words; the caller has finished with it.
+// The workgroup's decoded splats out: the box of the finite centres and the
+// least index of a non-finite position, reduced in the wave and across the
+// workgroup into the workgroup's own slot of o.partials (reduced over the
+// workgroups afterwards by launch_box_reduce: no global atomics), and
+// centres, Sigma and harmonics staged in LDS so that each goes out as one
+// contiguous run of whole lines. lds holds 256 (10 + W) words; the caller has
+// finished with it.
 template <int DEG>
 __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t cnt, bool valid, const Decoded<DEG> &d,
                                      uint32_t *lds) {
   constexpr uint32_t W = DEG ? sh_words(DEG) : 0;
-  __shared__ uint32_t box[9];  // ordered min xyz, ordered max xyz, first zero per axis
+  __shared__ uint32_t box[kBoxWords];  // ordered min xyz, ordered max xyz, first zero per axis, first bad
   const uint32_t t = threadIdx.x;
-  if (t < 9) box[t] = (t < 3 || t >= 6) ? 0xFFFFFFFFu : 0u;
-  if (valid && !d.finite) atomicMin(o.bad, i0 + t);
+  if (t < kBoxWords) box[t] = (t < 3 || t >= 6) ? 0xFFFFFFFFu : 0u;
+  __syncthreads();
+  if (valid && !d.finite) atomicMin(&box[9], i0 + t);
   const bool in_box = valid && d.finite;
   const float p[3] = {d.centre.x, d.centre.y, d.centre.z};
   uint32_t lo[3], hi[3], zi[3];
@@ -213,9 +216,7 @@ __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t c
     for (uint32_t k = t; k < cnt * W; k += 256) gs[k] = sh[k];
   }
   __syncthreads();
-  if (t < 3 && box[t] != 0xFFFFFFFFu) atomicMin(&o.box[t], box[t]);
-  if (t >= 3 && t < 6 && box[t] != 0u) atomicMax(&o.box[t], box[t]);
-  if (t >= 6 && t < 9 && box[t] != 0xFFFFFFFFu) atomicMin(&o.zero[t - 6], box[t]);
+  if (t < kBoxWords) o.partials[(size_t)blockIdx.x * kBoxWords + t] = box[t];
 }
 
 constexpr uint32_t emit_words(int DEG) { return 256u * (10u + (DEG ? sh_words(DEG) : 0u)); }

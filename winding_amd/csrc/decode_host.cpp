@@ -77,17 +77,16 @@ int decode_into(wd_context *ctx, uint64_t n, uint32_t degree, const std::functio
   if (e == hipSuccess) e = hipMalloc(&c->covs, n * 24);
   if (e == hipSuccess && W) e = hipMalloc(&c->sh, (n * W + 1) * 4);
   if (e == hipSuccess && W) e = hipMemsetAsync(c->sh, 0, (n * W + 1) * 4, ctx->stream);
-  if (e == hipSuccess) e = hipMalloc(&scratch, 12 * sizeof(uint32_t));
-  // bad index, ordered min xyz, ordered max xyz, first zero per axis
-  const uint32_t init[12] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u,
-                             0u,          0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u};
-  if (e == hipSuccess) e = hipMemcpyAsync(scratch, init, sizeof init, hipMemcpyHostToDevice, ctx->stream);
+  // bad index, ordered min xyz, ordered max xyz, first zero per axis, 2 spare; then a slot a workgroup
+  const uint32_t groups = (uint32_t)((n + 255) / 256);
+  if (e == hipSuccess) e = hipMalloc(&scratch, (12 + (size_t)wd::kBoxWords * groups) * sizeof(uint32_t));
   if (e != hipSuccess) {
     undo();
     return wd::hip_fail(e, "splats: decode buffers");
   }
-  wd::DecodeOut o{c->centers, c->covs, c->sh, W, scratch, scratch + 1, scratch + 7};
+  wd::DecodeOut o{c->centers, c->covs, c->sh, W, scratch, scratch + 1, scratch + 7, scratch + 12};
   int rc = launch(o);
+  if (rc == WD_OK) rc = wd::launch_box_reduce(ctx, o.partials, groups, o.box, o.box + 3, o.zero, o.bad);
   uint32_t res[12];
   if (rc == WD_OK) {
     e = hipMemcpyAsync(res, scratch, sizeof res, hipMemcpyDeviceToHost, ctx->stream);

@@ -16,10 +16,15 @@ __device__ __forceinline__ uint32_t ordered_bits(float f) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-__global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, uint32_t n, uint32_t *out12) {
-  // out12: [0..2] ordered min, [3..5] ordered max, [6] first bad index, [7..9] first index whose
-  // coordinate k is a zero of either sign (put()'s strict compares keep the first of equal values, so
-  // a zero minimum or maximum takes that zero's sign)
+// A workgroup's share of the cloud, its box reduced in LDS and written to its
+// own slot of partials (kBoxWords words): ordered min xyz, ordered max xyz,
+// the first index whose coordinate k is a zero of either sign (put()'s strict
+// compares keep the first of equal values, so a zero minimum or maximum takes
+// that zero's sign), the first non-finite index.
+__global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, uint32_t n, uint32_t *partials) {
+  __shared__ uint32_t box[kBoxWords];
+  if (threadIdx.x < kBoxWords) box[threadIdx.x] = threadIdx.x >= 3 && threadIdx.x < 6 ? 0u : 0xFFFFFFFFu;
+  __syncthreads();
   uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0, 0, 0}, bad = 0xFFFFFFFFu;
   uint32_t zero[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
@@ -42,11 +47,47 @@ __global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, u
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    atomicMin(&out12[k], lo[k]);
-    atomicMax(&out12[3 + k], hi[k]);
-    if (zero[k] != 0xFFFFFFFFu) atomicMin(&out12[7 + k], zero[k]);
+    atomicMin(&box[k], lo[k]);
+    atomicMax(&box[3 + k], hi[k]);
+    if (zero[k] != 0xFFFFFFFFu) atomicMin(&box[6 + k], zero[k]);
   }
-  if (bad != 0xFFFFFFFFu) atomicMin(&out12[6], bad);
+  if (bad != 0xFFFFFFFFu) atomicMin(&box[9], bad);
+  __syncthreads();
+  if (threadIdx.x < kBoxWords) partials[(size_t)blockIdx.x * kBoxWords + threadIdx.x] = box[threadIdx.x];
+}
+
+// The slots of count workgroups (k_bounds, the decoders) reduced by one
+// workgroup: the box to lo3 and hi3, the first zeros to zero3 and the first
+// non-finite index to bad.
+__global__ __launch_bounds__(256) void k_box_reduce(const uint32_t *__restrict__ partials, uint32_t count,
+                                                    uint32_t *lo3, uint32_t *hi3, uint32_t *zero3, uint32_t *bad) {
+  __shared__ uint32_t box[kBoxWords];
+  const uint32_t t = threadIdx.x;
+  auto is_max = [](uint32_t k) { return k >= 3 && k < 6; };
+  if (t < kBoxWords) box[t] = is_max(t) ? 0u : 0xFFFFFFFFu;
+  __syncthreads();
+  uint32_t v[kBoxWords];
+#pragma unroll
+  for (uint32_t k = 0; k < kBoxWords; ++k) v[k] = is_max(k) ? 0u : 0xFFFFFFFFu;
+  for (uint32_t w = t; w < count; w += 256) {
+#pragma unroll
+    for (uint32_t k = 0; k < kBoxWords; ++k) {
+      const uint32_t x = partials[(size_t)w * kBoxWords + k];
+      v[k] = is_max(k) ? max(v[k], x) : min(v[k], x);
+    }
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kBoxWords; ++k) {
+    if (is_max(k))
+      atomicMax(&box[k], v[k]);
+    else
+      atomicMin(&box[k], v[k]);
+  }
+  __syncthreads();
+  if (t < 3) lo3[t] = box[t];
+  else if (t < 6) hi3[t - 3] = box[t];
+  else if (t < 9) zero3[t - 6] = box[t];
+  else if (t == 9) *bad = box[9];
 }
 
 // ---------------------------------------------------------------------------
@@ -178,15 +219,25 @@ const char *const kKernelNames[kKernelCount] = {
     "k_bounds", "k_project", "k_scan", "k_scatter_lo", "k_hist",        "k_scan2",
     "k_scatter_hi", "k_batch", "k_mlp", "k_bin_table", "k_bin_scatter", "k_bin_sort", "k_particle_keys", "k_decode"};
 
-int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch12) {
-  static const uint32_t init[12] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0,           0, 0,
-                                    0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0};
-  WD_HIP(hipMemcpyAsync(scratch12, init, sizeof init, hipMemcpyHostToDevice, ctx->stream), "cloud check: upload");
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->cus * 4);
+uint32_t bounds_partials(const wd_context *ctx) { return 4u * (uint32_t)(ctx->cus > 0 ? ctx->cus : 1); }
+
+int launch_box_reduce(wd_context *ctx, const uint32_t *partials, uint32_t count, uint32_t *lo3, uint32_t *hi3,
+                      uint32_t *zero3, uint32_t *bad) {
+  hipLaunchKernelGGL(k_box_reduce, dim3(1), dim3(256), 0, ctx->stream, partials, count, lo3, hi3, zero3, bad);
+  WD_HIP(hipGetLastError(), "cloud check: box launch");
+  return WD_OK;
+}
+
+int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch) {
+  const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>((n + 255) / 256, (uint64_t)bounds_partials(ctx)));
+  uint32_t *partials = scratch + 12;
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKBounds, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)records, (uint32_t)n, scratch12);
+  hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)records, (uint32_t)n, partials);
   WD_HIP(hipGetLastError(), "cloud check: launch");
+  int rc = launch_box_reduce(ctx, partials, blocks, scratch, scratch + 3, scratch + 7, scratch + 6);
+  if (rc) return rc;
   return ctx->prof_end(kKBounds, ev);
 }
 
