@@ -166,7 +166,7 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t x) {
 
 // The workgroup's decoded splats out: the box of the finite centres and the
 // least index of a non-finite position, reduced in the wave and across the
-// workgroup into the workgroup's own slot of o.partials (reduced over the
+// workgroup into the workgroup's own column of o.partials (reduced over the
 // workgroups afterwards by launch_box_reduce: no global atomics), and
 // centres, Sigma and harmonics staged in LDS so that each goes out as one
 // contiguous run of whole lines. lds holds 256 (10 + W) words; the caller has
@@ -216,7 +216,7 @@ __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t c
     for (uint32_t k = t; k < cnt * W; k += 256) gs[k] = sh[k];
   }
   __syncthreads();
-  if (t < kBoxWords) o.partials[(size_t)blockIdx.x * kBoxWords + t] = box[t];
+  if (t < kBoxWords) o.partials[(size_t)t * gridDim.x + blockIdx.x] = box[t];  // word-major
 }
 
 constexpr uint32_t emit_words(int DEG) { return 256u * (10u + (DEG ? sh_words(DEG) : 0u)); }

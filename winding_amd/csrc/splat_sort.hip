@@ -16,78 +16,116 @@ __device__ __forceinline__ uint32_t ordered_bits(float f) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-// A workgroup's share of the cloud, its box reduced in LDS and written to its
-// own slot of partials (kBoxWords words): ordered min xyz, ordered max xyz,
-// the first index whose coordinate k is a zero of either sign (put()'s strict
-// compares keep the first of equal values, so a zero minimum or maximum takes
-// that zero's sign), the first non-finite index.
-__global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, uint32_t n, uint32_t *partials) {
-  __shared__ uint32_t box[kBoxWords];
-  if (threadIdx.x < kBoxWords) box[threadIdx.x] = threadIdx.x >= 3 && threadIdx.x < 6 ? 0u : 0xFFFFFFFFu;
-  __syncthreads();
-  uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0, 0, 0}, bad = 0xFFFFFFFFu;
-  uint32_t zero[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    const uint4 a = rec[2 * (size_t)i];
-    const float p[3] = {f32_of(a.x), f32_of(a.y), f32_of(a.z)};
-    bool finite = true;
+// A box of kBoxWords words: ordered min xyz, ordered max xyz (words 3..5, the
+// maxima), the first index whose coordinate k is a zero of either sign (put()'s
+// strict compares keep the first of equal values, so a zero minimum or maximum
+// takes that zero's sign), the first non-finite index.
+__device__ __forceinline__ bool box_is_max(uint32_t k) { return k >= 3 && k < 6; }
+__device__ __forceinline__ uint32_t box_fold(uint32_t k, uint32_t a, uint32_t b) {
+  return box_is_max(k) ? max(a, b) : min(a, b);
+}
+__device__ __forceinline__ void box_empty(uint32_t (&v)[kBoxWords]) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) finite = finite && isfinite(p[k]);
-    if (!finite) {
-      bad = min(bad, i);
-      continue;
-    }
+  for (uint32_t k = 0; k < kBoxWords; ++k) v[k] = box_is_max(k) ? 0u : 0xFFFFFFFFu;
+}
+// Folds a wave's boxes into every lane's, with shuffles (no LDS atomics, whose
+// same-address form the compiler turns into a lane-by-lane loop).
+__device__ __forceinline__ void box_wave(uint32_t (&v)[kBoxWords]) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const uint32_t o = ordered_bits(p[k]);
-      lo[k] = min(lo[k], o);
-      hi[k] = max(hi[k], o);
-      if (p[k] == 0.0f) zero[k] = min(zero[k], i);
-    }
-  }
+  for (uint32_t m = 32; m >= 1; m >>= 1)
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    atomicMin(&box[k], lo[k]);
-    atomicMax(&box[3 + k], hi[k]);
-    if (zero[k] != 0xFFFFFFFFu) atomicMin(&box[6 + k], zero[k]);
-  }
-  if (bad != 0xFFFFFFFFu) atomicMin(&box[9], bad);
-  __syncthreads();
-  if (threadIdx.x < kBoxWords) partials[(size_t)blockIdx.x * kBoxWords + threadIdx.x] = box[threadIdx.x];
+    for (uint32_t k = 0; k < kBoxWords; ++k) v[k] = box_fold(k, v[k], (uint32_t)__shfl_xor((int)v[k], (int)m, 64));
 }
 
-// The slots of count workgroups (k_bounds, the decoders) reduced by one
-// workgroup: the box to lo3 and hi3, the first zeros to zero3 and the first
-// non-finite index to bad.
-__global__ __launch_bounds__(256) void k_box_reduce(const uint32_t *__restrict__ partials, uint32_t count,
-                                                    uint32_t *lo3, uint32_t *hi3, uint32_t *zero3, uint32_t *bad) {
-  __shared__ uint32_t box[kBoxWords];
+// A workgroup's share of the cloud, its box folded in the waves and then
+// across them, written to its own column of partials (word k of workgroup g
+// at k * gridDim.x + g). Each thread has four records' loads in flight.
+__global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, uint32_t n, uint32_t *partials) {
+  constexpr uint32_t kIn = 4;
+  __shared__ uint32_t waves[4][kBoxWords];
   const uint32_t t = threadIdx.x;
-  auto is_max = [](uint32_t k) { return k >= 3 && k < 6; };
-  if (t < kBoxWords) box[t] = is_max(t) ? 0u : 0xFFFFFFFFu;
-  __syncthreads();
   uint32_t v[kBoxWords];
+  box_empty(v);
+  const uint32_t stride = gridDim.x * 256u * kIn;
+  for (uint32_t i0 = blockIdx.x * 256u * kIn + t; i0 < n; i0 += stride) {
+    uint4 a[kIn];
 #pragma unroll
-  for (uint32_t k = 0; k < kBoxWords; ++k) v[k] = is_max(k) ? 0u : 0xFFFFFFFFu;
-  for (uint32_t w = t; w < count; w += 256) {
+    for (uint32_t j = 0; j < kIn; ++j) {
+      const uint32_t i = i0 + j * 256u;
+      a[j] = i < n ? rec[2 * (size_t)i] : make_uint4(0, 0, 0, 0);
+    }
 #pragma unroll
-    for (uint32_t k = 0; k < kBoxWords; ++k) {
-      const uint32_t x = partials[(size_t)w * kBoxWords + k];
-      v[k] = is_max(k) ? max(v[k], x) : min(v[k], x);
+    for (uint32_t j = 0; j < kIn; ++j) {
+      const uint32_t i = i0 + j * 256u;
+      const float p[3] = {f32_of(a[j].x), f32_of(a[j].y), f32_of(a[j].z)};
+      const bool finite = isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]);
+      if (i >= n) continue;
+      if (!finite) {
+        v[9] = min(v[9], i);
+        continue;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t o = ordered_bits(p[k]);
+        v[k] = min(v[k], o);
+        v[3 + k] = max(v[3 + k], o);
+        if (p[k] == 0.0f) v[6 + k] = min(v[6 + k], i);
+      }
     }
   }
+  box_wave(v);
+  if ((t & 63u) == 0)
 #pragma unroll
-  for (uint32_t k = 0; k < kBoxWords; ++k) {
-    if (is_max(k))
-      atomicMax(&box[k], v[k]);
-    else
-      atomicMin(&box[k], v[k]);
-  }
+    for (uint32_t k = 0; k < kBoxWords; ++k) waves[t >> 6][k] = v[k];
   __syncthreads();
-  if (t < 3) lo3[t] = box[t];
-  else if (t < 6) hi3[t - 3] = box[t];
-  else if (t < 9) zero3[t - 6] = box[t];
-  else if (t == 9) *bad = box[9];
+  if (t < kBoxWords) {
+    uint32_t r = waves[0][t];
+#pragma unroll
+    for (uint32_t q = 1; q < 4; ++q) r = box_fold(t, r, waves[q][t]);
+    partials[(size_t)t * gridDim.x + blockIdx.x] = r;
+  }
+}
+
+// The boxes of count workgroups (k_bounds, the decoders) reduced by one
+// workgroup: the box to lo3 and hi3, the first zeros to zero3 and the first
+// non-finite index to bad. The boxes lie word-major (word k of workgroup g at
+// k * count + g), so each of the ten loads of four slots a thread keeps in
+// flight is coalesced; each wave folds its lanes, and ten threads fold the
+// sixteen waves.
+__global__ __launch_bounds__(1024) void k_box_reduce(const uint32_t *__restrict__ partials, uint32_t count,
+                                                     uint32_t *lo3, uint32_t *hi3, uint32_t *zero3, uint32_t *bad) {
+  constexpr uint32_t kIn = 4;  // slots a thread has in flight
+  __shared__ uint32_t waves[16][kBoxWords];
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  uint32_t v[kBoxWords];
+  box_empty(v);
+  for (uint32_t s0 = 0; s0 < count; s0 += 1024u * kIn) {
+    uint32_t x[kIn][kBoxWords];
+#pragma unroll
+    for (uint32_t j = 0; j < kIn; ++j) {
+      const uint32_t slot = s0 + j * 1024u + t;
+#pragma unroll
+      for (uint32_t k = 0; k < kBoxWords; ++k)
+        x[j][k] = slot < count ? partials[(size_t)k * count + slot] : (box_is_max(k) ? 0u : 0xFFFFFFFFu);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kIn; ++j)
+#pragma unroll
+      for (uint32_t k = 0; k < kBoxWords; ++k) v[k] = box_fold(k, v[k], x[j][k]);
+  }
+  box_wave(v);
+  if (lane == 0)
+#pragma unroll
+    for (uint32_t k = 0; k < kBoxWords; ++k) waves[w][k] = v[k];
+  __syncthreads();
+  if (t < kBoxWords) {
+    uint32_t r = waves[0][t];
+    for (uint32_t q = 1; q < 16; ++q) r = box_fold(t, r, waves[q][t]);
+    if (t < 3) lo3[t] = r;
+    else if (t < 6) hi3[t - 3] = r;
+    else if (t < 9) zero3[t - 6] = r;
+    else *bad = r;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -223,14 +261,14 @@ uint32_t bounds_partials(const wd_context *ctx) { return 4u * (uint32_t)(ctx->cu
 
 int launch_box_reduce(wd_context *ctx, const uint32_t *partials, uint32_t count, uint32_t *lo3, uint32_t *hi3,
                       uint32_t *zero3, uint32_t *bad) {
-  hipLaunchKernelGGL(k_box_reduce, dim3(1), dim3(256), 0, ctx->stream, partials, count, lo3, hi3, zero3, bad);
+  hipLaunchKernelGGL(k_box_reduce, dim3(1), dim3(1024), 0, ctx->stream, partials, count, lo3, hi3, zero3, bad);
   WD_HIP(hipGetLastError(), "cloud check: box launch");
   return WD_OK;
 }
 
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch) {
   const uint32_t blocks = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>((n + 255) / 256, (uint64_t)bounds_partials(ctx)));
+      1, std::min<uint64_t>((n + 1023) / 1024, (uint64_t)bounds_partials(ctx)));  // 1024 records a workgroup a pass
   uint32_t *partials = scratch + 12;
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKBounds, &ev)) return WD_E_HIP;

@@ -334,11 +334,12 @@ extern const char *const kKernelNames[kKernelCount];
 // Kernel launchers (splat_sort.hip). Enqueue on ctx->stream.
 // The cloud check's box is reduced without global atomics: each workgroup
 // writes kBoxWords words (ordered min xyz, ordered max xyz, first zero per
-// axis, first non-finite index) to a slot of its own, and one workgroup
-// reduces the slots (launch_box_reduce).
+// axis, first non-finite index) to a column of its own (word k of workgroup g
+// of a launch of G at k * G + g), and one workgroup reduces the columns
+// (launch_box_reduce, with count = G).
 constexpr uint32_t kBoxWords = 10;
 // scratch: 12 words of results (ordered min xyz, ordered max xyz, first bad
-// index, first zero per axis), then bounds_partials(ctx) slots.
+// index, first zero per axis), then bounds_partials(ctx) columns.
 uint32_t bounds_partials(const wd_context *ctx);
 int launch_bounds(wd_context *ctx, const uint8_t *records, uint64_t n, uint32_t *scratch);
 int launch_box_reduce(wd_context *ctx, const uint32_t *partials, uint32_t count, uint32_t *lo3, uint32_t *hi3,
@@ -407,7 +408,7 @@ struct DecodeOut {
   uint32_t *bad;
   uint32_t *box;
   uint32_t *zero;  // per axis, the least index whose coordinate is a zero of either sign
-  uint32_t *partials;  // kBoxWords a workgroup: its own box, first zeros and first bad index
+  uint32_t *partials;  // kBoxWords a workgroup, word-major: its own box, first zeros and first bad index
 };
 // .ply property types, as PLY_TYPES names them (src/scene/splats.js:392-397).
 enum PlyType : uint8_t { kPlyI8, kPlyU8, kPlyI16, kPlyU16, kPlyI32, kPlyU32, kPlyF32, kPlyF64 };
