@@ -64,28 +64,26 @@ __device__ __forceinline__ uint32_t signed_half(double v) { return v < 0 ? 0x800
 
 // signed_half of a value that is an f32, from its bits: the same answer
 // (halfBits' tests and its rounding, half up, are exact on the f32's
-// mantissa) in integer operations instead of FP64 ones.
+// mantissa) in integer operations and selects instead of FP64 ones and
+// branches. |v| = m 2^(e - 150) with m the 24-bit significand: a subnormal
+// half is round(|v| 2^24) = (m + 2^(s - 1)) >> s, s = 126 - e in [14, 23]; a
+// normal one the top 10 mantissa bits plus one when the 13 dropped are half
+// or more (a carry into the exponent is halfBits' mantissa == 1024 step).
+// Equal to the branch-by-branch form for all 2^32 inputs.
 __device__ __forceinline__ uint32_t signed_half_f32(float v) {
   const uint32_t bits = __float_as_uint(v);
-  const uint32_t sign = (bits >> 16) & 0x8000u;
-  const uint32_t a = bits & 0x7FFFFFFFu;  // |v|
-  uint32_t h;
-  if (a > 0x7F800000u || a == 0u) {
-    h = 0u;                                // NaN, zero
-  } else if (a >= 0x477FE000u) {
-    h = 0x7bffu;                           // |v| >= 65504, infinity included
-  } else if (a < 0x33800000u) {
-    h = 0u;                                // |v| < 2^-24
-  } else if (a < 0x38800000u) {
-    // subnormal: round(|v| / 2^-24), half up; |v| 2^24 < 1024 is exact in f32
-    const float x = __uint_as_float(a) * 16777216.0f;
-    const float r = floorf(x);
-    h = (uint32_t)r + ((x - r >= 0.5f) ? 1u : 0u);
-  } else {
-    // normal: the top 10 mantissa bits, plus one when the 13 dropped bits are half or more
-    h = (((a >> 23) - 112u) << 10) + ((a >> 13) & 1023u) + ((a >> 12) & 1u);
-  }
-  return (v < 0.0f ? sign : 0u) | h;
+  const uint32_t a = bits & 0x7FFFFFFFu, e = a >> 23;  // |v|
+  const uint32_t m = (a & 0x7FFFFFu) | 0x800000u;
+  const uint32_t s = (e >= 103u && e <= 112u) ? 126u - e : 14u;
+  const uint32_t sub = (m + (1u << (s - 1u))) >> s;
+  const uint32_t normal = ((e - 112u) << 10) + ((a >> 13) & 1023u) + ((a >> 12) & 1u);
+  const uint32_t h = a < 0x33800000u   ? 0u       // |v| < 2^-24, zero included
+                     : a < 0x38800000u ? sub      // |v| < 2^-14
+                     : a < 0x477FE000u ? normal   // |v| < 65504 less half a step
+                     : a <= 0x7F800000u ? 0x7bffu  // to infinity
+                                        : 0u;     // NaN
+  const bool negative = (bits >> 31) != 0u && a != 0u && a <= 0x7F800000u;  // v < 0
+  return (negative ? 0x8000u : 0u) | h;
 }
 
 // The float's bits in an order that compares as the float does.
@@ -169,9 +167,9 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t x) {
 // workgroup into the workgroup's own column of o.partials (reduced over the
 // workgroups afterwards by launch_box_reduce: no global atomics), and
 // centres, Sigma and harmonics staged in LDS so that each goes out as one
-// contiguous run of whole lines. lds holds 256 (10 + W) words; the caller has
-// finished with it.
-template <int DEG>
+// contiguous run of whole lines. The workgroup is G splats; lds holds
+// G (10 + W) words, and the caller has finished with it.
+template <int DEG, uint32_t G = 256>
 __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t cnt, bool valid, const Decoded<DEG> &d,
                                      uint32_t *lds) {
   constexpr uint32_t W = DEG ? sh_words(DEG) : 0;
@@ -189,7 +187,7 @@ __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t c
     hi[k] = wave_max(in_box ? ordered(p[k]) : 0u);
     zi[k] = wave_min(in_box && p[k] == 0.0f ? i0 + t : 0xFFFFFFFFu);
   }
-  uint32_t *centres = lds, *covs = lds + 256 * 4, *sh = lds + 256 * 10;
+  uint32_t *centres = lds, *covs = lds + G * 4, *sh = lds + G * 10;
   if (valid) {
     reinterpret_cast<float4 *>(centres)[t] = d.centre;
 #pragma unroll
@@ -208,18 +206,18 @@ __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t c
     }
   }
   uint32_t *gc = reinterpret_cast<uint32_t *>(o.centers) + (size_t)i0 * 4;
-  for (uint32_t k = t; k < cnt * 4; k += 256) gc[k] = centres[k];
+  for (uint32_t k = t; k < cnt * 4; k += G) gc[k] = centres[k];
   uint32_t *gv = reinterpret_cast<uint32_t *>(o.covs) + (size_t)i0 * 6;
-  for (uint32_t k = t; k < cnt * 6; k += 256) gv[k] = covs[k];
+  for (uint32_t k = t; k < cnt * 6; k += G) gv[k] = covs[k];
   if constexpr (DEG > 0) {
     uint32_t *gs = o.sh + (size_t)i0 * W;
-    for (uint32_t k = t; k < cnt * W; k += 256) gs[k] = sh[k];
+    for (uint32_t k = t; k < cnt * W; k += G) gs[k] = sh[k];
   }
   __syncthreads();
   if (t < kBoxWords) o.partials[(size_t)t * gridDim.x + blockIdx.x] = box[t];  // word-major
 }
 
-constexpr uint32_t emit_words(int DEG) { return 256u * (10u + (DEG ? sh_words(DEG) : 0u)); }
+constexpr uint32_t emit_words(int DEG, uint32_t G = 256) { return G * (10u + (DEG ? sh_words(DEG) : 0u)); }
 
 // One .ply property of a vertex, as a JavaScript number.
 __device__ __forceinline__ double ply_get(const uint8_t *base, uint32_t off, uint32_t type) {
@@ -236,21 +234,33 @@ __device__ __forceinline__ double ply_get(const uint8_t *base, uint32_t off, uin
   }
 }
 
-// parsePly's vertex loop (src/scene/splats.js:457-467), a workgroup 256
+// parsePly's vertex loop (src/scene/splats.js:457-467), a workgroup kPlyGroup
 // vertices. STAGED (a stride and offsets of whole words, every property a
 // float, stride <= kPlyStagedStride): the workgroup's vertices are first
 // copied into LDS with 16-byte loads, so the body is read once, in order.
+// Workgroups smaller than the other decoders' take less LDS each, so more of
+// them share a CU and one's loads overlap another's arithmetic.
+constexpr uint32_t kPlyGroup = 128;
 template <int DEG, bool STAGED>
-__global__ __launch_bounds__(256) void k_decode_ply(const uint8_t *__restrict__ body, uint32_t n, PlyLayout L,
-                                                    DecodeOut o) {
+__global__ __launch_bounds__(kPlyGroup) void k_decode_ply(const uint8_t *__restrict__ body, uint32_t n, PlyLayout L,
+                                                          DecodeOut o) {
+  constexpr uint32_t G = kPlyGroup;
   extern __shared__ __align__(16) uint32_t lds[];
-  const uint32_t t = threadIdx.x, i0 = blockIdx.x * 256u, cnt = min(256u, n - i0);
+  const uint32_t t = threadIdx.x, i0 = blockIdx.x * G, cnt = min(G, n - i0);
   const bool valid = t < cnt;
   const uint8_t *b;
   if constexpr (STAGED) {
     const uint4 *src = reinterpret_cast<const uint4 *>(body + (size_t)i0 * L.stride);
     const uint32_t n16 = (cnt * L.stride + 15u) / 16u;  // the body is padded to 16 bytes past its end
-    for (uint32_t k = t; k < n16; k += 256) reinterpret_cast<uint4 *>(lds)[k] = src[k];
+    // every load issued before the first LDS store: a thread's 16-byte pieces all in flight at once
+    constexpr uint32_t kPieces = kPlyStagedStride / 16u;  // G vertices of at most kPlyStagedStride bytes
+    uint4 piece[kPieces];
+    // Past n16 a thread loads its last piece again and stores it past the staged vertices, in
+    // the launch's G kPlyStagedStride bytes of LDS: no branch, so no load waits behind a store.
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j) piece[j] = src[min(t + j * G, n16 - 1u)];
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j) reinterpret_cast<uint4 *>(lds)[t + j * G] = piece[j];
     __syncthreads();
     b = reinterpret_cast<const uint8_t *>(lds) + (size_t)t * L.stride;
   } else {
@@ -279,23 +289,80 @@ __global__ __launch_bounds__(256) void k_decode_ply(const uint8_t *__restrict__ 
     }
   }
   if constexpr (STAGED) __syncthreads();  // the staged vertices are read; the LDS takes the outputs
-  emit(o, i0, cnt, valid, d, lds);
+  emit<DEG, G>(o, i0, cnt, valid, d, lds);
 }
 
+// The words of the .spz's six sections that a workgroup's 256 splats read, staged
+// into LDS one section after another: all of them at most 256 (9 + 1 + 3 + 3 + 4 + 72)
+// bytes, plus a word of misalignment a section.
+constexpr uint32_t kSpzStageWords = 256u * 92u / 4u + 6u;
+constexpr uint32_t kSpzStageBatch = 12;  // loads a thread has in flight
+
 // parseSpz's loop (src/scene/splats.js:257-301), over the sections of a gunzipped .spz.
+// The workgroup first copies its splats' bytes of each section into LDS with
+// coalesced word loads (the data is 4-byte aligned and padded past its end),
+// then each lane reads its own bytes from there.
 template <int DEG>
 __global__ __launch_bounds__(256) void k_decode_spz(const uint8_t *__restrict__ data, uint32_t n, SpzLayout L,
                                                     DecodeOut o) {
   extern __shared__ __align__(16) uint32_t lds[];
   const uint32_t t = threadIdx.x, i0 = blockIdx.x * 256u, cnt = min(256u, n - i0);
   const bool valid = t < cnt;
-  const uint32_t i = i0 + t;
+  // section k: bytes [sec + i0 w, sec + (i0 + cnt) w), from the word holding its first byte
+  const uint64_t sec[6] = {L.positions, L.alphas, L.colours, L.scales, L.rotations, L.sh};
+  const uint32_t width[6] = {L.version == 1 ? 6u : 9u, 1u, 3u, 3u, L.rotation_bytes, L.file_coefficients * 3u};
+  uint64_t word0[6];
+  uint32_t cum[7], lead[6];
+  cum[0] = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const uint64_t b0 = sec[k] + (uint64_t)i0 * width[k], b1 = b0 + (uint64_t)cnt * width[k];
+    word0[k] = b0 >> 2;
+    lead[k] = (uint32_t)(b0 & 3u);
+    cum[k + 1] = cum[k] + (uint32_t)(((b1 + 3u) >> 2) - (b0 >> 2));
+  }
+  const uint32_t total = cum[6];  // <= kSpzStageWords
+  const uint32_t *words = reinterpret_cast<const uint32_t *>(data);
+  // A colour, a scale and a harmonic are each a function of one byte: the
+  // workgroup makes the three tables of 256 once, while its first loads are in
+  // flight, from the same expressions a splat would evaluate.
+  __shared__ double scale_of[256];
+  __shared__ uint16_t half_of[256];
+  __shared__ uint8_t colour_of[256];
+  for (uint32_t base = 0; base < total; base += 256u * kSpzStageBatch) {
+    uint32_t v[kSpzStageBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kSpzStageBatch; ++j) {
+      const uint32_t x = min(base + j * 256u + t, total - 1u);  // past total: a repeat, not stored
+      uint64_t w = word0[0] + x;
+#pragma unroll
+      for (int k = 1; k < 6; ++k)
+        if (x >= cum[k]) w = word0[k] + (x - cum[k]);
+      v[j] = words[w];
+    }
+    if (base == 0) {
+      scale_of[t] = exp(t / 16.0 - 10.0);
+      colour_of[t] = (uint8_t)js_byte(0.5 + kShC0 * ((t / 255.0 - 0.5) / 0.15));
+      half_of[t] = (uint16_t)signed_half_f32(((int)t - 128) / 128.0f);  // (byte - 128) / 128, exact in f32
+    }
+    // each value used here, unconditionally, so that no load sinks into the stores' branches
+#pragma unroll
+    for (uint32_t j = 0; j < kSpzStageBatch; ++j) asm volatile("" : "+v"(v[j]));
+#pragma unroll
+    for (uint32_t j = 0; j < kSpzStageBatch; ++j) {
+      const uint32_t x = base + j * 256u + t;
+      if (x < total) lds[x] = v[j];
+    }
+  }
+  __syncthreads();
+  const uint8_t *staged = reinterpret_cast<const uint8_t *>(lds);
+  auto at = [&](int k) { return staged + 4u * cum[k] + lead[k] + t * width[k]; };
   Decoded<DEG> d{};
   if (valid) {
     double pos[3];
     for (int a = 0; a < 3; ++a) {
       if (L.version == 1) {
-        const uint8_t *p = data + L.positions + (size_t)i * 6 + a * 2;
+        const uint8_t *p = at(0) + a * 2;
         const uint32_t h = p[0] | (p[1] << 8);
         const int e = (h >> 10) & 31;
         const uint32_t m = h & 1023u;
@@ -303,13 +370,13 @@ __global__ __launch_bounds__(256) void k_decode_spz(const uint8_t *__restrict__ 
         pos[a] = e == 31 ? (m ? NAN : sign * INFINITY)
                          : sign * (e == 0 ? (double)m * 0x1p-24 : (1.0 + m / 1024.0) * ldexp(1.0, e - 15));
       } else {
-        const uint8_t *p = data + L.positions + (size_t)i * 9 + a * 3;
+        const uint8_t *p = at(0) + a * 3;
         const int32_t fixed = (int32_t)((uint32_t)(p[0] | (p[1] << 8) | (p[2] << 16)) << 8) >> 8;
         pos[a] = (double)fixed * L.fixed;
       }
     }
     double q[4];
-    const uint8_t *r = data + L.rotations + (size_t)i * L.rotation_bytes;
+    const uint8_t *r = at(4);
     if (L.rotation_bytes == 3) {
       for (int a = 0; a < 3; ++a) q[a] = r[a] / 127.5 - 1.0;
       const double s = 1.0 - (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
@@ -331,20 +398,23 @@ __global__ __launch_bounds__(256) void k_decode_spz(const uint8_t *__restrict__ 
       const double s = 1.0 - sum;
       q[largest] = sqrt(s > 0.0 ? s : 0.0);
     }
-    const uint8_t *c = data + L.colours + (size_t)i * 3;
-    const uint8_t *s = data + L.scales + (size_t)i * 3;
-    auto colour = [&](int k) { return js_byte(0.5 + kShC0 * ((c[k] / 255.0 - 0.5) / 0.15)); };
-    auto scale = [&](int k) { return exp(s[k] / 16.0 - 10.0); };
+    const uint8_t *c = at(2);
+    const uint8_t *s = at(3);
+    auto colour = [&](int k) { return (uint32_t)colour_of[c[k]]; };
+    auto scale = [&](int k) { return scale_of[s[k]]; };
     // RUB to RDF: a position's y and z negate, and so do a rotation's y and z
-    put(d, pos[0], -pos[1], -pos[2], colour(0), colour(1), colour(2), data[L.alphas + i], scale(0), scale(1),
-        scale(2), q[3], q[0], -q[1], -q[2]);
-    const uint8_t *h = data + L.sh + (size_t)i * L.file_coefficients * 3;
+    put(d, pos[0], -pos[1], -pos[2], colour(0), colour(1), colour(2), *at(1), scale(0), scale(1), scale(2), q[3],
+        q[0], -q[1], -q[2]);
+    const uint8_t *h = at(5);
 #pragma unroll
     for (uint32_t k = 0; k < kShCoeffs[DEG]; ++k) {
-      // (byte - 128) / 128, flipped: exact in f32
-      const float flip = ((kSpzShNegative >> k) & 1u) ? -1.0f : 1.0f;
-      put_sh_f32(d, k, (h[k * 3] - 128) / 128.0f * flip, (h[k * 3 + 1] - 128) / 128.0f * flip,
-                 (h[k * 3 + 2] - 128) / 128.0f * flip);
+      // (byte - 128) / 128, flipped: the table's half with its sign turned, but for a zero
+      const uint32_t flip = ((kSpzShNegative >> k) & 1u) ? 0x8000u : 0u;
+#pragma unroll
+      for (uint32_t c3 = 0; c3 < 3; ++c3) {
+        const uint32_t bits = half_of[h[k * 3 + c3]];
+        set_half(d, 3 * k + c3, bits ? bits ^ flip : 0u);
+      }
     }
   }
   emit(o, i0, cnt, valid, d, lds);
@@ -409,7 +479,8 @@ __global__ __launch_bounds__(256) void k_decode_sog(SogLayout L, DecodeOut o) {
 template <typename K>
 static K pick(const K (&forms)[4], uint32_t degree) { return forms[degree <= 3 ? degree : 0]; }
 
-int launch_decode_ply(wd_context *ctx, const uint8_t *body, uint32_t n, const PlyLayout &L, const DecodeOut &o) {
+int launch_decode_ply(wd_context *ctx, const uint8_t *body, uint32_t n, const PlyLayout &L, const DecodeOut &o,
+                      uint32_t *groups) {
   using Kernel = void (*)(const uint8_t *, uint32_t, PlyLayout, DecodeOut);
   static const Kernel kDirect[4] = {k_decode_ply<0, false>, k_decode_ply<1, false>, k_decode_ply<2, false>,
                                     k_decode_ply<3, false>};
@@ -419,37 +490,40 @@ int launch_decode_ply(wd_context *ctx, const uint8_t *body, uint32_t n, const Pl
   bool staged = L.stride % 4 == 0 && L.stride <= kPlyStagedStride;
   for (uint32_t k = 0; k < (uint32_t)kPlyNeeds + L.rest && staged; ++k)
     staged = L.type[k] == kPlyF32 && L.off[k] % 4 == 0;
-  const size_t lds = std::max<size_t>(staged ? 256u * L.stride : 0u, 4u * emit_words(3));
+  const size_t lds = std::max<size_t>(staged ? kPlyGroup * kPlyStagedStride : 0u, 4u * emit_words(3, kPlyGroup));
+  *groups = (n + kPlyGroup - 1) / kPlyGroup;
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKDecode, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(pick(staged ? kStaged : kDirect, degree), dim3((n + 255) / 256), dim3(256), lds, ctx->stream,
+  hipLaunchKernelGGL(pick(staged ? kStaged : kDirect, degree), dim3(*groups), dim3(kPlyGroup), lds, ctx->stream,
                      body, n, L, o);
   WD_HIP(hipGetLastError(), "splats: .ply decode launch");
   if (ctx->prof_end(kKDecode, ev)) return WD_E_HIP;
   return WD_OK;
 }
 
-int launch_decode_spz(wd_context *ctx, const uint8_t *data, uint32_t n, const SpzLayout &L, const DecodeOut &o) {
+int launch_decode_spz(wd_context *ctx, const uint8_t *data, uint32_t n, const SpzLayout &L, const DecodeOut &o,
+                      uint32_t *groups) {
   using Kernel = void (*)(const uint8_t *, uint32_t, SpzLayout, DecodeOut);
   static const Kernel kForms[4] = {k_decode_spz<0>, k_decode_spz<1>, k_decode_spz<2>, k_decode_spz<3>};
   const uint32_t degree = L.coefficients == 15 ? 3 : L.coefficients == 8 ? 2 : L.coefficients == 3 ? 1 : 0;
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKDecode, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(pick(kForms, degree), dim3((n + 255) / 256), dim3(256), 4u * emit_words(3), ctx->stream, data, n,
-                     L, o);
+  static_assert(kSpzStageWords <= emit_words(3), "the staged sections fit the LDS the outputs take");
+  *groups = (n + 255) / 256;
+  hipLaunchKernelGGL(pick(kForms, degree), dim3(*groups), dim3(256), 4u * emit_words(3), ctx->stream, data, n, L, o);
   WD_HIP(hipGetLastError(), "splats: .spz decode launch");
   if (ctx->prof_end(kKDecode, ev)) return WD_E_HIP;
   return WD_OK;
 }
 
-int launch_decode_sog(wd_context *ctx, const SogLayout &L, const DecodeOut &o) {
+int launch_decode_sog(wd_context *ctx, const SogLayout &L, const DecodeOut &o, uint32_t *groups) {
   using Kernel = void (*)(SogLayout, DecodeOut);
   static const Kernel kForms[4] = {k_decode_sog<0>, k_decode_sog<1>, k_decode_sog<2>, k_decode_sog<3>};
   const uint32_t degree = L.per == 15 ? 3 : L.per == 8 ? 2 : L.per == 3 ? 1 : 0;
   hipEvent_t ev = nullptr;
   if (ctx->prof_begin(kKDecode, &ev)) return WD_E_HIP;
-  hipLaunchKernelGGL(pick(kForms, degree), dim3((L.count + 255) / 256), dim3(256), 4u * emit_words(3), ctx->stream,
-                     L, o);
+  *groups = (L.count + 255) / 256;
+  hipLaunchKernelGGL(pick(kForms, degree), dim3(*groups), dim3(256), 4u * emit_words(3), ctx->stream, L, o);
   WD_HIP(hipGetLastError(), "splats: .sog decode launch");
   if (ctx->prof_end(kKDecode, ev)) return WD_E_HIP;
   return WD_OK;

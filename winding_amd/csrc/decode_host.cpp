@@ -56,7 +56,8 @@ uint16_t signed_half_host(double v) {
 }
 
 // The decoded splats' buffers and the launch that fills them; frees all on failure.
-int decode_into(wd_context *ctx, uint64_t n, uint32_t degree, const std::function<int(const wd::DecodeOut &)> &launch,
+int decode_into(wd_context *ctx, uint64_t n, uint32_t degree,
+                const std::function<int(const wd::DecodeOut &, uint32_t *)> &launch,
                 wd_cloud **out, float *min_out, float *max_out) {
   if (n > WD_MAX_SORT)
     return fail(WD_E_CAPACITY, "splats: %llu splats are more than this device holds in one buffer, %u",
@@ -78,14 +79,17 @@ int decode_into(wd_context *ctx, uint64_t n, uint32_t degree, const std::functio
   if (e == hipSuccess && W) e = hipMalloc(&c->sh, (n * W + 1) * 4);
   if (e == hipSuccess && W) e = hipMemsetAsync(c->sh, 0, (n * W + 1) * 4, ctx->stream);
   // bad index, ordered min xyz, ordered max xyz, first zero per axis, 2 spare; then a slot a workgroup
-  const uint32_t groups = (uint32_t)((n + 255) / 256);
-  if (e == hipSuccess) e = hipMalloc(&scratch, (12 + (size_t)wd::kBoxWords * groups) * sizeof(uint32_t));
+  // a box a workgroup: every decoder's workgroups are 64 splats or more
+  const uint64_t most_groups = (n + 63) / 64;
+  if (e == hipSuccess) e = hipMalloc(&scratch, (12 + (size_t)wd::kBoxWords * most_groups) * sizeof(uint32_t));
   if (e != hipSuccess) {
     undo();
     return wd::hip_fail(e, "splats: decode buffers");
   }
   wd::DecodeOut o{c->centers, c->covs, c->sh, W, scratch, scratch + 1, scratch + 7, scratch + 12};
-  int rc = launch(o);
+  uint32_t groups = 0;
+  int rc = launch(o, &groups);
+  if (rc == WD_OK && (groups == 0 || groups > most_groups)) rc = fail(WD_E_STATE, "splats: a decode of %u workgroups", groups);
   if (rc == WD_OK) rc = wd::launch_box_reduce(ctx, o.partials, groups, o.box, o.box + 3, o.zero, o.bad);
   uint32_t res[12];
   if (rc == WD_OK) {
@@ -269,7 +273,9 @@ int wd_decode_ply(wd_context *ctx, const void *bytes, uint64_t len, wd_cloud **o
   WD_HIP(body.from(data + end, n * stride), "splats: .ply upload");
   return decode_into(
       ctx, n, (uint32_t)degree,
-      [&](const wd::DecodeOut &o) { return wd::launch_decode_ply(ctx, (const uint8_t *)body.p, (uint32_t)n, L, o); },
+      [&](const wd::DecodeOut &o, uint32_t *groups) {
+        return wd::launch_decode_ply(ctx, (const uint8_t *)body.p, (uint32_t)n, L, o, groups);
+      },
       out, min_out, max_out);
 }
 
@@ -323,7 +329,10 @@ int wd_decode_spz(wd_context *ctx, const void *bytes, uint64_t len, wd_cloud **o
   WD_HIP(dev.from(data, at), "splats: .spz upload");
   return decode_into(
       ctx, count, degree,
-      [&](const wd::DecodeOut &o) { return wd::launch_decode_spz(ctx, (const uint8_t *)dev.p, count, L, o); }, out,
+      [&](const wd::DecodeOut &o, uint32_t *groups) {
+        return wd::launch_decode_spz(ctx, (const uint8_t *)dev.p, count, L, o, groups);
+      },
+      out,
       min_out, max_out);
 }
 
@@ -389,7 +398,8 @@ int wd_decode_sog(wd_context *ctx, const wd_sog *sog, wd_cloud **out, float min_
   L.scale_exp = (const double *)tabs[1].p;
   L.sh_half = (const uint16_t *)tabs[2].p;
   return decode_into(
-      ctx, count, degree, [&](const wd::DecodeOut &o) { return wd::launch_decode_sog(ctx, L, o); }, out, min_out,
+      ctx, count, degree, [&](const wd::DecodeOut &o, uint32_t *groups) { return wd::launch_decode_sog(ctx, L, o, groups); },
+      out, min_out,
       max_out);
 }
 

@@ -438,9 +438,13 @@ struct SogLayout {
   const double *scale_exp;
   const uint16_t *sh_half;
 };
-int launch_decode_ply(wd_context *ctx, const uint8_t *body, uint32_t n, const PlyLayout &L, const DecodeOut &o);
-int launch_decode_spz(wd_context *ctx, const uint8_t *data, uint32_t n, const SpzLayout &L, const DecodeOut &o);
-int launch_decode_sog(wd_context *ctx, const SogLayout &L, const DecodeOut &o);
+// Each writes the workgroups it launched (at most n / 64 rounded up), whose
+// boxes launch_box_reduce then folds.
+int launch_decode_ply(wd_context *ctx, const uint8_t *body, uint32_t n, const PlyLayout &L, const DecodeOut &o,
+                      uint32_t *groups);
+int launch_decode_spz(wd_context *ctx, const uint8_t *data, uint32_t n, const SpzLayout &L, const DecodeOut &o,
+                      uint32_t *groups);
+int launch_decode_sog(wd_context *ctx, const SogLayout &L, const DecodeOut &o, uint32_t *groups);
 
 // The particle sort over slots [first, first + n) of a pool (view6: eye, forward in FP64;
 // edges: the 4096 FP64 bucket edges on the device).
