@@ -405,6 +405,33 @@ def test_gpu_sog_matches_oracle_and_the_reference_fixture(ctx):
 
 
 @pytest.mark.gpu
+def test_gpu_box_and_first_bad_over_many_workgroups(ctx):
+    """600,001 vertices: 4,688 workgroups' boxes, more than one pass of the box reduce (4,096 boxes),
+    equal to the centres' min and max; then non-finite positions in two late vertices, in different
+    workgroups, of which the first is the one reported."""
+    n = 600001
+    rng = np.random.default_rng(11)
+    v = np.zeros((n, len(NEEDS)), np.float32)
+    v[:, :3] = rng.normal(0, 3, (n, 3))
+    v[:, 3:7] = rng.normal(0, 1, (n, 4))
+    v[:, 7:10] = rng.normal(-3, 1, (n, 3))
+    v[:, 10:14] = rng.normal(0, 1, (n, 4))
+    head = "ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % n
+    head = (head + "".join("property float %s\n" % p for p in NEEDS) + "end_header\n").encode()
+    cloud = wa.decode_ply(ctx, head + v.tobytes())
+    try:
+        assert cloud.count == n
+        assert np.array_equal(np.asarray(cloud.min, np.float32), v[:, :3].min(0))
+        assert np.array_equal(np.asarray(cloud.max, np.float32), v[:, :3].max(0))
+    finally:
+        cloud.destroy()
+    v[555555, 1] = np.nan
+    v[590000, 0] = np.inf
+    with pytest.raises(wa.WindingError, match="splat 555555 has a position that is not a number"):
+        wa.decode_ply(ctx, head + v.tobytes())
+
+
+@pytest.mark.gpu
 def test_gpu_decoder_refusals(ctx):
     with pytest.raises(wa.WindingError, match="no end_header"):
         wa.decode_ply(ctx, b"ply\nformat binary_little_endian 1.0\n")
