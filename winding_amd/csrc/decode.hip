@@ -86,6 +86,19 @@ __device__ __forceinline__ uint32_t signed_half_f32(float v) {
   return (negative ? 0x8000u : 0u) | h;
 }
 
+// signed_half_f32 for a wave: a value that is zero or in a normal half's range
+// (2^-14 <= |v| < 65504 less half a step) takes the short form, the top 10
+// mantissa bits and the exponent rebased in one add, plus the first dropped bit;
+// the branch-by-branch form runs only for a value that some lane of the wave
+// holds outside that range (equal to it on every value the short form takes).
+__device__ __forceinline__ uint32_t signed_half_f32_wave(float v) {
+  const uint32_t bits = __float_as_uint(v), a = bits & 0x7FFFFFFFu;
+  const bool usual = a == 0u || a - 0x38800000u < 0x477FE000u - 0x38800000u;
+  uint32_t h = a == 0u ? 0u : ((a >> 13) + ((a >> 12) & 1u) - 0x1C000u) | ((bits >> 16) & 0x8000u);
+  if (__builtin_expect(!__all(usual), 0)) h = usual ? h : signed_half_f32(v);
+  return h;
+}
+
 // The float's bits in an order that compares as the float does.
 __device__ __forceinline__ uint32_t ordered(float x) {
   const uint32_t b = __float_as_uint(x);
@@ -146,9 +159,9 @@ __device__ __forceinline__ void put_sh(Decoded<DEG> &d, uint32_t k, double r, do
 // The same for values that are f32s (a .ply's float properties, a .spz's byte fractions).
 template <int DEG>
 __device__ __forceinline__ void put_sh_f32(Decoded<DEG> &d, uint32_t k, float r, float g, float b) {
-  set_half(d, 3 * k, signed_half_f32(r));
-  set_half(d, 3 * k + 1, signed_half_f32(g));
-  set_half(d, 3 * k + 2, signed_half_f32(b));
+  set_half(d, 3 * k, signed_half_f32_wave(r));
+  set_half(d, 3 * k + 1, signed_half_f32_wave(g));
+  set_half(d, 3 * k + 2, signed_half_f32_wave(b));
 }
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
