@@ -309,7 +309,6 @@ __global__ __launch_bounds__(kPlyGroup) void k_decode_ply(const uint8_t *__restr
 // into LDS one section after another: all of them at most 256 (9 + 1 + 3 + 3 + 4 + 72)
 // bytes, plus a word of misalignment a section.
 constexpr uint32_t kSpzStageWords = 256u * 92u / 4u + 6u;
-constexpr uint32_t kSpzStageBatch = 12;  // loads a thread has in flight
 
 // parseSpz's loop (src/scene/splats.js:257-301), over the sections of a gunzipped .spz.
 // The workgroup first copies its splats' bytes of each section into LDS with
@@ -334,39 +333,49 @@ __global__ __launch_bounds__(256) void k_decode_spz(const uint8_t *__restrict__ 
     lead[k] = (uint32_t)(b0 & 3u);
     cum[k + 1] = cum[k] + (uint32_t)(((b1 + 3u) >> 2) - (b0 >> 2));
   }
-  const uint32_t total = cum[6];  // <= kSpzStageWords
   const uint32_t *words = reinterpret_cast<const uint32_t *>(data);
   // A colour, a scale and a harmonic are each a function of one byte: the
-  // workgroup makes the three tables of 256 once, while its first loads are in
+  // workgroup makes the three tables of 256 once, while its loads are in
   // flight, from the same expressions a splat would evaluate.
   __shared__ double scale_of[256];
   __shared__ uint16_t half_of[256];
   __shared__ uint8_t colour_of[256];
-  for (uint32_t base = 0; base < total; base += 256u * kSpzStageBatch) {
-    uint32_t v[kSpzStageBatch];
+  // Section by section, from a base the whole workgroup shares: a section of
+  // w bytes a splat is at most 64 w + 1 words (3, 1, 1, 1, 2 and 19 loads a
+  // thread for 9, 1, 3, 3, 4 and 72 bytes). Every load is issued before the
+  // first store; past a section's words a thread loads its last word again and
+  // does not store it.
+  uint32_t vp[3], va[1], vc[1], vs[1], vr[2], vh[19];
+  auto load = [&](auto &v, int k) {
+    const uint32_t *src = words + word0[k];
+    const uint32_t nw = cum[k + 1] - cum[k];
+    constexpr uint32_t M = sizeof(v) / sizeof(v[0]);
 #pragma unroll
-    for (uint32_t j = 0; j < kSpzStageBatch; ++j) {
-      const uint32_t x = min(base + j * 256u + t, total - 1u);  // past total: a repeat, not stored
-      uint64_t w = word0[0] + x;
+    for (uint32_t q = 0; q < M; ++q)
+      if (M < 4 || q * 256u < nw) v[q] = src[min(t + q * 256u, nw - 1u)];  // the test is the workgroup's
+  };
+  auto store = [&](auto &v, int k) {
+    const uint32_t nw = cum[k + 1] - cum[k];
+    constexpr uint32_t M = sizeof(v) / sizeof(v[0]);
 #pragma unroll
-      for (int k = 1; k < 6; ++k)
-        if (x >= cum[k]) w = word0[k] + (x - cum[k]);
-      v[j] = words[w];
-    }
-    if (base == 0) {
-      scale_of[t] = exp(t / 16.0 - 10.0);
-      colour_of[t] = (uint8_t)js_byte(0.5 + kShC0 * ((t / 255.0 - 0.5) / 0.15));
-      half_of[t] = (uint16_t)signed_half_f32(((int)t - 128) / 128.0f);  // (byte - 128) / 128, exact in f32
-    }
-    // each value used here, unconditionally, so that no load sinks into the stores' branches
-#pragma unroll
-    for (uint32_t j = 0; j < kSpzStageBatch; ++j) asm volatile("" : "+v"(v[j]));
-#pragma unroll
-    for (uint32_t j = 0; j < kSpzStageBatch; ++j) {
-      const uint32_t x = base + j * 256u + t;
-      if (x < total) lds[x] = v[j];
-    }
-  }
+    for (uint32_t q = 0; q < M; ++q)
+      if (t + q * 256u < nw) lds[cum[k] + t + q * 256u] = v[q];
+  };
+  load(vp, 0);
+  load(va, 1);
+  load(vc, 2);
+  load(vs, 3);
+  load(vr, 4);
+  load(vh, 5);
+  scale_of[t] = exp(t / 16.0 - 10.0);
+  colour_of[t] = (uint8_t)js_byte(0.5 + kShC0 * ((t / 255.0 - 0.5) / 0.15));
+  half_of[t] = (uint16_t)signed_half_f32(((int)t - 128) / 128.0f);  // (byte - 128) / 128, exact in f32
+  store(vp, 0);
+  store(va, 1);
+  store(vc, 2);
+  store(vs, 3);
+  store(vr, 4);
+  store(vh, 5);
   __syncthreads();
   const uint8_t *staged = reinterpret_cast<const uint8_t *>(lds);
   auto at = [&](int k) { return staged + 4u * cum[k] + lead[k] + t * width[k]; };
