@@ -218,14 +218,16 @@ __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t c
       atomicMin(&box[6 + k], zi[k]);
     }
   }
-  uint32_t *gc = reinterpret_cast<uint32_t *>(o.centers) + (size_t)i0 * 4;
-  for (uint32_t k = t; k < cnt * 4; k += G) gc[k] = centres[k];
-  uint32_t *gv = reinterpret_cast<uint32_t *>(o.covs) + (size_t)i0 * 6;
-  for (uint32_t k = t; k < cnt * 6; k += G) gv[k] = covs[k];
-  if constexpr (DEG > 0) {
-    uint32_t *gs = o.sh + (size_t)i0 * W;
-    for (uint32_t k = t; k < cnt * W; k += G) gs[k] = sh[k];
-  }
+  // Each region leaves in 16-byte pieces, then its last words: the LDS regions start at
+  // multiples of 16 bytes, and so do the outputs (i0 is a multiple of 128 splats).
+  auto copy_out = [&](uint32_t *dst, const uint32_t *src, uint32_t words) {
+    const uint32_t n4 = words >> 2;
+    for (uint32_t k = t; k < n4; k += G) reinterpret_cast<uint4 *>(dst)[k] = reinterpret_cast<const uint4 *>(src)[k];
+    for (uint32_t k = 4u * n4 + t; k < words; k += G) dst[k] = src[k];
+  };
+  copy_out(reinterpret_cast<uint32_t *>(o.centers) + (size_t)i0 * 4, centres, cnt * 4);
+  copy_out(reinterpret_cast<uint32_t *>(o.covs) + (size_t)i0 * 6, covs, cnt * 6);
+  if constexpr (DEG > 0) copy_out(o.sh + (size_t)i0 * W, sh, cnt * W);
   __syncthreads();
   if (t < kBoxWords) o.partials[(size_t)t * gridDim.x + blockIdx.x] = box[t];  // word-major
 }
