@@ -255,7 +255,7 @@ __device__ __forceinline__ double ply_get(const uint8_t *base, uint32_t off, uin
 // copied into LDS with 16-byte loads, so the body is read once, in order.
 // A workgroup is one wave: the LDS still holds ten of them a CU, and no wave
 // waits at a barrier for another (88 µs for a degree-3 .ply of 1M vertices,
-// against 98 at 128 vertices a workgroup and 105 at 256).
+// against 98 at 128 vertices a workgroup).
 constexpr uint32_t kPlyGroup = 64;
 template <int DEG, bool STAGED>
 __global__ __launch_bounds__(kPlyGroup) void k_decode_ply(const uint8_t *__restrict__ body, uint32_t n, PlyLayout L,
