@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdint>
 
+#include "splat_device.h"
 #include "wd_internal.h"
 
 namespace wd {
@@ -229,7 +230,9 @@ __device__ __forceinline__ void emit(const DecodeOut &o, uint32_t i0, uint32_t c
   copy_out(reinterpret_cast<uint32_t *>(o.covs) + (size_t)i0 * 6, covs, cnt * 6);
   if constexpr (DEG > 0) copy_out(o.sh + (size_t)i0 * W, sh, cnt * W);
   __syncthreads();
-  if (t < kBoxWords) o.partials[(size_t)t * gridDim.x + blockIdx.x] = box[t];  // word-major
+  // word-major, and a column per workgroup in XCD-contiguous order, so that the
+  // neighbouring words of a line come from workgroups that share an L2
+  if (t < kBoxWords) o.partials[(size_t)t * gridDim.x + xcd_item(blockIdx.x, gridDim.x)] = box[t];
 }
 
 constexpr uint32_t emit_words(int DEG, uint32_t G = 256) { return G * (10u + (DEG ? sh_words(DEG) : 0u)); }

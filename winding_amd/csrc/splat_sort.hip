@@ -38,8 +38,10 @@ __device__ __forceinline__ void box_wave(uint32_t (&v)[kBoxWords]) {
 }
 
 // A workgroup's share of the cloud, its box folded in the waves and then
-// across them, written to its own column of partials (word k of workgroup g
-// at k * gridDim.x + g). Each thread has four records' loads in flight.
+// across them, written to its own column of partials (word k of column g at
+// k * gridDim.x + g, the columns in XCD-contiguous order, so that neighbouring
+// words of a line come from workgroups that share an L2). Each thread has four
+// records' loads in flight.
 __global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, uint32_t n, uint32_t *partials) {
   constexpr uint32_t kIn = 4;
   __shared__ uint32_t waves[4][kBoxWords];
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(256) void k_bounds(const uint4 *__restrict__ rec, u
     uint32_t r = waves[0][t];
 #pragma unroll
     for (uint32_t q = 1; q < 4; ++q) r = box_fold(t, r, waves[q][t]);
-    partials[(size_t)t * gridDim.x + blockIdx.x] = r;
+    partials[(size_t)t * gridDim.x + xcd_item(blockIdx.x, gridDim.x)] = r;  // XCD-contiguous columns
   }
 }
 
