@@ -78,8 +78,8 @@ int decode_into(wd_context *ctx, uint64_t n, uint32_t degree,
   if (e == hipSuccess) e = hipMalloc(&c->covs, n * 24);
   if (e == hipSuccess && W) e = hipMalloc(&c->sh, (n * W + 1) * 4);
   if (e == hipSuccess && W) e = hipMemsetAsync(c->sh, 0, (n * W + 1) * 4, ctx->stream);
-  // bad index, ordered min xyz, ordered max xyz, first zero per axis, 2 spare; then a slot a workgroup
-  // a box a workgroup: every decoder's workgroups are 64 splats or more
+  // bad index, ordered min xyz, ordered max xyz, first zero per axis, 2 spare; then a box column a
+  // workgroup (every decoder's workgroups are 64 splats or more)
   const uint64_t most_groups = (n + 63) / 64;
   if (e == hipSuccess) e = hipMalloc(&scratch, (12 + (size_t)wd::kBoxWords * most_groups) * sizeof(uint32_t));
   if (e != hipSuccess) {
