@@ -333,8 +333,8 @@ def check_cloud(cloud, want):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("degree, types", [(0, None), (3, None), (1, {"x": "double", "opacity": "double",
-                                                                      "f_dc_0": "short", "rot_0": "int"})])
+@pytest.mark.parametrize("degree, types", [(0, None), (1, None), (2, None), (3, None),
+                                           (1, {"x": "double", "opacity": "double", "f_dc_0": "short", "rot_0": "int"})])
 def test_gpu_ply_matches_oracle(ctx, degree, types):
     data = random_ply(3000, degree, seed=degree, types=types)
     want = oracle.decode_ply(data)
