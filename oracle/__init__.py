@@ -1,8 +1,9 @@
 """ctypes binding of the CPU oracle (oracle/splat_oracle.c).
 
 TEST INFRASTRUCTURE ONLY. The parity tests, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py load it as the checker; nothing in the product
-(winding_amd/) imports it. Build: python -m winding_amd.build.
+CPU-baseline leg of bench.py load it as the answer the GPU's results are
+compared with; nothing in the product (winding_amd/) imports it. Build:
+python -m winding_amd.build.
 """
 import ctypes as C
 import os

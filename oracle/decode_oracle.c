@@ -1,7 +1,7 @@
 /*
  * decode_oracle.c -- CPU restatement of Winding's capture decoders, in FP64
  * as the reference runs them in JavaScript numbers. TEST INFRASTRUCTURE ONLY:
- * the parity tests load it as the checker; the product never links it.
+ * the parity tests compare the GPU's decodes with it; the product never links it.
  *
  * Functions and what each follows (Winding v1.9.0, src/scene/splats.js):
  *   od_put          put(): centre, colour word, covariance from the

@@ -1,7 +1,8 @@
 /*
  * splat_oracle.c -- CPU restatement of Winding's splat projection, key and
  * stable counting sort, in FP64. TEST INFRASTRUCTURE ONLY: the parity tests,
- * the smoke check and the benchmark's CPU-baseline leg load it as the judge.
+ * the smoke check and the benchmark's CPU-baseline leg load it as the answer
+ * the GPU's results are compared with.
  * The product (libwinding_amd.so) never links, calls or falls back to it.
  *
  * Canonical definition (the reference's formulas, decided in FP64):
