@@ -63,6 +63,37 @@ def test_batch_uneven_jobs_and_replays(ctx, schedule, super_tiles, kpt):
             state.destroy()
 
 
+@pytest.mark.parametrize("schedule,super_tiles,kpt", [(wa.WD_SCHEDULE_PHASED, 2, 8), (wa.WD_SCHEDULE_PERSISTENT, 2, 16)])
+def test_batch_job_with_no_survivors(ctx, schedule, super_tiles, kpt):
+    """A job whose view culls every splat (all behind the camera) between two that keep most: its count
+    is 0 and its passes after the projection have no tiles; the jobs beside it still match the oracle."""
+    jobs, data = [], []
+    for j, n in enumerate([70001, 300000, 12289]):
+        recs = wa.synth_records(j % 2, 300 + j, n)
+        view = wl.view_for(recs, (0.0, 1.0, 10.0), 1280, 720)
+        if j == 1:
+            view.model_view[14] += 1000.0  # everything behind the camera
+        jobs.append((wa.Cloud(ctx, recs), view, wa.State(ctx, n)))
+        data.append(recs)
+    batch = wa.Batch(ctx, len(jobs))
+    try:
+        batch.configure(schedule, super_tiles, kpt)
+        batch.set(jobs)
+        for replay in range(2):
+            batch.run()
+            ctx.sync()
+            assert batch.stuck() == 0
+            for j, ((cloud, view, state), recs) in enumerate(zip(jobs, data)):
+                check_job(state, recs, view, "replay %d job %d" % (replay, j))
+            assert jobs[1][2].stats().count == 0
+            print("replay %d ok" % replay, flush=True)
+    finally:
+        batch.destroy()
+        for cloud, _, state in jobs:
+            cloud.destroy()
+            state.destroy()
+
+
 def test_batch_64_scenes_32m(ctx):
     seed, sizes = wl.batch_sizes()
     st = wl.Stream(seed)
