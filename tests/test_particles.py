@@ -130,6 +130,31 @@ def test_gpu_particle_order_is_bit_exact(ctx, n):
 
 
 @pytest.mark.gpu
+def test_gpu_pool_with_no_live_particles(ctx):
+    """Every particle's age has reached its lifetime: nothing is drawn, the count is 0, and the same state
+    then sorts a live pool exactly."""
+    pool = wa.workloads.particle_pool(5000, seed=3)
+    words = pool.view(np.float32).reshape(len(pool), -1)
+    dead = pool.copy()
+    dead.view(np.float32).reshape(len(pool), -1)[:, 3] = words[:, 7]  # age = lifetime
+    want_dead, _, keep = oracle.particle_run(dead, EYE, FWD)
+    assert len(want_dead) == 0 and not keep.any()
+    st = wa.State(ctx, 5000)
+    try:
+        for data, want in ((dead, want_dead), (pool, oracle.particle_run(pool, EYE, FWD)[0])):
+            p = wa.Pool(ctx, data)
+            try:
+                st.sort_particles(p, wa.particle_view(EYE, FWD))
+                order, _, stats = st.read()
+                assert stats.count == len(want)
+                assert np.array_equal(order, want)
+            finally:
+                p.destroy()
+    finally:
+        st.destroy()
+
+
+@pytest.mark.gpu
 def test_gpu_emitter_rings_of_one_pool(ctx):
     # two emitters' rings back to back in one pool: each sort gives its ring's slots, far to near
     pool = wa.workloads.particle_pool(50000, seed=9)
