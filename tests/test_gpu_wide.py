@@ -47,3 +47,51 @@ def test_sort_past_2_24_splats_matches_oracle(ctx):
     finally:
         cloud.destroy()
         st.destroy()
+
+
+def test_sort_at_the_size_limit(ctx):
+    """WD_MAX_SORT splats (2^28): a 2^24-splat cloud tiled 16 times, so that every copy has the same
+    survivors, keys and cov2d, and the stable order of the whole follows from the copy's: within a key,
+    copy 0's indices, then copy 1's, and so on. The copy's own sort is held to the oracle above."""
+    n_small, copies = 1 << 24, 16
+    n = n_small * copies
+    assert n == wa.WD_MAX_SORT
+    small = wa.synth_records(0, 99, n_small)
+    view = wl.view_for(small, (0.5, 1.5, 10.0), 1920, 1080)
+    st = wa.State(ctx, n)
+    try:
+        cloud = wa.Cloud(ctx, small)
+        try:
+            st.sort(cloud, view)
+            s_order, s_cov, s_stats = st.read(n_cov=n_small)
+            s_keys, s_keep = st.read_keys(n_small)
+            s_box = np.concatenate([cloud.min, cloud.max]).view(np.uint32).copy()
+        finally:
+            cloud.destroy()
+        big = np.tile(small, (copies, 1))
+        cloud = wa.Cloud(ctx, big)
+        del big
+        try:
+            assert np.array_equal(np.concatenate([cloud.min, cloud.max]).view(np.uint32), s_box)  # copies keep the box
+            st.sort(cloud, view)
+            order, _, stats = st.read()
+            print("n=%d V=%d (copy V=%d)" % (n, stats.count, s_stats.count), flush=True)
+            assert stats.count == copies * s_stats.count
+            # the order the copies' stable sort implies: a key's group of the copy's order, once a copy
+            k_sorted = s_keys[s_order].astype(np.int64)
+            counts = np.bincount(k_sorted, minlength=65536)
+            starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+            p = np.arange(len(s_order), dtype=np.int64)
+            s, L = starts[k_sorted], counts[k_sorted]
+            want = np.empty(copies * len(s_order), dtype=np.uint32)
+            for c in range(copies):
+                want[copies * s + c * L + (p - s)] = s_order + np.uint32(c * n_small)
+            assert np.array_equal(order, want), "order differs first at %d" % int(np.argmax(order != want))
+            assert int(order.max()) >= n - n_small  # indices of the last copy made it through
+            # cov2d of the last copy equals the copy's own, bit for bit
+            st_cov = st.read(n_cov=n)[1]
+            assert np.array_equal(st_cov[n - n_small:][s_keep].view(np.uint32), s_cov[s_keep].view(np.uint32))
+        finally:
+            cloud.destroy()
+    finally:
+        st.destroy()
